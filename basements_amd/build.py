@@ -1,0 +1,88 @@
+"""Build recipe for libbm_mpm.so (gfx950) and the C++ host test driver.
+
+``python -m basements_amd.build`` compiles every HIP translation unit under
+``basements_amd/csrc`` with hipcc for gfx950 and links them in-tree into
+``basements_amd/lib/libbm_mpm.so``.  It always recompiles.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libbm_mpm.so")
+HOSTTEST_SRC = os.path.join(ROOT, "tests", "cpp", "host_api_test.cpp")
+HOSTTEST_BIN = os.path.join(LIBDIR, "host_api_test")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# Flags shared by every HIP TU.  Packed-FP32 instruction selection is turned
+# off for the whole library; FP atomics use the hardware instructions
+# (coarse-grained device memory only).
+COMMON = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+# (source, extra flags).  Reference-order kernels round every a*b+c separately.
+UNITS = [
+    ("mpm_ref_kernels.hip", ["-ffp-contract=off"]),
+    ("bm_mpm_api.hip", ["-ffp-contract=off"]),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} ... {cmd[-1]}")
+    return r
+
+
+def build_library(verbose: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    tmpdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(tmpdir, exist_ok=True)
+    for src, extra in UNITS:
+        obj = os.path.join(tmpdir, src.replace(".hip", ".o"))
+        cmd = [HIPCC, *COMMON, *extra, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    shutil.rmtree(tmpdir, ignore_errors=True)
+    return LIB
+
+
+def build_host_test(verbose: bool = False) -> str | None:
+    """The C++ host-API test driver (links libbm_mpm.so through the C ABI only)."""
+    if not os.path.exists(HOSTTEST_SRC):
+        return None
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "host"),
+           HOSTTEST_SRC, "-o", HOSTTEST_BIN, "-L", LIBDIR, "-lbm_mpm", f"-Wl,-rpath,{LIBDIR}",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    return HOSTTEST_BIN
+
+
+def build(verbose: bool = False):
+    lib = build_library(verbose)
+    build_host_test(verbose)
+    return lib
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,616 @@
+// bm_mpm_api.hip -- the C ABI of libbm_mpm.so (declared in include/bm_mpm.h).
+//
+// Host side of the handle: owns every device buffer, queues the substep
+// kernels on its stream, converts the reference's POD structs, and turns HIP
+// failures into status codes plus a last-error string.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpm_internal.hpp"
+
+using namespace bm;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(BM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                 \
+    } while (0)
+
+enum Phase { PH_EMPTY = 0, PH_P2G = 1, PH_UPDATED = 2 };
+
+struct bm_mpm_solver {
+    bm_mpm_desc desc;
+    Domain dom;
+    int64_t nblocks = 0, nnodes = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    float* pbuf = nullptr;  // 26 * cap floats
+    int64_t cap = 0, n = 0;
+
+    GridBuf grid[2] = {};
+    int cur = 0;
+    uint8_t* bever = nullptr;
+    int* d_err = nullptr;
+    unsigned long long* d_scratch_u64 = nullptr;  // 2 words: mass total, max mass bits
+    unsigned long long* d_fallbacks = nullptr;
+    double* d_partials = nullptr;
+    double* d_out7 = nullptr;
+    int max_partials = 0;
+
+    // MPMSolver parameters (mpm_solver.h:100-109 defaults)
+    float E = 1.0e5f, nu = 0.3f, rho = 1500.0f, alpha = 0.40f, mu_floor = 0.4f;
+    float g[3] = {0.0f, -9.81f, 0.0f};
+    int walls = 0;
+
+    int q_exp = 0;
+    float max_mass = 0.0f;
+    int phase = PH_EMPTY;
+
+    PView pview() const { return PView{pbuf, cap, n}; }
+    Phys phys() const {
+        Phys p;
+        p.mu = E / (2.0f * (1.0f + nu));
+        p.lambda = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+        p.alpha = alpha;
+        p.kohesion = (3.0f * p.lambda + 2.0f * p.mu) / (2.0f * p.mu);
+        p.gx = g[0];
+        p.gy = g[1];
+        p.gz = g[2];
+        p.mu_floor = mu_floor;
+        p.walls = walls;
+        return p;
+    }
+    double qscale() const { return std::ldexp(1.0, q_exp); }
+    double qscale_inv() const { return std::ldexp(1.0, -q_exp); }
+};
+
+// Mass quantum: 2^(ilogb(max mass) - 39) kg, so a particle of the largest
+// mass is 2^39..2^40 quanta and any particle within 2^16 of it is an exact
+// integer number of quanta.
+static int quantum_exp_for(float max_mass) {
+    if (!(max_mass > 0.0f) || !std::isfinite(max_mass)) return 0;
+    return std::ilogb(max_mass) - 39;
+}
+
+static int check_handle(bm_mpm_handle h) {
+    if (!h) return fail(BM_ERR_INVALID_ARG, "null handle");
+    return BM_OK;
+}
+
+static int alloc_grid(GridBuf& g, int64_t nnodes, int64_t nblocks, bool with_force) {
+    HIP_TRY(hipMalloc(&g.mq, sizeof(long long) * nnodes));
+    HIP_TRY(hipMalloc(&g.mv, sizeof(float4) * nnodes));
+    if (with_force) HIP_TRY(hipMalloc(&g.f, sizeof(float4) * nnodes));
+    HIP_TRY(hipMalloc(&g.bflag, sizeof(uint32_t) * nblocks));
+    HIP_TRY(hipMalloc(&g.blist, sizeof(int32_t) * nblocks));
+    HIP_TRY(hipMalloc(&g.bcount, sizeof(int32_t)));
+    HIP_TRY(hipMemset(g.mq, 0, sizeof(long long) * nnodes));
+    HIP_TRY(hipMemset(g.mv, 0, sizeof(float4) * nnodes));
+    if (with_force) HIP_TRY(hipMemset(g.f, 0, sizeof(float4) * nnodes));
+    HIP_TRY(hipMemset(g.bflag, 0, sizeof(uint32_t) * nblocks));
+    HIP_TRY(hipMemset(g.bcount, 0, sizeof(int32_t)));
+    return BM_OK;
+}
+
+static void free_grid(GridBuf& g) {
+    hipFree(g.mq);
+    hipFree(g.mv);
+    hipFree(g.f);
+    hipFree(g.bflag);
+    hipFree(g.blist);
+    hipFree(g.bcount);
+    g = GridBuf{};
+}
+
+static int sync_and_check(bm_mpm_handle h) {
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int err = 0;
+    HIP_TRY(hipMemcpy(&err, h->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err & 1) {
+        int zero = 0;
+        HIP_TRY(hipMemcpy(h->d_err, &zero, sizeof(int), hipMemcpyHostToDevice));
+        return fail(BM_ERR_OUT_OF_DOMAIN, "a particle stencil left the grid domain");
+    }
+    return BM_OK;
+}
+
+extern "C" {
+
+int bm_abi_version(void) { return BM_MPM_ABI_VERSION; }
+const char* bm_last_error(void) { return g_last_error.c_str(); }
+
+int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
+    if (!desc || !out) return fail(BM_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (!(desc->dx > 0.0f) || !std::isfinite(desc->dx)) return fail(BM_ERR_INVALID_ARG, "dx must be > 0");
+    if (desc->max_particles < 0) return fail(BM_ERR_INVALID_ARG, "max_particles < 0");
+    for (int a = 0; a < 3; ++a)
+        if (desc->grid_dims[a] <= 0) return fail(BM_ERR_INVALID_ARG, "grid_dims must be > 0");
+    if (desc->mode != BM_MODE_REFERENCE && desc->mode != BM_MODE_FAST)
+        return fail(BM_ERR_INVALID_ARG, "unknown mode %d", desc->mode);
+    if (desc->mode == BM_MODE_FAST) return fail(BM_ERR_NOT_SUPPORTED, "BM_MODE_FAST is not built in this library");
+
+    auto* h = new bm_mpm_solver();
+    h->desc = *desc;
+    Domain& d = h->dom;
+    d.ox = desc->grid_origin[0];
+    d.oy = desc->grid_origin[1];
+    d.oz = desc->grid_origin[2];
+    d.nx = (desc->grid_dims[0] + 3) & ~3;
+    d.ny = (desc->grid_dims[1] + 3) & ~3;
+    d.nz = (desc->grid_dims[2] + 3) & ~3;
+    d.nbx = d.nx / 4;
+    d.nby = d.ny / 4;
+    d.nbz = d.nz / 4;
+    d.dx = desc->dx;
+    d.inv_dx = 1.0f / desc->dx;  // SPGridCPU(dx): inv_dx = 1/dx (spgrid_cpu.h:182)
+    h->nblocks = (int64_t)d.nbx * d.nby * d.nbz;
+    h->nnodes = h->nblocks * 64;
+    if (h->nblocks > (int64_t)1 << 30) {
+        delete h;
+        return fail(BM_ERR_INVALID_ARG, "grid too large");
+    }
+    h->cap = desc->max_particles;
+
+    int rc = BM_OK;
+    auto bail = [&](int code) {
+        bm_mpm_destroy(h);
+        return code;
+    };
+    if (desc->stream) {
+        h->stream = (hipStream_t)desc->stream;
+    } else {
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+            return bail(fail(BM_ERR_HIP, "hipStreamCreate failed"));
+        h->own_stream = true;
+    }
+    if (h->cap > 0 && hipMalloc(&h->pbuf, sizeof(float) * PCOMP * h->cap) != hipSuccess)
+        return bail(fail(BM_ERR_OOM, "particle arrays (%lld particles)", (long long)h->cap));
+    if ((rc = alloc_grid(h->grid[0], h->nnodes, h->nblocks, true)) != BM_OK) return bail(BM_ERR_OOM);
+    if (hipMalloc(&h->bever, h->nblocks) != hipSuccess) return bail(fail(BM_ERR_OOM, "block flags"));
+    hipMemset(h->bever, 0, h->nblocks);
+    h->max_partials = (int)((h->nblocks + 3) / 4) + 1;
+    if (hipMalloc(&h->d_partials, sizeof(double) * 7 * h->max_partials) != hipSuccess)
+        return bail(fail(BM_ERR_OOM, "coupler partials"));
+    if (hipMalloc(&h->d_out7, sizeof(double) * 8) != hipSuccess) return bail(fail(BM_ERR_OOM, "coupler out"));
+    if (hipMalloc(&h->d_err, sizeof(int)) != hipSuccess) return bail(fail(BM_ERR_OOM, "err flag"));
+    if (hipMalloc(&h->d_scratch_u64, sizeof(unsigned long long) * 2) != hipSuccess) return bail(fail(BM_ERR_OOM, "scratch"));
+    if (hipMalloc(&h->d_fallbacks, sizeof(unsigned long long)) != hipSuccess) return bail(fail(BM_ERR_OOM, "counter"));
+    hipMemset(h->d_err, 0, sizeof(int));
+    hipMemset(h->d_fallbacks, 0, sizeof(unsigned long long));
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(BM_ERR_HIP, "device sync after allocation"));
+    *out = h;
+    return BM_OK;
+}
+
+int bm_mpm_destroy(bm_mpm_handle h) {
+    if (!h) return BM_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    hipFree(h->pbuf);
+    free_grid(h->grid[0]);
+    free_grid(h->grid[1]);
+    hipFree(h->bever);
+    hipFree(h->d_partials);
+    hipFree(h->d_out7);
+    hipFree(h->d_err);
+    hipFree(h->d_scratch_u64);
+    hipFree(h->d_fallbacks);
+    if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return BM_OK;
+}
+
+int bm_mpm_synchronize(bm_mpm_handle h) {
+    if (int rc = check_handle(h)) return rc;
+    return sync_and_check(h);
+}
+
+int bm_mpm_set_material(bm_mpm_handle h, float E, float nu, float rho) {
+    if (int rc = check_handle(h)) return rc;
+    h->E = E;
+    h->nu = nu;
+    if (rho > 1e-12f && h->rho > 1e-12f && rho != h->rho && h->n > 0) {
+        const float rescale = h->rho / rho;
+        launch_scale_v0(h->pview(), rescale, h->stream);
+        HIP_TRY(hipGetLastError());
+    }
+    h->rho = rho;
+    return BM_OK;
+}
+int bm_mpm_set_gravity(bm_mpm_handle h, float gx, float gy, float gz) {
+    if (int rc = check_handle(h)) return rc;
+    h->g[0] = gx;
+    h->g[1] = gy;
+    h->g[2] = gz;
+    return BM_OK;
+}
+int bm_mpm_set_plastic_alpha(bm_mpm_handle h, float a) {
+    if (int rc = check_handle(h)) return rc;
+    h->alpha = a;
+    return BM_OK;
+}
+int bm_mpm_set_floor_friction(bm_mpm_handle h, float mu) {
+    if (int rc = check_handle(h)) return rc;
+    h->mu_floor = mu;
+    return BM_OK;
+}
+int bm_mpm_set_walls(bm_mpm_handle h, int32_t thickness) {
+    if (int rc = check_handle(h)) return rc;
+    if (thickness < 0) return fail(BM_ERR_INVALID_ARG, "wall thickness < 0");
+    h->walls = thickness;
+    return BM_OK;
+}
+
+// Copy host SoA (3n/9n interleaved per particle) into component-major device arrays.
+static int upload_range(bm_mpm_handle h, int64_t off, int64_t n, const float* x, const float* v, const float* F,
+                        const float* Cm, const float* m, const float* V0) {
+    std::vector<float> tmp((size_t)n);
+    auto put = [&](int comp, const float* src, int stride, int k) -> int {
+        for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = src[i * stride + k];
+        HIP_TRY(hipMemcpyAsync(h->pbuf + comp * h->cap + off, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice,
+                               h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return BM_OK;
+    };
+    for (int k = 0; k < 3; ++k)
+        if (int rc = put(PX + k, x, 3, k)) return rc;
+    for (int k = 0; k < 3; ++k)
+        if (int rc = put(PV + k, v, 3, k)) return rc;
+    for (int k = 0; k < 9; ++k)
+        if (int rc = put(PF + k, F, 9, k)) return rc;
+    for (int k = 0; k < 9; ++k)
+        if (int rc = put(PC + k, Cm, 9, k)) return rc;
+    if (int rc = put(PM, m, 1, 0)) return rc;
+    if (int rc = put(PV0, V0, 1, 0)) return rc;
+    return BM_OK;
+}
+
+static void refresh_quantum(bm_mpm_handle h) { h->q_exp = quantum_exp_for(h->max_mass); }
+
+int bm_mpm_add_particles(bm_mpm_handle h, int64_t n, const float* pos, const float* vel, const float* mass) {
+    if (int rc = check_handle(h)) return rc;
+    if (n < 0 || (n > 0 && (!pos || !vel || !mass))) return fail(BM_ERR_INVALID_ARG, "bad particle arrays");
+    if (h->n + n > h->cap) return fail(BM_ERR_OOM, "capacity %lld exceeded", (long long)h->cap);
+    if (n == 0) return BM_OK;
+    std::vector<float> F((size_t)n * 9, 0.0f), Cm((size_t)n * 9, 0.0f), V0((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        F[i * 9 + 0] = F[i * 9 + 4] = F[i * 9 + 8] = 1.0f;
+        V0[i] = (h->rho > 1e-12f) ? (mass[i] / h->rho) : 1.0f;
+        h->max_mass = std::max(h->max_mass, mass[i]);
+    }
+    if (int rc = upload_range(h, h->n, n, pos, vel, F.data(), Cm.data(), mass, V0.data())) return rc;
+    h->n += n;
+    refresh_quantum(h);
+    h->phase = PH_EMPTY;
+    return BM_OK;
+}
+
+int bm_mpm_upload_particles(bm_mpm_handle h, int64_t n, const bm_particles* p) {
+    if (int rc = check_handle(h)) return rc;
+    if (n < 0 || !p) return fail(BM_ERR_INVALID_ARG, "bad particle arrays");
+    if (n > h->cap) return fail(BM_ERR_OOM, "capacity %lld exceeded", (long long)h->cap);
+    if (n > 0 && (!p->x || !p->v || !p->F || !p->C || !p->m || !p->V0))
+        return fail(BM_ERR_INVALID_ARG, "null particle field");
+    h->max_mass = 0.0f;
+    for (int64_t i = 0; i < n; ++i) h->max_mass = std::max(h->max_mass, p->m[i]);
+    if (n > 0)
+        if (int rc = upload_range(h, 0, n, p->x, p->v, p->F, p->C, p->m, p->V0)) return rc;
+    h->n = n;
+    refresh_quantum(h);
+    h->phase = PH_EMPTY;
+    return BM_OK;
+}
+
+int bm_mpm_download_particles(bm_mpm_handle h, bm_particles* p) {
+    if (int rc = check_handle(h)) return rc;
+    if (!p) return fail(BM_ERR_INVALID_ARG, "null output");
+    if (int rc = sync_and_check(h)) return rc;
+    const int64_t n = h->n;
+    if (n == 0) return BM_OK;
+    std::vector<float> tmp((size_t)n);
+    auto get = [&](int comp, float* dst, int stride, int k) -> int {
+        if (!dst) return BM_OK;
+        HIP_TRY(hipMemcpy(tmp.data(), h->pbuf + comp * h->cap, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) dst[i * stride + k] = tmp[(size_t)i];
+        return BM_OK;
+    };
+    for (int k = 0; k < 3; ++k)
+        if (int rc = get(PX + k, p->x, 3, k)) return rc;
+    for (int k = 0; k < 3; ++k)
+        if (int rc = get(PV + k, p->v, 3, k)) return rc;
+    for (int k = 0; k < 9; ++k)
+        if (int rc = get(PF + k, p->F, 9, k)) return rc;
+    for (int k = 0; k < 9; ++k)
+        if (int rc = get(PC + k, p->C, 9, k)) return rc;
+    if (int rc = get(PM, p->m, 1, 0)) return rc;
+    if (int rc = get(PV0, p->V0, 1, 0)) return rc;
+    return BM_OK;
+}
+
+int bm_mpm_num_particles(bm_mpm_handle h, int64_t* n) {
+    if (int rc = check_handle(h)) return rc;
+    if (!n) return fail(BM_ERR_INVALID_ARG, "null output");
+    *n = h->n;
+    return BM_OK;
+}
+
+int bm_mpm_particles_device(bm_mpm_handle h, bm_particles_device* out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(BM_ERR_INVALID_ARG, "null output");
+    out->x = h->pbuf + PX * h->cap;
+    out->v = h->pbuf + PV * h->cap;
+    out->F = h->pbuf + PF * h->cap;
+    out->C = h->pbuf + PC * h->cap;
+    out->m = h->pbuf + PM * h->cap;
+    out->V0 = h->pbuf + PV0 * h->cap;
+    out->capacity = h->cap;
+    return BM_OK;
+}
+
+int bm_mpm_commit_device_particles(bm_mpm_handle h, int64_t n) {
+    if (int rc = check_handle(h)) return rc;
+    if (n < 0 || n > h->cap) return fail(BM_ERR_INVALID_ARG, "count %lld outside [0, capacity]", (long long)n);
+    h->n = n;
+    unsigned int bits = 0;
+    HIP_TRY(hipMemsetAsync(h->d_scratch_u64, 0, sizeof(unsigned long long) * 2, h->stream));
+    launch_max_mass(h->pview(), (unsigned int*)h->d_scratch_u64, h->stream);
+    HIP_TRY(hipMemcpyAsync(&bits, h->d_scratch_u64, sizeof(unsigned int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::memcpy(&h->max_mass, &bits, sizeof(float));
+    refresh_quantum(h);
+    h->phase = PH_EMPTY;
+    return BM_OK;
+}
+
+// ------------------------------------------------------------------ substep
+int bm_mpm_p2g(bm_mpm_handle h, float dt) {
+    (void)dt;  // p2g_single ignores dt (mpm_solver.h:168)
+    if (int rc = check_handle(h)) return rc;
+    GridBuf& g = h->grid[h->cur];
+    launch_clear(g, (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream);
+    launch_p2g_ref(h->pview(), g, h->bever, h->dom, h->phys(), h->qscale_inv(), h->d_err, h->stream);
+    HIP_TRY(hipGetLastError());
+    h->phase = PH_P2G;
+    return BM_OK;
+}
+
+int bm_mpm_grid_update(bm_mpm_handle h, float dt) {
+    if (int rc = check_handle(h)) return rc;
+    if (h->phase != PH_P2G) return fail(BM_ERR_STATE, "grid_update needs a preceding p2g");
+    launch_update(h->grid[h->cur], (int)h->nblocks, h->dom, h->phys(), h->qscale(), dt, h->stream);
+    HIP_TRY(hipGetLastError());
+    h->phase = PH_UPDATED;
+    return BM_OK;
+}
+
+int bm_mpm_g2p(bm_mpm_handle h, float dt) {
+    if (int rc = check_handle(h)) return rc;
+    if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "g2p needs an updated grid");
+    launch_g2p_ref(h->pview(), h->grid[h->cur], h->dom, h->phys(), dt, h->d_err, h->stream);
+    HIP_TRY(hipGetLastError());
+    return BM_OK;
+}
+
+int bm_mpm_step(bm_mpm_handle h, float dt) {
+    if (int rc = bm_mpm_p2g(h, dt)) return rc;
+    if (int rc = bm_mpm_grid_update(h, dt)) return rc;
+    return bm_mpm_g2p(h, dt);
+}
+
+int bm_mpm_step_n(bm_mpm_handle h, float dt, int32_t k) {
+    if (k < 0) return fail(BM_ERR_INVALID_ARG, "k < 0");
+    for (int32_t i = 0; i < k; ++i)
+        if (int rc = bm_mpm_step(h, dt)) return rc;
+    return BM_OK;
+}
+
+// ------------------------------------------------------------------ couplers
+static ColliderDev make_collider(bm_mpm_handle h, const bm_collider* b) {
+    ColliderDev c{};
+    const float w = b->quat[0], x = b->quat[1], y = b->quat[2], z = b->quat[3];
+    // Matrix3::from_quaternion, matrix3.h:432-448
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y,
+                wz = w * z;
+    const float R[9] = {1.0f - 2.0f * (yy + zz), 2.0f * (xy - wz),        2.0f * (xz + wy),
+                        2.0f * (xy + wz),        1.0f - 2.0f * (xx + zz), 2.0f * (yz - wx),
+                        2.0f * (xz - wy),        2.0f * (yz + wx),        1.0f - 2.0f * (xx + yy)};
+    for (int i = 0; i < 9; ++i) c.Rl[i] = R[i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) c.Rw[i * 3 + j] = R[j * 3 + i];
+    for (int a = 0; a < 3; ++a) {
+        c.c[a] = b->center[a];
+        c.he[a] = b->half_extents[a];
+        c.vl[a] = b->lin_vel[a];
+        c.wa[a] = b->ang_vel[a];
+    }
+    c.friction = b->friction;
+    c.shape = b->shape;
+    // M1 per-node body mass, mpm_rigid_coupler_pic.h:104-108
+    const float dx = h->dom.dx;
+    const float vol = 8.0f * b->half_extents[0] * b->half_extents[1] * b->half_extents[2];
+    const int approx = std::max(1, (int)std::round(vol / (dx * dx * dx)));
+    c.mb = b->mass / (float)approx;
+    // |R| he world AABB, mpm_rigid_coupler.cpp:70-76
+    for (int a = 0; a < 3; ++a) {
+        const float hw = std::fabs(R[a * 3 + 0]) * c.he[0] + std::fabs(R[a * 3 + 1]) * c.he[1] +
+                         std::fabs(R[a * 3 + 2]) * c.he[2];
+        c.lo[a] = c.c[a] - hw;
+        c.hi[a] = c.c[a] + hw;
+    }
+    // block range: every block whose node-centre box may overlap the AABB,
+    // widened by one block; the kernel applies the reference's exact cull.
+    const int org[3] = {h->dom.ox, h->dom.oy, h->dom.oz};
+    const int nbv[3] = {h->dom.nbx, h->dom.nby, h->dom.nbz};
+    for (int a = 0; a < 3; ++a) {
+        const double lo_n = std::floor((double)c.lo[a] / dx - 0.5) - org[a];
+        const double hi_n = std::ceil((double)c.hi[a] / dx - 0.5) - org[a];
+        long long blo = (long long)std::floor(lo_n / 4.0) - 1;
+        long long bhi = (long long)std::floor(hi_n / 4.0) + 1;
+        blo = std::max<long long>(blo, 0);
+        bhi = std::min<long long>(bhi, nbv[a] - 1);
+        c.b0[a] = (int)blo;
+        c.nb[a] = (bhi >= blo) ? (int)(bhi - blo + 1) : 0;
+    }
+    return c;
+}
+
+int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!body || !out) return fail(BM_ERR_INVALID_ARG, "null argument");
+    if (method != BM_COUPLER_M0_DIRICHLET && method != BM_COUPLER_M1_PIC_RIGID)
+        return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
+    std::memset(out, 0, sizeof(*out));
+    if (!(dt > 0.0f)) return BM_OK;
+    if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "couple needs an updated grid");
+    ColliderDev c = make_collider(h, body);
+    if (couple_partials_needed(c) > h->max_partials) return fail(BM_ERR_INVALID_ARG, "collider range too large");
+    launch_couple(method, c, h->grid[h->cur], h->bever, h->dom, dt, h->d_partials, h->d_out7, h->stream);
+    HIP_TRY(hipGetLastError());
+    double r[7];
+    HIP_TRY(hipMemcpyAsync(r, h->d_out7, sizeof(double) * 7, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int a = 0; a < 3; ++a) {
+        out->force[a] = (float)r[a];
+        out->torque[a] = (float)r[3 + a];
+    }
+    out->nodes_inside = (int32_t)std::llround(r[6]);
+    return BM_OK;
+}
+
+// ------------------------------------------------------------------ grid access
+int bm_mpm_download_grid(bm_mpm_handle h, const int32_t org[3], const int32_t dims[3], int64_t* mass_q, float* mass,
+                         float* vel, float* force, uint8_t* active, uint8_t* alloc) {
+    if (int rc = check_handle(h)) return rc;
+    if (!org || !dims) return fail(BM_ERR_INVALID_ARG, "null box");
+    for (int a = 0; a < 3; ++a)
+        if (dims[a] < 0) return fail(BM_ERR_INVALID_ARG, "negative box");
+    const int64_t tot = (int64_t)dims[0] * dims[1] * dims[2];
+    if (tot == 0) return BM_OK;
+    long long* d_mq = nullptr;
+    float *d_mass = nullptr, *d_vel = nullptr, *d_force = nullptr;
+    uint8_t *d_act = nullptr, *d_alloc = nullptr;
+    int rc = BM_OK;
+    auto cleanup = [&]() {
+        hipFree(d_mq);
+        hipFree(d_mass);
+        hipFree(d_vel);
+        hipFree(d_force);
+        hipFree(d_act);
+        hipFree(d_alloc);
+    };
+    if ((mass_q && hipMalloc(&d_mq, 8 * tot) != hipSuccess) || (mass && hipMalloc(&d_mass, 4 * tot) != hipSuccess) ||
+        (vel && hipMalloc(&d_vel, 12 * tot) != hipSuccess) || (force && hipMalloc(&d_force, 12 * tot) != hipSuccess) ||
+        (active && hipMalloc(&d_act, tot) != hipSuccess) || (alloc && hipMalloc(&d_alloc, tot) != hipSuccess)) {
+        cleanup();
+        return fail(BM_ERR_OOM, "grid download staging");
+    }
+    const GridBuf& g = h->grid[h->cur];
+    launch_grid_gather(g, h->bever, h->dom, org, dims, h->qscale(), h->phase == PH_UPDATED ? 1 : 0, d_mq, d_mass,
+                       d_vel, d_force, d_act, d_alloc, h->stream);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && mass_q) e = hipMemcpy(mass_q, d_mq, 8 * tot, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && mass) e = hipMemcpy(mass, d_mass, 4 * tot, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && vel) e = hipMemcpy(vel, d_vel, 12 * tot, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && force) e = hipMemcpy(force, d_force, 12 * tot, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && active) e = hipMemcpy(active, d_act, tot, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && alloc) e = hipMemcpy(alloc, d_alloc, tot, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) rc = fail(BM_ERR_HIP, "grid download: %s", hipGetErrorString(e));
+    return rc;
+}
+
+int bm_mpm_upload_grid(bm_mpm_handle h, const int32_t org[3], const int32_t dims[3], const float* mass,
+                       const float* vel, const uint8_t* active) {
+    if (int rc = check_handle(h)) return rc;
+    if (!org || !dims || !mass || !vel || !active) return fail(BM_ERR_INVALID_ARG, "null argument");
+    for (int a = 0; a < 3; ++a)
+        if (dims[a] < 0) return fail(BM_ERR_INVALID_ARG, "negative box");
+    const int64_t tot = (int64_t)dims[0] * dims[1] * dims[2];
+    GridBuf& g = h->grid[h->cur];
+    if (h->phase == PH_EMPTY) launch_clear(g, (int)h->nblocks, h->stream);
+    if (tot > 0) {
+        float *d_mass = nullptr, *d_vel = nullptr;
+        uint8_t* d_act = nullptr;
+        if (hipMalloc(&d_mass, 4 * tot) != hipSuccess || hipMalloc(&d_vel, 12 * tot) != hipSuccess ||
+            hipMalloc(&d_act, tot) != hipSuccess) {
+            hipFree(d_mass);
+            hipFree(d_vel);
+            hipFree(d_act);
+            return fail(BM_ERR_OOM, "grid upload staging");
+        }
+        hipError_t e = hipMemcpy(d_mass, mass, 4 * tot, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_vel, vel, 12 * tot, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_act, active, tot, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            launch_grid_scatter(g, h->bever, h->dom, org, dims, h->qscale_inv(), d_mass, d_vel, d_act, h->stream);
+            e = hipStreamSynchronize(h->stream);
+        }
+        hipFree(d_mass);
+        hipFree(d_vel);
+        hipFree(d_act);
+        if (e != hipSuccess) return fail(BM_ERR_HIP, "grid upload: %s", hipGetErrorString(e));
+    }
+    h->phase = PH_UPDATED;
+    return BM_OK;
+}
+
+int bm_mpm_grid_mass_total(bm_mpm_handle h, int64_t* total) {
+    if (int rc = check_handle(h)) return rc;
+    if (!total) return fail(BM_ERR_INVALID_ARG, "null output");
+    unsigned long long v = 0;
+    HIP_TRY(hipMemsetAsync(h->d_scratch_u64, 0, sizeof(unsigned long long), h->stream));
+    launch_mass_total(h->grid[h->cur], h->d_scratch_u64, h->stream);
+    HIP_TRY(hipMemcpyAsync(&v, h->d_scratch_u64, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    if (int rc = sync_and_check(h)) return rc;
+    *total = (int64_t)v;
+    return BM_OK;
+}
+
+int bm_mpm_mass_quantum_exp(bm_mpm_handle h, int32_t* e) {
+    if (int rc = check_handle(h)) return rc;
+    if (!e) return fail(BM_ERR_INVALID_ARG, "null output");
+    *e = h->q_exp;
+    return BM_OK;
+}
+
+int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallbacks) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = sync_and_check(h)) return rc;
+    if (active_blocks) {
+        int c = 0;
+        HIP_TRY(hipMemcpy(&c, h->grid[h->cur].bcount, sizeof(int), hipMemcpyDeviceToHost));
+        *active_blocks = c;
+    }
+    if (fp64_fallbacks) {
+        unsigned long long f = 0;
+        HIP_TRY(hipMemcpy(&f, h->d_fallbacks, sizeof(f), hipMemcpyDeviceToHost));
+        *fp64_fallbacks = (int64_t)f;
+    }
+    return BM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,552 @@
+// mpm_ref_kernels.hip -- grid bookkeeping, the reference-order substep
+// kernels, and the M0 / M1 couplers.
+//
+// Built with -ffp-contract=off so every a*b+c is rounded the way the
+// reference's scalar C++ rounds it (mpm_solver.h, svd.h compiled without FMA).
+// The grid is dense over a fixed domain and block-linear (4^3-node blocks, the
+// reference's block size, spgrid_cpu.h:12-15).  A block list per grid buffer
+// records the blocks P2G touched so that the clear and the grid update visit
+// only those blocks; a persistent per-block "ever touched" flag reproduces the
+// reference's never-freed block set (spgrid_cpu.h:46-65), which decides the
+// node set M1 may claim.
+
+#include "mpm_device.hpp"
+#include "mpm_internal.hpp"
+
+namespace bm {
+
+// --------------------------------------------------------------- bookkeeping
+// Zero every node of the blocks listed in g and drop their flags.  One wave
+// per block; grid-stride over the device-side count.
+__global__ void __launch_bounds__(64) k_clear_blocks(GridBuf g) {
+    const int cnt = *g.bcount;
+    for (int b = blockIdx.x; b < cnt; b += gridDim.x) {
+        const int blk = g.blist[b];
+        const int64_t nid = (int64_t)blk * 64 + threadIdx.x;
+        g.mq[nid] = 0;
+        g.mv[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (g.f) g.f[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (threadIdx.x == 0) g.bflag[blk] = 0u;
+    }
+}
+
+__global__ void k_reset_count(GridBuf g) { *g.bcount = 0; }
+
+// Record that block blk holds data of this grid use.
+__device__ __forceinline__ void touch_block(const GridBuf& g, uint8_t* bever, int blk) {
+    if (g.bflag[blk] == 0u && atomicExch(&g.bflag[blk], 1u) == 0u) {
+        const int slot = atomicAdd(g.bcount, 1);
+        g.blist[slot] = blk;
+    }
+    if (bever && bever[blk] == 0) bever[blk] = 1;
+}
+
+__device__ __forceinline__ bool stencil_in_domain(const Domain& d, const Stencil& s) {
+    const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+    return lx >= 0 && ly >= 0 && lz >= 0 && lx + 2 < d.nx && ly + 2 < d.ny && lz + 2 < d.nz;
+}
+
+__device__ __forceinline__ m3<float> load_m3(const PView& P, int comp, int64_t i) {
+    m3<float> r;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) r.a[c] = P.p[(comp + c) * P.cap + i];
+    return r;
+}
+__device__ __forceinline__ void store_m3(const PView& P, int comp, int64_t i, const m3<float>& r) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) P.p[(comp + c) * P.cap + i] = r.a[c];
+}
+
+// --------------------------------------------------------------- P2G (reference order)
+// mpm_solver.h:168-248 with the fixed-point mass deposit.  One particle per
+// lane; node accumulation with device-scope float / int64 atomics.
+__global__ void __launch_bounds__(256) k_p2g_ref(PView P, GridBuf g, uint8_t* bever, Domain d, Phys ph,
+                                                 double qscale_inv, int* err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
+    Stencil s;
+    make_stencil(x, y, z, d.inv_dx, s);
+    if (!stencil_in_domain(d, s)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const v3<float> vel = mk(P.p[(PV + 0) * P.cap + i], P.p[(PV + 1) * P.cap + i], P.p[(PV + 2) * P.cap + i]);
+    const m3<float> F = load_m3(P, PF, i);
+    const m3<float> Cm = load_m3(P, PC, i);
+    const float mass = P.p[PM * P.cap + i];
+    const float V0 = P.p[PV0 * P.cap + i];
+
+    m3<float> U, V;
+    v3<float> sg;
+    svd_ref(F, U, sg, V);
+    sg.x = fmaxf(sg.x, 1e-6f);
+    sg.y = fmaxf(sg.y, 1e-6f);
+    sg.z = fmaxf(sg.z, 1e-6f);
+    const m3<float> tau = hencky_from_svd(U, sg, ph.mu, ph.lambda);
+    const float ff = -4.0f * d.inv_dx * d.inv_dx * V0;
+    m3<float> K;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) K.a[c] = tau.a[c] * ff;
+
+    // integer quanta of this particle; the stencil centre takes the remainder
+    const double qp = rint((double)mass * qscale_inv);
+    long long rest = (long long)qp;
+#pragma unroll
+    for (int a = 0; a < 27; ++a) {
+        if (a == 13) continue;
+        const float w = s.w[0][a / 9] * s.w[1][(a / 3) % 3] * s.w[2][a % 3];
+        rest -= mass_quanta(w, qp);
+    }
+
+    const int lx0 = s.b[0] - d.ox, ly0 = s.b[1] - d.oy, lz0 = s.b[2] - d.oz;
+    int last_blk = -1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float w = s.w[0][a] * s.w[1][b] * s.w[2][c];
+                if (w == 0.0f) continue;
+                const v3<float> dd = mk(((float)(s.b[0] + a) - s.f[0] - 0.5f) * d.dx,
+                                        ((float)(s.b[1] + b) - s.f[1] - 0.5f) * d.dx,
+                                        ((float)(s.b[2] + c) - s.f[2] - 0.5f) * d.dx);
+                const v3<float> aff = mvec(Cm, dd);
+                const float mc = w * mass;
+                const v3<float> mom = (vel + aff) * mc;
+                const v3<float> fo = mvec(K, dd) * w;
+                const int64_t nid = node_of(d, lx0 + a, ly0 + b, lz0 + c);
+                const long long dq = (a == 1 && b == 1 && c == 1) ? rest : mass_quanta(w, qp);
+                atomicAdd((unsigned long long*)&g.mq[nid], (unsigned long long)dq);
+                float* nm = (float*)&g.mv[nid];
+                atomicAdd(nm + 0, mom.x);
+                atomicAdd(nm + 1, mom.y);
+                atomicAdd(nm + 2, mom.z);
+                float* nf = (float*)&g.f[nid];
+                atomicAdd(nf + 0, fo.x);
+                atomicAdd(nf + 1, fo.y);
+                atomicAdd(nf + 2, fo.z);
+                const int blk = (int)(nid >> 6);
+                if (blk != last_blk) {
+                    touch_block(g, bever, blk);
+                    last_blk = blk;
+                }
+            }
+}
+
+// --------------------------------------------------------------- grid update
+// mpm_solver.h:250-291 plus the optional domain walls.  force_on selects
+// whether a separate force buffer is folded in (reference mode) or the force
+// was already folded into the momentum by P2G (fast mode).
+__global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d, Phys ph, double qscale, float dt) {
+    const int cnt = *g.bcount;
+    for (int b = blockIdx.x; b < cnt; b += gridDim.x) {
+        const int blk = g.blist[b];
+        const int64_t nid = (int64_t)blk * 64 + threadIdx.x;
+        const long long mq = g.mq[nid];
+        float4 q = g.mv[nid];
+        if (mq == 0) continue;  // inactive node (no particle weight)
+        const float m = (float)((double)mq * qscale);
+        q.w = m;
+        if (m > 1e-6f) {
+            v3<float> mom = mk(q.x, q.y, q.z);
+            if (g.f) {
+                const float4 fo = g.f[nid];
+                mom = mom + mk(fo.x, fo.y, fo.z) * dt;
+            }
+            v3<float> v = divg(mom, m);
+            v = v + mk(ph.gx * dt, ph.gy * dt, ph.gz * dt);
+            const int bz = blk % d.nbz;
+            const int by = (blk / d.nbz) % d.nby;
+            const int bx = blk / (d.nbz * d.nby);
+            const int t = threadIdx.x;
+            const int gj = d.oy + by * 4 + ((t >> 2) & 3);
+            const float node_y = ((float)gj + 0.5f) * d.dx;
+            if (node_y < d.dx && v.y < 0.0f) {
+                const float vn = -v.y;
+                v.y = 0.0f;
+                const float tx = v.x, tz = v.z;
+                const float tm = sqrtf(tx * tx + tz * tz);
+                const float budget = ph.mu_floor * vn;
+                if (tm > 1e-12f) {
+                    const float sc = fmaxf(0.0f, (tm - budget) / tm);
+                    v.x = tx * sc;
+                    v.z = tz * sc;
+                }
+            }
+            if (ph.walls > 0) {
+                const int lx = bx * 4 + (t >> 4), ly = by * 4 + ((t >> 2) & 3), lz = bz * 4 + (t & 3);
+                const int W = ph.walls;
+                if ((lx < W && v.x < 0.f) || (lx >= d.nx - W && v.x > 0.f)) v.x = 0.f;
+                if ((ly < W && v.y < 0.f) || (ly >= d.ny - W && v.y > 0.f)) v.y = 0.f;
+                if ((lz < W && v.z < 0.f) || (lz >= d.nz - W && v.z > 0.f)) v.z = 0.f;
+            }
+            q.x = v.x;
+            q.y = v.y;
+            q.z = v.z;
+        }
+        g.mv[nid] = q;
+    }
+}
+
+// --------------------------------------------------------------- G2P (reference order)
+// mpm_solver.h:293-348 and apply_plasticity :114-166.
+__global__ void __launch_bounds__(256) k_g2p_ref(PView P, GridBuf g, Domain d, Phys ph, float dt, int* err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
+    Stencil s;
+    make_stencil(x, y, z, d.inv_dx, s);
+    if (!stencil_in_domain(d, s)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int lx0 = s.b[0] - d.ox, ly0 = s.b[1] - d.oy, lz0 = s.b[2] - d.oz;
+    const float kC = 4.0f * d.inv_dx * d.inv_dx;
+    v3<float> v = mk(0.f, 0.f, 0.f);
+    m3<float> Cm;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Cm.a[c] = 0.f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float w = s.w[0][a] * s.w[1][b] * s.w[2][c];
+                if (w == 0.0f) continue;
+                const float4 q = g.mv[node_of(d, lx0 + a, ly0 + b, lz0 + c)];
+                const v3<float> dd = mk(((float)(s.b[0] + a) - s.f[0] - 0.5f) * d.dx,
+                                        ((float)(s.b[1] + b) - s.f[1] - 0.5f) * d.dx,
+                                        ((float)(s.b[2] + c) - s.f[2] - 0.5f) * d.dx);
+                const v3<float> wv = mk(q.x, q.y, q.z) * w;
+                v = v + wv;
+                const float wvv[3] = {wv.x, wv.y, wv.z}, ddv[3] = {dd.x, dd.y, dd.z};
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) Cm.a[r * 3 + cc] = Cm.a[r * 3 + cc] + (wvv[r] * ddv[cc]) * kC;
+            }
+    x = x + v.x * dt;
+    y = y + v.y * dt;
+    z = z + v.z * dt;
+
+    const m3<float> F = load_m3(P, PF, i);
+    m3<float> A;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A.a[c] = ((c % 4 == 0) ? 1.0f : 0.0f) + Cm.a[c] * dt;
+    const m3<float> Ftr = mmul(A, F);
+    m3<float> U, V;
+    v3<float> sg;
+    svd_ref(Ftr, U, sg, V);
+    const Material M{ph.mu, ph.lambda, ph.alpha, ph.kohesion};
+    const v3<float> sn = dp_project(sg, M);
+    const m3<float> Fn = recompose(U, sn, V);
+
+    if (y < 0.0f) {
+        y = 0.0f;
+        v.y = 0.0f;
+    }
+    P.p[(PX + 0) * P.cap + i] = x;
+    P.p[(PX + 1) * P.cap + i] = y;
+    P.p[(PX + 2) * P.cap + i] = z;
+    P.p[(PV + 0) * P.cap + i] = v.x;
+    P.p[(PV + 1) * P.cap + i] = v.y;
+    P.p[(PV + 2) * P.cap + i] = v.z;
+    store_m3(P, PF, i, Fn);
+    store_m3(P, PC, i, Cm);
+}
+
+// --------------------------------------------------------------- couplers
+__device__ __forceinline__ bool shape_inside_d(const ColliderDev& b, v3<float> l) {
+    if (b.shape == 1) {
+        const float R = b.he[0], L = b.he[1];
+        const float r2 = l.x * l.x + l.z * l.z;
+        return fabsf(l.y) <= L && r2 <= R * R;
+    }
+    return fabsf(l.x) <= b.he[0] && fabsf(l.y) <= b.he[1] && fabsf(l.z) <= b.he[2];
+}
+__device__ __forceinline__ v3<float> shape_normal_d(const ColliderDev& b, v3<float> l) {
+    if (b.shape == 1) {
+        const float R = b.he[0], L = b.he[1];
+        const float r = sqrtf(l.x * l.x + l.z * l.z);
+        const float ad = L - fabsf(l.y), rd = R - r;
+        if (ad < rd) return mk(0.f, l.y >= 0.f ? 1.f : -1.f, 0.f);
+        if (r > 1e-9f) return mk(l.x / r, 0.f, l.z / r);
+        return mk(1.f, 0.f, 0.f);
+    }
+    const float ddx = b.he[0] - fabsf(l.x), ddy = b.he[1] - fabsf(l.y), ddz = b.he[2] - fabsf(l.z);
+    if (ddx <= ddy && ddx <= ddz) return mk(l.x >= 0.f ? 1.f : -1.f, 0.f, 0.f);
+    if (ddy <= ddz) return mk(0.f, l.y >= 0.f ? 1.f : -1.f, 0.f);
+    return mk(0.f, 0.f, l.z >= 0.f ? 1.f : -1.f);
+}
+
+// One workgroup of 256 lanes = 4 grid blocks of the scanned range.  Each lane
+// produces one node's reaction contribution (7 values); the workgroup reduces
+// them in a fixed tree order in FP64 and writes one partial per workgroup.
+__global__ void __launch_bounds__(256) k_couple(int method, ColliderDev b, GridBuf g, uint8_t* bever, Domain d,
+                                                float dt, double* partials) {
+    __shared__ double red[7][256];
+    const int t = threadIdx.x;
+    const int64_t total_blocks = (int64_t)b.nb[0] * b.nb[1] * b.nb[2];
+    const int64_t rb = (int64_t)blockIdx.x * 4 + (t >> 6);
+    double c7[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (rb < total_blocks) {
+        const int rbz = (int)(rb % b.nb[2]);
+        const int rby = (int)((rb / b.nb[2]) % b.nb[1]);
+        const int rbx = (int)(rb / ((int64_t)b.nb[2] * b.nb[1]));
+        const int bx = b.b0[0] + rbx, by = b.b0[1] + rby, bz = b.b0[2] + rbz;
+        const int blk = (bx * d.nby + by) * d.nbz + bz;
+        // global node index of the block origin
+        const int ox = d.ox + bx * 4, oy = d.oy + by * 4, oz = d.oz + bz * 4;
+        // block cull, mpm_rigid_coupler.cpp:82-91
+        const float dx = d.dx;
+        const float blx = ((float)ox + 0.5f) * dx, bly = ((float)oy + 0.5f) * dx, blz = ((float)oz + 0.5f) * dx;
+        const float bhx = ((float)(ox + 3) + 0.5f) * dx, bhy = ((float)(oy + 3) + 0.5f) * dx,
+                    bhz = ((float)(oz + 3) + 0.5f) * dx;
+        bool culled = (bhx < b.lo[0] || blx > b.hi[0]) || (bhy < b.lo[1] || bly > b.hi[1]) ||
+                      (bhz < b.lo[2] || blz > b.hi[2]);
+        // M1 sees only blocks that exist in the sparse grid (ever touched)
+        if (method == 1 && bever[blk] == 0) culled = true;
+        const int ln = t & 63;
+        const int li = ln >> 4, lj = (ln >> 2) & 3, lk = ln & 3;
+        const int64_t nid = (int64_t)blk * 64 + ln;
+        if (!culled) {
+            float4 q = g.mv[nid];
+            const bool active = q.w > 0.0f;
+            const bool visit = (method == 0) ? (active && q.w > 1e-8f) : true;
+            if (visit) {
+                const v3<float> xn = mk(((float)(ox + li) + 0.5f) * dx, ((float)(oy + lj) + 0.5f) * dx,
+                                        ((float)(oz + lk) + 0.5f) * dx);
+                const v3<float> r = xn - mk(b.c[0], b.c[1], b.c[2]);
+                m3<float> Rw, Rl;
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    Rw.a[c] = b.Rw[c];
+                    Rl.a[c] = b.Rl[c];
+                }
+                const v3<float> loc = mvec(Rw, r);
+                if (shape_inside_d(b, loc)) {
+                    const v3<float> vb = mk(b.vl[0], b.vl[1], b.vl[2]) + cross(mk(b.wa[0], b.wa[1], b.wa[2]), r);
+                    if (method == 0) {
+                        // mpm_rigid_coupler.cpp:108-138
+                        const v3<float> nl = shape_normal_d(b, loc);
+                        const v3<float> v0 = mk(q.x, q.y, q.z);
+                        v3<float> n = mvec(Rl, nl);
+                        const float nlen = sqrtf(dot(n, n));
+                        n = (nlen < 1e-6f) ? mk(0.f, 0.f, 0.f) : divg(n, nlen);
+                        const float vnn = dot(v0, n), vnb = dot(vb, n);
+                        const float vnc = fmaxf(vnn, vnb);
+                        const v3<float> vtn = v0 - n * vnn;
+                        const v3<float> vtb = vb - n * vnb;
+                        const v3<float> vtnew = vtn + (vtb - vtn) * fminf(1.0f, b.friction);
+                        const v3<float> va = n * vnc + vtnew;
+                        const v3<float> dp = (va - v0) * q.w;
+                        q.x = va.x;
+                        q.y = va.y;
+                        q.z = va.z;
+                        g.mv[nid] = q;
+                        const v3<float> fo = divg(dp, dt);
+                        const v3<float> to = divg(cross(r, dp), dt);
+                        c7[0] = -(double)fo.x;
+                        c7[1] = -(double)fo.y;
+                        c7[2] = -(double)fo.z;
+                        c7[3] = -(double)to.x;
+                        c7[4] = -(double)to.y;
+                        c7[5] = -(double)to.z;
+                        c7[6] = 1.0;
+                    } else {
+                        // mpm_rigid_coupler_pic.h:145-177
+                        v3<float> va;
+                        if (!active || q.w < 1e-9f) {
+                            q = make_float4(vb.x, vb.y, vb.z, b.mb);
+                            va = vb;
+                            touch_block(g, nullptr, blk);  // cleared with the rest next substep
+                        } else {
+                            const v3<float> pold = mk(q.x, q.y, q.z) * q.w;
+                            const v3<float> padd = vb * b.mb;
+                            const float mn = q.w + b.mb;
+                            va = divg(pold + padd, mn);
+                            q = make_float4(va.x, va.y, va.z, mn);
+                        }
+                        g.mv[nid] = q;
+                        const v3<float> dF = (va - vb) * (b.mb / dt);
+                        const v3<float> to = cross(r, dF);
+                        c7[0] = dF.x;
+                        c7[1] = dF.y;
+                        c7[2] = dF.z;
+                        c7[3] = to.x;
+                        c7[4] = to.y;
+                        c7[5] = to.z;
+                        c7[6] = 1.0;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) red[k][t] = c7[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) red[k][t] += red[k][t + s];
+        }
+        __syncthreads();
+    }
+    if (t < 7) partials[(int64_t)blockIdx.x * 7 + t] = red[t][0];
+}
+
+// Sum the per-workgroup partials in index order (deterministic).
+__global__ void __launch_bounds__(64) k_reduce_partials(const double* partials, int nparts, double* out7) {
+    const int t = threadIdx.x;
+    if (t < 7) {
+        double s = 0.0;
+        for (int i = 0; i < nparts; ++i) s += partials[(int64_t)i * 7 + t];
+        out7[t] = s;
+    }
+}
+
+// --------------------------------------------------------------- grid I/O
+__global__ void k_grid_gather(GridBuf g, uint8_t* bever, Domain d, int org0, int org1, int org2, int n0, int n1,
+                              int n2, double qscale, int updated, long long* o_mq, float* o_mass, float* o_vel,
+                              float* o_force, uint8_t* o_act, uint8_t* o_alloc) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = (int64_t)n0 * n1 * n2;
+    if (idx >= tot) return;
+    const int k = (int)(idx % n2), j = (int)((idx / n2) % n1), i = (int)(idx / ((int64_t)n2 * n1));
+    const int lx = org0 + i - d.ox, ly = org1 + j - d.oy, lz = org2 + k - d.oz;
+    long long mq = 0;
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f), f = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint8_t al = 0;
+    if (lx >= 0 && ly >= 0 && lz >= 0 && lx < d.nx && ly < d.ny && lz < d.nz) {
+        const int64_t nid = node_of(d, lx, ly, lz);
+        mq = g.mq[nid];
+        q = g.mv[nid];
+        if (g.f) f = g.f[nid];
+        al = bever ? bever[nid >> 6] : 0;
+    }
+    const float mass = updated ? q.w : (float)((double)mq * qscale);
+    if (o_mq) o_mq[idx] = mq;
+    if (o_mass) o_mass[idx] = mass;
+    if (o_vel) {
+        o_vel[3 * idx] = q.x;
+        o_vel[3 * idx + 1] = q.y;
+        o_vel[3 * idx + 2] = q.z;
+    }
+    if (o_force) {
+        o_force[3 * idx] = f.x;
+        o_force[3 * idx + 1] = f.y;
+        o_force[3 * idx + 2] = f.z;
+    }
+    if (o_act) o_act[idx] = (mq != 0 || (updated && q.w != 0.0f)) ? 1 : 0;
+    if (o_alloc) o_alloc[idx] = al;
+}
+
+__global__ void k_grid_scatter(GridBuf g, uint8_t* bever, Domain d, int org0, int org1, int org2, int n0, int n1,
+                               int n2, double qscale_inv, const float* i_mass, const float* i_vel,
+                               const uint8_t* i_act) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = (int64_t)n0 * n1 * n2;
+    if (idx >= tot) return;
+    if (!i_act[idx]) return;
+    const int k = (int)(idx % n2), j = (int)((idx / n2) % n1), i = (int)(idx / ((int64_t)n2 * n1));
+    const int lx = org0 + i - d.ox, ly = org1 + j - d.oy, lz = org2 + k - d.oz;
+    if (!(lx >= 0 && ly >= 0 && lz >= 0 && lx < d.nx && ly < d.ny && lz < d.nz)) return;
+    const int64_t nid = node_of(d, lx, ly, lz);
+    const float m = i_mass[idx];
+    g.mv[nid] = make_float4(i_vel[3 * idx], i_vel[3 * idx + 1], i_vel[3 * idx + 2], m);
+    long long mq = (long long)rint((double)m * qscale_inv);
+    g.mq[nid] = mq;
+    touch_block(g, bever, (int)(nid >> 6));
+}
+
+__global__ void __launch_bounds__(256) k_mass_total(GridBuf g, unsigned long long* out) {
+    const int cnt = *g.bcount;
+    const int64_t tot = (int64_t)cnt * 64;
+    unsigned long long s = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += (int64_t)gridDim.x * blockDim.x) {
+        const int blk = g.blist[k >> 6];
+        s += (unsigned long long)g.mq[(int64_t)blk * 64 + (k & 63)];
+    }
+    atomicAdd(out, s);
+}
+
+// V0 *= rescale for every particle (set_material density change, mpm_solver.h:45-49).
+__global__ void k_scale_v0(PView P, float rescale) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.n) P.p[PV0 * P.cap + i] *= rescale;
+}
+
+// Largest particle mass, as float bits (masses are non-negative).
+__global__ void k_max_mass(PView P, unsigned int* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned int v = 0;
+    if (i < P.n) v = __float_as_uint(fmaxf(P.p[PM * P.cap + i], 0.0f));
+    atomicMax(out, v);
+}
+
+// ================================================================ launchers
+static inline int grid_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+void launch_clear(const GridBuf& g, int nblocks, hipStream_t s) {
+    const int grid = nblocks < 8192 ? (nblocks > 0 ? nblocks : 1) : 8192;
+    hipLaunchKernelGGL(k_clear_blocks, dim3(grid), dim3(64), 0, s, g);
+    hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s, g);
+}
+void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Domain& d, const Phys& ph,
+                    double qscale_inv, int* err, hipStream_t s) {
+    if (P.n == 0) return;
+    hipLaunchKernelGGL(k_p2g_ref, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, g, bever, d, ph, qscale_inv, err);
+}
+void launch_update(const GridBuf& g, int nblocks, const Domain& d, const Phys& ph, double qscale, float dt,
+                   hipStream_t s) {
+    const int grid = nblocks < 16384 ? (nblocks > 0 ? nblocks : 1) : 16384;
+    hipLaunchKernelGGL(k_update, dim3(grid), dim3(64), 0, s, g, d, ph, qscale, dt);
+}
+void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phys& ph, float dt, int* err,
+                    hipStream_t s) {
+    if (P.n == 0) return;
+    hipLaunchKernelGGL(k_g2p_ref, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, g, d, ph, dt, err);
+}
+int couple_partials_needed(const ColliderDev& b) {
+    const int64_t tb = (int64_t)b.nb[0] * b.nb[1] * b.nb[2];
+    return (int)((tb + 3) / 4);
+}
+void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* bever, const Domain& d, float dt,
+                   double* partials, double* out7, hipStream_t s) {
+    const int nparts = couple_partials_needed(b);
+    if (nparts > 0)
+        hipLaunchKernelGGL(k_couple, dim3(nparts), dim3(256), 0, s, method, b, g, bever, d, dt, partials);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partials, nparts, out7);
+}
+void launch_grid_gather(const GridBuf& g, uint8_t* bever, const Domain& d, const int* org, const int* dims,
+                        double qscale, int updated, long long* o_mq, float* o_mass, float* o_vel, float* o_force,
+                        uint8_t* o_act, uint8_t* o_alloc, hipStream_t s) {
+    const int64_t tot = (int64_t)dims[0] * dims[1] * dims[2];
+    if (tot == 0) return;
+    hipLaunchKernelGGL(k_grid_gather, dim3(grid_for(tot, 256)), dim3(256), 0, s, g, bever, d, org[0], org[1], org[2],
+                       dims[0], dims[1], dims[2], qscale, updated, o_mq, o_mass, o_vel, o_force, o_act, o_alloc);
+}
+void launch_grid_scatter(const GridBuf& g, uint8_t* bever, const Domain& d, const int* org, const int* dims,
+                         double qscale_inv, const float* i_mass, const float* i_vel, const uint8_t* i_act,
+                         hipStream_t s) {
+    const int64_t tot = (int64_t)dims[0] * dims[1] * dims[2];
+    if (tot == 0) return;
+    hipLaunchKernelGGL(k_grid_scatter, dim3(grid_for(tot, 256)), dim3(256), 0, s, g, bever, d, org[0], org[1],
+                       org[2], dims[0], dims[1], dims[2], qscale_inv, i_mass, i_vel, i_act);
+}
+void launch_mass_total(const GridBuf& g, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mass_total, dim3(1024), dim3(256), 0, s, g, out);
+}
+void launch_scale_v0(const PView& P, float rescale, hipStream_t s) {
+    if (P.n == 0) return;
+    hipLaunchKernelGGL(k_scale_v0, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, rescale);
+}
+void launch_max_mass(const PView& P, unsigned int* out, hipStream_t s) {
+    if (P.n == 0) return;
+    hipLaunchKernelGGL(k_max_mass, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, out);
+}
+
+}  // namespace bm

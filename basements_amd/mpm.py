@@ -1,0 +1,430 @@
+"""Python host mirror of the MLS-MPM soil substep over the libbm_mpm.so C ABI.
+
+Two layers:
+
+* :class:`Solver` -- one device handle (``bm_mpm_handle``) with an explicit
+  grid domain; every method is one C-ABI call (include/bm_mpm.h).
+* :class:`MPMSolver` -- the reference's ``basements::mpm::MPMSolver`` API
+  (include/basements/physics/mpm/mpm_solver.h:15-95 in the reference) with the
+  same method names and argument meaning; it derives the grid domain from the
+  particle cloud the way the reference's unbounded sparse grid would allocate
+  it, and mirrors get_particles()/get_grid() with host copies.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is visible
+these classes raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+from .build import LIB as LIB_PATH
+
+INCLUDE_H = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "bm_mpm.h")
+
+BM_OK = 0
+STATUS_NAMES = {0: "BM_OK", 1: "BM_ERR_INVALID_ARG", 2: "BM_ERR_OOM", 3: "BM_ERR_HIP", 4: "BM_ERR_NONFINITE",
+                5: "BM_ERR_NOT_SUPPORTED", 6: "BM_ERR_OUT_OF_DOMAIN", 7: "BM_ERR_STATE"}
+MODE_REFERENCE = 0
+MODE_FAST = 1
+M0 = 0
+M1 = 1
+SHAPE_BOX = 0
+SHAPE_CYLINDER = 1
+
+
+class BMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+# ------------------------------------------------------------------ C structs
+class Desc(C.Structure):
+    _fields_ = [("dx", C.c_float), ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_int32 * 3),
+                ("max_particles", C.c_int64), ("mode", C.c_int32), ("sort_every", C.c_int32),
+                ("stream", C.c_void_p)]
+
+
+class HostParticles(C.Structure):
+    _fields_ = [("x", C.POINTER(C.c_float)), ("v", C.POINTER(C.c_float)), ("F", C.POINTER(C.c_float)),
+                ("C", C.POINTER(C.c_float)), ("m", C.POINTER(C.c_float)), ("V0", C.POINTER(C.c_float))]
+
+
+class DeviceParticles(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("v", C.c_void_p), ("F", C.c_void_p), ("C", C.c_void_p), ("m", C.c_void_p),
+                ("V0", C.c_void_p), ("capacity", C.c_int64)]
+
+
+class ColliderC(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("quat", C.c_float * 4), ("half_extents", C.c_float * 3),
+                ("lin_vel", C.c_float * 3), ("ang_vel", C.c_float * 3), ("friction", C.c_float),
+                ("mass", C.c_float), ("shape", C.c_int32)]
+
+
+class ReactionC(C.Structure):
+    _fields_ = [("force", C.c_float * 3), ("torque", C.c_float * 3), ("nodes_inside", C.c_int32)]
+
+
+@dataclass
+class Collider:
+    """RigidColliderState (mpm_rigid_coupler.h:72-84); quat = (w, x, y, z)."""
+
+    center: tuple = (0.0, 0.0, 0.0)
+    quat: tuple = (1.0, 0.0, 0.0, 0.0)
+    half_extents: tuple = (0.5, 0.5, 0.5)
+    lin_vel: tuple = (0.0, 0.0, 0.0)
+    ang_vel: tuple = (0.0, 0.0, 0.0)
+    friction: float = 0.5
+    mass: float = 1.0
+    shape: int = SHAPE_BOX
+
+    def to_c(self) -> ColliderC:
+        c = ColliderC()
+        c.center[:] = [float(v) for v in self.center]
+        c.quat[:] = [float(v) for v in self.quat]
+        c.half_extents[:] = [float(v) for v in self.half_extents]
+        c.lin_vel[:] = [float(v) for v in self.lin_vel]
+        c.ang_vel[:] = [float(v) for v in self.ang_vel]
+        c.friction = float(self.friction)
+        c.mass = float(self.mass)
+        c.shape = int(self.shape)
+        return c
+
+
+# ------------------------------------------------------------------ library
+_lib = None
+
+_SIGS = {
+    "bm_abi_version": (C.c_int, []),
+    "bm_last_error": (C.c_char_p, []),
+    "bm_mpm_create": (C.c_int, [C.POINTER(Desc), C.POINTER(C.c_void_p)]),
+    "bm_mpm_destroy": (C.c_int, [C.c_void_p]),
+    "bm_mpm_synchronize": (C.c_int, [C.c_void_p]),
+    "bm_mpm_set_material": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+    "bm_mpm_set_gravity": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+    "bm_mpm_set_plastic_alpha": (C.c_int, [C.c_void_p, C.c_float]),
+    "bm_mpm_set_floor_friction": (C.c_int, [C.c_void_p, C.c_float]),
+    "bm_mpm_set_walls": (C.c_int, [C.c_void_p, C.c_int32]),
+    "bm_mpm_add_particles": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bm_mpm_upload_particles": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(HostParticles)]),
+    "bm_mpm_download_particles": (C.c_int, [C.c_void_p, C.POINTER(HostParticles)]),
+    "bm_mpm_num_particles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "bm_mpm_particles_device": (C.c_int, [C.c_void_p, C.POINTER(DeviceParticles)]),
+    "bm_mpm_commit_device_particles": (C.c_int, [C.c_void_p, C.c_int64]),
+    "bm_mpm_step": (C.c_int, [C.c_void_p, C.c_float]),
+    "bm_mpm_step_n": (C.c_int, [C.c_void_p, C.c_float, C.c_int32]),
+    "bm_mpm_p2g": (C.c_int, [C.c_void_p, C.c_float]),
+    "bm_mpm_grid_update": (C.c_int, [C.c_void_p, C.c_float]),
+    "bm_mpm_g2p": (C.c_int, [C.c_void_p, C.c_float]),
+    "bm_mpm_couple": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ColliderC), C.c_float, C.POINTER(ReactionC)]),
+    "bm_mpm_download_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bm_mpm_upload_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bm_mpm_grid_mass_total": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "bm_mpm_mass_quantum_exp": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "bm_mpm_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+}
+
+
+def header_symbols(path: str = INCLUDE_H) -> list[str]:
+    """Every function the public header declares."""
+    txt = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(bm_\w+)\s*\(", txt, flags=re.M)))
+
+
+def lib():
+    """Load libbm_mpm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built; run `python -m basements_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        if L.bm_abi_version() != 1:
+            raise RuntimeError("libbm_mpm.so ABI version mismatch")
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != BM_OK:
+        raise BMError(rc, lib().bm_last_error().decode())
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, np.float32))
+    return a if n is None else a.reshape(n)
+
+
+# ------------------------------------------------------------------ Solver
+class Solver:
+    """One device-resident scene behind a ``bm_mpm_handle``."""
+
+    def __init__(self, dx, origin, dims, max_particles, mode=MODE_REFERENCE, sort_every=0, stream=None):
+        L = lib()
+        d = Desc()
+        d.dx = float(dx)
+        d.grid_origin[:] = [int(v) for v in origin]
+        d.grid_dims[:] = [int(v) for v in dims]
+        d.max_particles = int(max_particles)
+        d.mode = int(mode)
+        d.sort_every = int(sort_every)
+        d.stream = stream
+        h = C.c_void_p()
+        _check(L.bm_mpm_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.dx = float(dx)
+        self.origin = tuple(int(v) for v in origin)
+        self.dims = tuple((int(v) + 3) // 4 * 4 for v in dims)
+        self.mode = mode
+        self.cap = int(max_particles)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bm_mpm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # parameters --------------------------------------------------------
+    def set_material(self, E, nu, rho):
+        _check(lib().bm_mpm_set_material(self.h, E, nu, rho))
+
+    def set_gravity(self, g):
+        _check(lib().bm_mpm_set_gravity(self.h, *[float(v) for v in g]))
+
+    def set_plastic_alpha(self, a):
+        _check(lib().bm_mpm_set_plastic_alpha(self.h, a))
+
+    def set_floor_friction(self, mu):
+        _check(lib().bm_mpm_set_floor_friction(self.h, mu))
+
+    def set_walls(self, thickness):
+        _check(lib().bm_mpm_set_walls(self.h, int(thickness)))
+
+    # particles ---------------------------------------------------------
+    def add_particles(self, pos, vel, mass):
+        pos = _f32(pos, -1)
+        n = pos.size // 3
+        vel = _f32(vel, -1)
+        mass = _f32(np.broadcast_to(np.asarray(mass, np.float32), (n,)), -1)
+        _check(lib().bm_mpm_add_particles(self.h, n, _ptr(pos), _ptr(vel), _ptr(mass)))
+
+    def upload(self, P):
+        n = int(np.asarray(P["x"]).reshape(-1, 3).shape[0])
+        arrs = {k: _f32(P[k], -1) for k in ("x", "v", "F", "C", "m", "V0")}
+        hp = HostParticles(*[arrs[k].ctypes.data_as(C.POINTER(C.c_float)) for k in ("x", "v", "F", "C", "m", "V0")])
+        _check(lib().bm_mpm_upload_particles(self.h, n, C.byref(hp)))
+
+    def num_particles(self):
+        n = C.c_int64()
+        _check(lib().bm_mpm_num_particles(self.h, C.byref(n)))
+        return n.value
+
+    def download(self):
+        n = self.num_particles()
+        out = {"x": np.zeros((n, 3), np.float32), "v": np.zeros((n, 3), np.float32),
+               "F": np.zeros((n, 3, 3), np.float32), "C": np.zeros((n, 3, 3), np.float32),
+               "m": np.zeros(n, np.float32), "V0": np.zeros(n, np.float32)}
+        hp = HostParticles(*[out[k].ctypes.data_as(C.POINTER(C.c_float)) for k in ("x", "v", "F", "C", "m", "V0")])
+        _check(lib().bm_mpm_download_particles(self.h, C.byref(hp)))
+        return out
+
+    def device_particles(self):
+        dp = DeviceParticles()
+        _check(lib().bm_mpm_particles_device(self.h, C.byref(dp)))
+        return dp
+
+    def commit_device_particles(self, n):
+        _check(lib().bm_mpm_commit_device_particles(self.h, int(n)))
+
+    # substep -----------------------------------------------------------
+    def step(self, dt, k=1):
+        if k == 1:
+            _check(lib().bm_mpm_step(self.h, dt))
+        else:
+            _check(lib().bm_mpm_step_n(self.h, dt, int(k)))
+
+    def p2g(self, dt=0.0):
+        _check(lib().bm_mpm_p2g(self.h, dt))
+
+    def grid_update(self, dt):
+        _check(lib().bm_mpm_grid_update(self.h, dt))
+
+    def g2p(self, dt):
+        _check(lib().bm_mpm_g2p(self.h, dt))
+
+    def synchronize(self):
+        _check(lib().bm_mpm_synchronize(self.h))
+
+    def couple(self, method, col: Collider, dt):
+        r = ReactionC()
+        c = col.to_c()
+        _check(lib().bm_mpm_couple(self.h, int(method), C.byref(c), float(dt), C.byref(r)))
+        return np.array(r.force[:], np.float64), np.array(r.torque[:], np.float64), int(r.nodes_inside)
+
+    # grid --------------------------------------------------------------
+    def download_grid(self, org=None, dims=None):
+        org = np.ascontiguousarray(self.origin if org is None else org, np.int32)
+        dims = np.ascontiguousarray(self.dims if dims is None else dims, np.int32)
+        sh = tuple(int(v) for v in dims)
+        nn = int(np.prod(sh))
+        mq = np.zeros(nn, np.int64)
+        mass = np.zeros(nn, np.float32)
+        vel = np.zeros((nn, 3), np.float32)
+        force = np.zeros((nn, 3), np.float32)
+        act = np.zeros(nn, np.uint8)
+        alloc = np.zeros(nn, np.uint8)
+        _check(lib().bm_mpm_download_grid(self.h, _ptr(org), _ptr(dims), _ptr(mq), _ptr(mass), _ptr(vel),
+                                          _ptr(force), _ptr(act), _ptr(alloc)))
+        return {"mass_q": mq.reshape(sh), "mass": mass.reshape(sh), "vel": vel.reshape(sh + (3,)),
+                "force": force.reshape(sh + (3,)), "active": act.reshape(sh).astype(bool),
+                "alloc": alloc.reshape(sh).astype(bool)}
+
+    def upload_grid(self, org, mass, vel, active):
+        org = np.ascontiguousarray(org, np.int32)
+        mass = _f32(mass)
+        dims = np.ascontiguousarray(mass.shape, np.int32)
+        vel = _f32(vel)
+        act = np.ascontiguousarray(active, np.uint8)
+        _check(lib().bm_mpm_upload_grid(self.h, _ptr(org), _ptr(dims), _ptr(mass), _ptr(vel), _ptr(act)))
+
+    def grid_mass_total(self):
+        t = C.c_int64()
+        _check(lib().bm_mpm_grid_mass_total(self.h, C.byref(t)))
+        return t.value
+
+    def mass_quantum_exp(self):
+        e = C.c_int32()
+        _check(lib().bm_mpm_mass_quantum_exp(self.h, C.byref(e)))
+        return e.value
+
+    def stats(self):
+        a, f = C.c_int64(), C.c_int64()
+        _check(lib().bm_mpm_stats(self.h, C.byref(a), C.byref(f)))
+        return {"active_blocks": a.value, "fp64_fallbacks": f.value}
+
+
+# ------------------------------------------------------------------ MPMSolver mirror
+class MPMSolver:
+    """``basements::mpm::MPMSolver`` (mpm_solver.h:15-95) on the GPU.
+
+    Particles are staged on the host until the first step; the device grid is
+    then sized to the particle cloud plus ``margin_cells`` on every side (the
+    reference's grid is unbounded and sparse).  ``get_particles_mutable()``
+    returns host arrays whose edits are uploaded before the next step.
+    """
+
+    def __init__(self, mode=MODE_REFERENCE, margin_cells=16, capacity=None):
+        self.dx = 0.1  # MPMSolver() : grid(0.1f)  (mpm_solver.h:17)
+        self.mode = mode
+        self.margin = int(margin_cells)
+        self.capacity = capacity
+        self._E, self._nu, self._rho = 1.0e5, 0.3, 1500.0
+        self._alpha, self._mu_floor = 0.40, 0.4
+        self._g = (0.0, -9.81, 0.0)
+        self._host = {k: [] for k in ("x", "v", "m")}
+        self._P = None          # host mirror (dict of arrays) when valid
+        self._dirty = False     # host mirror edited since the last upload
+        self._dev = None
+
+    def initialize(self, dx):
+        self.dx = float(dx)
+        self._dev = None
+
+    def add_particle(self, pos, vel, mass=1.0):
+        P = self._materialize()
+        m = np.float32(mass)
+        V0 = (m / np.float32(self._rho)) if np.float32(self._rho) > np.float32(1e-12) else np.float32(1.0)
+        for k, val in (("x", pos), ("v", vel)):
+            P[k] = np.concatenate([P[k], np.asarray(val, np.float32).reshape(1, 3)])
+        P["F"] = np.concatenate([P["F"], np.eye(3, dtype=np.float32)[None]])
+        P["C"] = np.concatenate([P["C"], np.zeros((1, 3, 3), np.float32)])
+        P["m"] = np.concatenate([P["m"], [m]]).astype(np.float32)
+        P["V0"] = np.concatenate([P["V0"], [V0]]).astype(np.float32)
+        self._dirty = True
+
+    def set_material(self, E, nu, rho):
+        self._E, self._nu = float(E), float(nu)
+        P = self._materialize()
+        old, new = np.float32(self._rho), np.float32(rho)
+        if new > 1e-12 and old > 1e-12 and new != old and P["m"].size:
+            P["V0"] = (P["V0"] * (old / new)).astype(np.float32)
+            self._dirty = True
+        self._rho = float(rho)
+
+    def set_gravity(self, g):
+        self._g = tuple(float(v) for v in g)
+
+    def set_plastic_alpha(self, a):
+        self._alpha = float(a)
+
+    def set_floor_friction(self, mu):
+        self._mu_floor = float(mu)
+
+    def get_particles(self):
+        return self._materialize()
+
+    def get_particles_mutable(self):
+        P = self._materialize()
+        self._dirty = True
+        return P
+
+    def get_grid(self):
+        if self._dev is None:
+            return None
+        return self._dev.download_grid()
+
+    def step(self, dt):
+        self._sync_device()
+        self._dev.step(dt)
+        self._P = None
+
+    # internals ---------------------------------------------------------
+    def _materialize(self):
+        if self._P is None:
+            if self._dev is not None:
+                self._P = self._dev.download()
+            else:
+                self._P = {"x": np.zeros((0, 3), np.float32), "v": np.zeros((0, 3), np.float32),
+                           "F": np.zeros((0, 3, 3), np.float32), "C": np.zeros((0, 3, 3), np.float32),
+                           "m": np.zeros(0, np.float32), "V0": np.zeros(0, np.float32)}
+        return self._P
+
+    def _domain_for(self, x):
+        lo = np.floor(x.min(0) / self.dx - 0.5).astype(int) - self.margin
+        hi = np.floor(x.max(0) / self.dx - 0.5).astype(int) + 2 + self.margin
+        return lo, hi - lo + 1
+
+    def _sync_device(self):
+        P = self._materialize()
+        n = P["x"].shape[0]
+        if self._dev is None or n > self._dev.cap:
+            org, dims = self._domain_for(P["x"]) if n else (np.zeros(3, int), np.full(3, 4))
+            cap = max(n, self.capacity or 0, 1)
+            self._dev = Solver(self.dx, org, dims, cap, self.mode)
+            self._dirty = True
+        d = self._dev
+        d.set_material(self._E, self._nu, self._rho)
+        d.set_gravity(self._g)
+        d.set_plastic_alpha(self._alpha)
+        d.set_floor_friction(self._mu_floor)
+        if self._dirty:
+            d.upload(P)
+            self._dirty = False

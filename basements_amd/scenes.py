@@ -1,0 +1,125 @@
+"""Seeded synthetic soil scenes (the benchmark and test inputs).
+
+Positions are drawn from a counter-based generator (Philox keyed by the seed)
+so the same (seed, particle index) always yields the same particle on every
+host.  Particles sit one per half-cell sub-cube, jittered uniformly inside it
+(8 per cell), with v = 0, F = I, C = 0 and V0 = m / rho as
+MPMSolver::add_particle sets them (mpm_solver.h:23-34 in the reference).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.Philox(key=int(seed)))
+
+
+def soil_block(dx, cell_lo, cell_hi, seed=0, rho=1600.0, ppc_axis=2):
+    """Particles filling cells [cell_lo, cell_hi) (integer cell coords of a dx grid).
+
+    ppc_axis sub-cubes per axis per cell (2 -> 8 particles per cell).
+    Returns the particle dict (float32 arrays) used by Solver.upload.
+    """
+    lo = np.asarray(cell_lo, np.int64)
+    hi = np.asarray(cell_hi, np.int64)
+    sub = (hi - lo) * ppc_axis
+    n = int(np.prod(sub))
+    idx = np.indices(tuple(int(s) for s in sub)).reshape(3, -1).T.astype(np.float64)
+    h = dx / ppc_axis
+    jit = _rng(seed).random((n, 3))
+    x = (lo[None, :] * dx + (idx + jit) * h).astype(np.float32)
+    # particle mass = rho * (sub-cube volume), rounded once to float32
+    m = np.float32(rho * h ** 3)
+    return make_particles(x, mass=m, rho=rho)
+
+
+def make_particles(x, v=None, mass=1.0, rho=1500.0, F=None, Cm=None):
+    x = np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1, 3))
+    n = x.shape[0]
+    m = np.broadcast_to(np.asarray(mass, np.float32), (n,)).astype(np.float32)
+    rho32 = np.float32(rho)
+    V0 = (m / rho32).astype(np.float32) if rho32 > np.float32(1e-12) else np.ones(n, np.float32)
+    return {
+        "x": x,
+        "v": np.zeros((n, 3), np.float32) if v is None else np.asarray(v, np.float32).reshape(n, 3),
+        "F": np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)) if F is None else np.asarray(F, np.float32).reshape(n, 3, 3),
+        "C": np.zeros((n, 3, 3), np.float32) if Cm is None else np.asarray(Cm, np.float32).reshape(n, 3, 3),
+        "m": m,
+        "V0": V0,
+    }
+
+
+def dp_alpha(friction_deg):
+    """alpha = sqrt(2/3) * 2 sin(phi) / (3 - sin(phi))  (Klar 2016; mpm_solver.h:103-104)."""
+    s = math.sin(math.radians(friction_deg))
+    return math.sqrt(2.0 / 3.0) * 2.0 * s / (3.0 - s)
+
+
+@dataclass
+class Scene:
+    name: str
+    dx: float
+    particles: dict
+    origin: tuple
+    dims: tuple
+    E: float = 3.5e5
+    nu: float = 0.3
+    rho: float = 1600.0
+    alpha: float = dp_alpha(35.0)
+    mu_floor: float = 0.4
+    gravity: tuple = (0.0, -9.81, 0.0)
+    dt: float = 1e-4
+    substeps: int = 1000
+    tool: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return int(self.particles["x"].shape[0])
+
+
+def box_drop(seed=0, scale=1):
+    """Soil block 0.5 x 0.25 x 0.5 m on a 64^3 grid (dx = 1/64 m), 8 ppc
+    (131,072 particles at scale 1), rho 1600, E 3.5e5, nu 0.3, friction 35 deg,
+    with a 0.1 m rigid cube 0.1 m above the surface (M0 Dirichlet).
+    ``scale`` > 1 shrinks the block per axis for quick tests."""
+    dx = 1.0 / 64.0
+    lo = np.array([16, 0, 16])
+    hi = np.array([48, 16, 48])
+    if scale > 1:
+        ext = (hi - lo) // scale
+        lo = np.array([32 - ext[0] // 2, 0, 32 - ext[2] // 2])
+        hi = lo + np.maximum(ext, 1)
+    P = soil_block(dx, lo, hi, seed=seed, rho=1600.0)
+    top = hi[1] * dx
+    tool = {"kind": "box", "half": (0.05, 0.05, 0.05), "center": (0.5, top + 0.1 + 0.05, 0.5), "friction": 0.5,
+            "method": 0}
+    return Scene("box_drop", dx, P, origin=(-4, -4, -4), dims=(72, 72, 72), tool=tool, substeps=1000)
+
+
+def wheel_bed(seed=0, friction_deg=None, cells=(120, 36, 120), scale=1):
+    """Soil bed 1.0 x 0.3 x 1.0 m at 8 ppc on a 256^3-node grid.
+
+    The stated particle count (~4.2M) fixes the spacing: dx = 1/120 m gives
+    120 x 36 x 120 cells = 4,147,200 particles; the 256^3-node domain then
+    spans 2.13 m per axis.  A cylinder wheel (r 0.1 m, width 0.08 m) rolls
+    across at 0.2 m/s under a 50 N load through the M1 coupler.  The friction
+    angle is drawn from U(25, 40) deg per seed unless given."""
+    dx = 1.0 / 120.0
+    c = np.array(cells) // scale
+    if friction_deg is None:
+        friction_deg = float(_rng(seed + 7919).uniform(25.0, 40.0))
+    lo = np.array([68, 0, 68]) if scale == 1 else np.array([4, 0, 4])
+    P = soil_block(dx, lo, lo + c, seed=seed, rho=1600.0)
+    dims = (256, 256, 256) if scale == 1 else tuple(int(v) for v in (c[0] + 16, c[1] + 24, c[2] + 16))
+    origin = (0, -4, 0)
+    top = float(c[1] * dx)
+    R = 0.1
+    tool = {"kind": "cylinder", "radius": R, "half_len": 0.04,
+            "center": (float((lo[0] + 0.2 * c[0]) * dx), top + R - 0.01, float((lo[2] + 0.5 * c[2]) * dx)),
+            "speed": 0.2, "load": 50.0, "friction": 0.5, "method": 1}
+    return Scene("wheel_bed", dx, P, origin=origin, dims=dims, alpha=dp_alpha(friction_deg), tool=tool,
+                 substeps=2000)

@@ -1,0 +1,196 @@
+/* bm_mpm.h -- C ABI of the MI355X MLS-MPM soil substep (libbm_mpm.so).
+ *
+ * One handle = one soil scene resident in HBM: particle state (FP32 SoA),
+ * a dense block-linear node grid over a fixed domain, and the scratch the
+ * kernels need.  Every entry point returns an int status (BM_OK = 0) and
+ * leaves a message for bm_last_error() on failure; no C++ exception crosses
+ * the boundary.  Work is queued on the handle's HIP stream; entry points that
+ * hand data back to host memory synchronise that stream first.
+ *
+ * The entry points replace the reference's C++ API for this path (paths
+ * relative to the reference root):
+ *   basements::mpm::MPMSolver           include/basements/physics/mpm/mpm_solver.h:15-95
+ *   MPMRigidCoupler::apply (M0)         include/basements/physics/coupling/mpm_rigid_coupler.h:212-214
+ *   MPMRigidCouplerPIC::apply (M1)      include/basements/physics/coupling/mpm_rigid_coupler_pic.h:88-90
+ *   TimeIntegrator::integrate           include/basements/physics/dynamics/integrator.h:19-57
+ * The C style (POD structs, int status, version check, last-error string)
+ * follows the reference's plugin ABI, include/basements/engine/plugins/plugin_capi.h.
+ *
+ * A handle is not thread-safe; distinct handles are independent.
+ */
+#ifndef BM_MPM_H
+#define BM_MPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BM_MPM_ABI_VERSION 1
+
+enum bm_status {
+    BM_OK = 0,
+    BM_ERR_INVALID_ARG = 1,
+    BM_ERR_OOM = 2,
+    BM_ERR_HIP = 3,
+    BM_ERR_NONFINITE = 4,
+    BM_ERR_NOT_SUPPORTED = 5,
+    BM_ERR_OUT_OF_DOMAIN = 6, /* a particle's 3x3x3 stencil left the grid domain */
+    BM_ERR_STATE = 7          /* phase called out of order (e.g. g2p before p2g) */
+};
+
+/* Substep pipeline.
+ *   BM_MODE_REFERENCE: the reference's phase order -- P2G (stress from an SVD
+ *     of F), grid update, couplers, G2P with the plasticity SVD -- with the
+ *     reference's operation order and FP32 Jacobi; force kept apart from
+ *     momentum on the grid.  mpm_solver.h:71-87.
+ *   BM_MODE_FAST: the same physics with G2P of substep n fused to P2G of
+ *     substep n+1 in one kernel; the Kirchhoff stress of n+1 comes from the
+ *     plasticity SVD of n (F_new = U diag(sigma') V^T is its own SVD), the
+ *     stress force is folded into the node momentum, particles are kept
+ *     binned by grid block, and the SVD has an FP64 fallback. */
+enum bm_mode { BM_MODE_REFERENCE = 0, BM_MODE_FAST = 1 };
+
+/* Coupler method ids; the numbering is the reference's M0/M1
+ * (research/mpm_rigid_coupling/bench_compare_all.cpp:60-63). */
+enum bm_coupler { BM_COUPLER_M0_DIRICHLET = 0, BM_COUPLER_M1_PIC_RIGID = 1 };
+
+enum bm_shape { BM_SHAPE_BOX = 0, BM_SHAPE_CYLINDER = 1 };
+
+typedef struct bm_mpm_solver* bm_mpm_handle;
+
+typedef struct {
+    float dx;                /* node spacing [m]; MPMSolver::initialize(dx) */
+    int32_t grid_origin[3];  /* node index of the domain's lowest corner */
+    int32_t grid_dims[3];    /* nodes per axis; rounded up to a multiple of 4 */
+    int64_t max_particles;   /* capacity of the particle arrays */
+    int32_t mode;            /* enum bm_mode */
+    int32_t sort_every;      /* BM_MODE_FAST: re-bin particles every k substeps (0 = default 16) */
+    void* stream;            /* hipStream_t to run on; NULL = the handle creates its own */
+} bm_mpm_desc;
+
+/* Host-side particle state, SoA, FP32.  Matrices are row-major 3x3 per
+ * particle (F[9*i + 3*r + c]).  Field meaning: struct Particle,
+ * include/basements/physics/mpm/particle.h:9-23 (volume = V0). */
+typedef struct {
+    float* x;  /* 3n */
+    float* v;  /* 3n */
+    float* F;  /* 9n */
+    float* C;  /* 9n */
+    float* m;  /* n  */
+    float* V0; /* n  */
+} bm_particles;
+
+/* Device pointers of the handle's own particle arrays: component-major SoA,
+ * component c of particle i at base[c * capacity + i]. */
+typedef struct {
+    float* x;  /* 3 arrays */
+    float* v;  /* 3 arrays */
+    float* F;  /* 9 arrays, row-major component order */
+    float* C;  /* 9 arrays */
+    float* m;
+    float* V0;
+    int64_t capacity;
+} bm_particles_device;
+
+/* RigidColliderState, mpm_rigid_coupler.h:72-84.  quat = (w, x, y, z);
+ * CYLINDER: half_extents = (radius, half_length, radius), axis = local Y. */
+typedef struct {
+    float center[3];
+    float quat[4];
+    float half_extents[3];
+    float lin_vel[3];
+    float ang_vel[3];
+    float friction;
+    float mass;
+    int32_t shape; /* enum bm_shape */
+} bm_collider;
+
+/* CouplingReaction, mpm_rigid_coupler.h:201-205. */
+typedef struct {
+    float force[3];
+    float torque[3];
+    int32_t nodes_inside;
+} bm_reaction;
+
+/* ---------------------------------------------------------------- library */
+int bm_abi_version(void);
+const char* bm_last_error(void);
+
+/* ---------------------------------------------------------------- lifetime */
+/* MPMSolver() + initialize(dx) (mpm_solver.h:17-21). */
+int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out);
+int bm_mpm_destroy(bm_mpm_handle h);
+int bm_mpm_synchronize(bm_mpm_handle h);
+
+/* ---------------------------------------------------------------- parameters */
+/* set_material (mpm_solver.h:42-51): rescales V0 of existing particles by
+ * rho_old/rho_new when the density changes. */
+int bm_mpm_set_material(bm_mpm_handle h, float young_modulus, float poisson_ratio, float bulk_density);
+int bm_mpm_set_gravity(bm_mpm_handle h, float gx, float gy, float gz);   /* mpm_solver.h:52 */
+int bm_mpm_set_plastic_alpha(bm_mpm_handle h, float alpha);               /* mpm_solver.h:53 */
+int bm_mpm_set_floor_friction(bm_mpm_handle h, float mu_floor);           /* mpm_solver.h:54 */
+/* Optional slip walls on the six domain faces, `thickness` nodes deep: the
+ * outward normal velocity of nodes in that layer is zeroed.  Off (0) by
+ * default; the reference has only the floor (mpm_solver.h:263-288). */
+int bm_mpm_set_walls(bm_mpm_handle h, int32_t thickness);
+
+/* ---------------------------------------------------------------- particles */
+/* add_particle for n particles (mpm_solver.h:23-34): F = I, C = 0,
+ * V0 = m / rho (1 when rho <= 1e-12).  pos, vel: 3n floats; mass: n floats. */
+int bm_mpm_add_particles(bm_mpm_handle h, int64_t n, const float* pos, const float* vel, const float* mass);
+/* Replace the whole particle state (get_particles_mutable(), mpm_solver.h:56-59). */
+int bm_mpm_upload_particles(bm_mpm_handle h, int64_t n, const bm_particles* host);
+/* Copy the state back in the caller's (insertion) order (get_particles(), mpm_solver.h:89). */
+int bm_mpm_download_particles(bm_mpm_handle h, bm_particles* host);
+int bm_mpm_num_particles(bm_mpm_handle h, int64_t* n);
+/* Device-resident access: the caller writes the arrays directly (already in
+ * HBM), then declares the count with bm_mpm_commit_device_particles. */
+int bm_mpm_particles_device(bm_mpm_handle h, bm_particles_device* out);
+int bm_mpm_commit_device_particles(bm_mpm_handle h, int64_t n);
+
+/* ---------------------------------------------------------------- substep */
+/* MPMSolver::step(dt) (mpm_solver.h:62-87): clear, P2G, grid update, G2P. */
+int bm_mpm_step(bm_mpm_handle h, float dt);
+/* k substeps back to back (one call, no host round trip). */
+int bm_mpm_step_n(bm_mpm_handle h, float dt, int32_t k);
+/* The phases of step(), for callers that act on the grid in between -- the
+ * couplers' contract is "between update_grid() and g2p"
+ * (mpm_rigid_coupler.h:209-211).  The reference keeps these private. */
+int bm_mpm_p2g(bm_mpm_handle h, float dt);
+int bm_mpm_grid_update(bm_mpm_handle h, float dt);
+int bm_mpm_g2p(bm_mpm_handle h, float dt);
+
+/* ---------------------------------------------------------------- couplers */
+/* M0 / M1 on the current grid (after bm_mpm_grid_update, before bm_mpm_g2p).
+ * Synchronous: the reaction is written to host memory.  dt <= 0 gives a zero
+ * reaction, as in the reference (mpm_rigid_coupler.cpp:62, _pic.h:92). */
+int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* out);
+
+/* ---------------------------------------------------------------- grid access */
+/* Dense copy of the node box [org, org + dims) (get_grid(), mpm_solver.h:90).
+ * Any output may be NULL.  mass_q: the int64 fixed-point node mass in units
+ * of 2^mass_quantum_exp kg; vel: momentum after P2G, velocity after the grid
+ * update (the reference keeps both in GridNode::velocity); force: the P2G
+ * stress force (BM_MODE_REFERENCE only); alloc: block ever touched by P2G. */
+int bm_mpm_download_grid(bm_mpm_handle h, const int32_t org[3], const int32_t dims[3], int64_t* mass_q,
+                         float* mass, float* vel, float* force, uint8_t* active, uint8_t* alloc);
+/* Seed nodes of the box directly (the reference's coupler tests seed
+ * GridNode velocity/mass/active, tests/test_mpm_rigid_coupler.cpp:26-92).
+ * Leaves the grid in the post-update phase. */
+int bm_mpm_upload_grid(bm_mpm_handle h, const int32_t org[3], const int32_t dims[3], const float* mass,
+                       const float* vel, const uint8_t* active);
+/* Sum of the fixed-point node mass of the last P2G (exactly the sum of the
+ * particles' quanta when every particle was inside the domain). */
+int bm_mpm_grid_mass_total(bm_mpm_handle h, int64_t* mass_q_total);
+int bm_mpm_mass_quantum_exp(bm_mpm_handle h, int32_t* e);
+/* Number of blocks touched by the last P2G, and of 3x3 SVDs that took the
+ * FP64 fallback since creation. */
+int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallbacks);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BM_MPM_H */
