@@ -34,6 +34,8 @@ COMMON = [
 # (source, extra flags).  Reference-order kernels round every a*b+c separately.
 UNITS = [
     ("mpm_ref_kernels.hip", ["-ffp-contract=off"]),
+    ("mpm_fast_kernels.hip", []),
+    ("mpm_fused.hip", []),
     ("bm_mpm_api.hip", ["-ffp-contract=off"]),
 ]
 

@@ -70,6 +70,77 @@ struct bm_mpm_solver {
     float max_mass = 0.0f;
     int phase = PH_EMPTY;
 
+    // fast mode: binned particle order, double-buffered particles and grids
+    bool fast = false;
+    float* pbuf2 = nullptr;
+    int* pid = nullptr;   // original (insertion) index of each stored particle
+    int* pid2 = nullptr;
+    int* redo = nullptr;    // particles redone with the FP64 SVD (one list per launch)
+    int* n_redo = nullptr;
+    BinScratch bs{};
+    int n_units = 0;
+    bool binned = false;      // units describe the current particle order
+    int since_bin = 0;        // substeps since the last re-binning
+    int sort_every = 16;
+    int unit_p = 512;
+    int variant = 1;          // fused-kernel variant (see bm_mpm_set_tuning): ordered read-modify-write
+    bool prepared = false;    // grid[prep] already holds the next substep's P2G
+    float dt_prepared = 0.0f;
+    int prep = 0;
+
+    // per-phase event timing
+    struct Timed {
+        int phase;
+        hipEvent_t a, b;
+    };
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<Timed> pending;
+    double t_ms[BM_NUM_PHASES] = {};
+    int64_t t_calls[BM_NUM_PHASES] = {};
+    int open_phase = -1;
+    hipEvent_t open_ev = nullptr;
+
+    hipEvent_t take_event() {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void begin(int ph) {
+        if (!timing) return;
+        open_phase = ph;
+        open_ev = take_event();
+        if (open_ev) hipEventRecord(open_ev, stream);
+    }
+    void end() {
+        if (!timing || open_phase < 0 || !open_ev) return;
+        hipEvent_t e = take_event();
+        if (e) {
+            hipEventRecord(e, stream);
+            pending.push_back(Timed{open_phase, open_ev, e});
+        }
+        open_phase = -1;
+        open_ev = nullptr;
+    }
+    // Fold finished event pairs into the accumulators (stream must be idle).
+    void harvest() {
+        for (auto& t : pending) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+                t_ms[t.phase] += ms;
+                t_calls[t.phase] += 1;
+            }
+            ev_pool.push_back(t.a);
+            ev_pool.push_back(t.b);
+        }
+        pending.clear();
+    }
+
     PView pview() const { return PView{pbuf, cap, n}; }
     Phys phys() const {
         Phys p;
@@ -152,7 +223,6 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
         if (desc->grid_dims[a] <= 0) return fail(BM_ERR_INVALID_ARG, "grid_dims must be > 0");
     if (desc->mode != BM_MODE_REFERENCE && desc->mode != BM_MODE_FAST)
         return fail(BM_ERR_INVALID_ARG, "unknown mode %d", desc->mode);
-    if (desc->mode == BM_MODE_FAST) return fail(BM_ERR_NOT_SUPPORTED, "BM_MODE_FAST is not built in this library");
 
     auto* h = new bm_mpm_solver();
     h->desc = *desc;
@@ -190,7 +260,29 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
     }
     if (h->cap > 0 && hipMalloc(&h->pbuf, sizeof(float) * PCOMP * h->cap) != hipSuccess)
         return bail(fail(BM_ERR_OOM, "particle arrays (%lld particles)", (long long)h->cap));
-    if ((rc = alloc_grid(h->grid[0], h->nnodes, h->nblocks, true)) != BM_OK) return bail(BM_ERR_OOM);
+    h->fast = desc->mode == BM_MODE_FAST;
+    if (desc->sort_every > 0) h->sort_every = desc->sort_every;
+    if ((rc = alloc_grid(h->grid[0], h->nnodes, h->nblocks, !h->fast)) != BM_OK) return bail(BM_ERR_OOM);
+    if (h->fast) {
+        if ((rc = alloc_grid(h->grid[1], h->nnodes, h->nblocks, false)) != BM_OK) return bail(BM_ERR_OOM);
+        if (h->cap > (int64_t)INT32_MAX) return bail(fail(BM_ERR_INVALID_ARG, "fast mode holds at most 2^31-1 particles"));
+        if (h->nblocks > (int64_t)INT32_MAX / 64) return bail(fail(BM_ERR_INVALID_ARG, "grid too large"));
+        const int64_t pc = std::max<int64_t>(h->cap, 1);
+        BinScratch& b = h->bs;
+        b.temp_bytes = scan_temp_bytes((int)h->nblocks);
+        if (hipMalloc(&h->pbuf2, sizeof(float) * PCOMP * pc) != hipSuccess ||
+            hipMalloc(&h->pid, sizeof(int) * pc) != hipSuccess || hipMalloc(&h->pid2, sizeof(int) * pc) != hipSuccess ||
+            hipMalloc(&b.key, sizeof(int) * pc) != hipSuccess || hipMalloc(&b.slot, sizeof(int) * pc) != hipSuccess ||
+            hipMalloc(&b.bcnt, sizeof(int) * h->nblocks) != hipSuccess ||
+            hipMalloc(&b.bstart, sizeof(int) * h->nblocks) != hipSuccess ||
+            hipMalloc(&b.ucnt, sizeof(int) * h->nblocks) != hipSuccess ||
+            hipMalloc(&b.ustart, sizeof(int) * h->nblocks) != hipSuccess ||
+            hipMalloc(&b.units, sizeof(int4) * (pc / 16 + h->nblocks + 1)) != hipSuccess ||
+            hipMalloc(&b.n_units, sizeof(int)) != hipSuccess ||
+            hipMalloc(&h->redo, sizeof(int) * pc) != hipSuccess || hipMalloc(&h->n_redo, sizeof(int)) != hipSuccess ||
+            hipMalloc(&b.temp, std::max<size_t>(b.temp_bytes, 16)) != hipSuccess)
+            return bail(fail(BM_ERR_OOM, "fast-mode scratch"));
+    }
     if (hipMalloc(&h->bever, h->nblocks) != hipSuccess) return bail(fail(BM_ERR_OOM, "block flags"));
     hipMemset(h->bever, 0, h->nblocks);
     h->max_partials = (int)((h->nblocks + 3) / 4) + 1;
@@ -199,9 +291,9 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
     if (hipMalloc(&h->d_out7, sizeof(double) * 8) != hipSuccess) return bail(fail(BM_ERR_OOM, "coupler out"));
     if (hipMalloc(&h->d_err, sizeof(int)) != hipSuccess) return bail(fail(BM_ERR_OOM, "err flag"));
     if (hipMalloc(&h->d_scratch_u64, sizeof(unsigned long long) * 2) != hipSuccess) return bail(fail(BM_ERR_OOM, "scratch"));
-    if (hipMalloc(&h->d_fallbacks, sizeof(unsigned long long)) != hipSuccess) return bail(fail(BM_ERR_OOM, "counter"));
+    if (hipMalloc(&h->d_fallbacks, 2 * sizeof(unsigned long long)) != hipSuccess) return bail(fail(BM_ERR_OOM, "counter"));
     hipMemset(h->d_err, 0, sizeof(int));
-    hipMemset(h->d_fallbacks, 0, sizeof(unsigned long long));
+    hipMemset(h->d_fallbacks, 0, 2 * sizeof(unsigned long long));
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(BM_ERR_HIP, "device sync after allocation"));
     *out = h;
     return BM_OK;
@@ -210,6 +302,8 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
 int bm_mpm_destroy(bm_mpm_handle h) {
     if (!h) return BM_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
+    h->harvest();
+    for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
     hipFree(h->pbuf);
     free_grid(h->grid[0]);
     free_grid(h->grid[1]);
@@ -219,6 +313,20 @@ int bm_mpm_destroy(bm_mpm_handle h) {
     hipFree(h->d_err);
     hipFree(h->d_scratch_u64);
     hipFree(h->d_fallbacks);
+    hipFree(h->pbuf2);
+    hipFree(h->pid);
+    hipFree(h->pid2);
+    hipFree(h->redo);
+    hipFree(h->n_redo);
+    hipFree(h->bs.key);
+    hipFree(h->bs.slot);
+    hipFree(h->bs.bcnt);
+    hipFree(h->bs.bstart);
+    hipFree(h->bs.ucnt);
+    hipFree(h->bs.ustart);
+    hipFree(h->bs.units);
+    hipFree(h->bs.n_units);
+    hipFree(h->bs.temp);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
     return BM_OK;
@@ -231,6 +339,7 @@ int bm_mpm_synchronize(bm_mpm_handle h) {
 
 int bm_mpm_set_material(bm_mpm_handle h, float E, float nu, float rho) {
     if (int rc = check_handle(h)) return rc;
+    if (E != h->E || nu != h->nu || rho != h->rho) h->prepared = false;  // the deposited stress is stale
     h->E = E;
     h->nu = nu;
     if (rho > 1e-12f && h->rho > 1e-12f && rho != h->rho && h->n > 0) {
@@ -291,6 +400,17 @@ static int upload_range(bm_mpm_handle h, int64_t off, int64_t n, const float* x,
 
 static void refresh_quantum(bm_mpm_handle h) { h->q_exp = quantum_exp_for(h->max_mass); }
 
+// The particle arrays were rewritten from outside: the stored order is the
+// caller's order again and nothing precomputed from the old state is valid.
+// `first` is the first slot whose identity is new (slots before it keep
+// theirs: add_particles appends behind the binned particles).
+static void particles_changed(bm_mpm_handle h, int64_t first = 0) {
+    h->phase = PH_EMPTY;
+    h->prepared = false;
+    h->binned = false;
+    if (h->fast && h->n > first) launch_iota(h->pid + first, h->n - first, first, h->stream);
+}
+
 int bm_mpm_add_particles(bm_mpm_handle h, int64_t n, const float* pos, const float* vel, const float* mass) {
     if (int rc = check_handle(h)) return rc;
     if (n < 0 || (n > 0 && (!pos || !vel || !mass))) return fail(BM_ERR_INVALID_ARG, "bad particle arrays");
@@ -303,9 +423,10 @@ int bm_mpm_add_particles(bm_mpm_handle h, int64_t n, const float* pos, const flo
         h->max_mass = std::max(h->max_mass, mass[i]);
     }
     if (int rc = upload_range(h, h->n, n, pos, vel, F.data(), Cm.data(), mass, V0.data())) return rc;
+    const int64_t first = h->n;
     h->n += n;
     refresh_quantum(h);
-    h->phase = PH_EMPTY;
+    particles_changed(h, first);
     return BM_OK;
 }
 
@@ -321,7 +442,7 @@ int bm_mpm_upload_particles(bm_mpm_handle h, int64_t n, const bm_particles* p) {
         if (int rc = upload_range(h, 0, n, p->x, p->v, p->F, p->C, p->m, p->V0)) return rc;
     h->n = n;
     refresh_quantum(h);
-    h->phase = PH_EMPTY;
+    particles_changed(h);
     return BM_OK;
 }
 
@@ -332,10 +453,20 @@ int bm_mpm_download_particles(bm_mpm_handle h, bm_particles* p) {
     const int64_t n = h->n;
     if (n == 0) return BM_OK;
     std::vector<float> tmp((size_t)n);
+    // fast mode stores particles binned by block: slot i holds particle pid[i]
+    std::vector<int> order;
+    if (h->fast) {
+        order.resize((size_t)n);
+        HIP_TRY(hipMemcpy(order.data(), h->pid, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
     auto get = [&](int comp, float* dst, int stride, int k) -> int {
         if (!dst) return BM_OK;
         HIP_TRY(hipMemcpy(tmp.data(), h->pbuf + comp * h->cap, sizeof(float) * n, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < n; ++i) dst[i * stride + k] = tmp[(size_t)i];
+        if (h->fast) {
+            for (int64_t i = 0; i < n; ++i) dst[(int64_t)order[(size_t)i] * stride + k] = tmp[(size_t)i];
+        } else {
+            for (int64_t i = 0; i < n; ++i) dst[i * stride + k] = tmp[(size_t)i];
+        }
         return BM_OK;
     };
     for (int k = 0; k < 3; ++k)
@@ -382,17 +513,80 @@ int bm_mpm_commit_device_particles(bm_mpm_handle h, int64_t n) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     std::memcpy(&h->max_mass, &bits, sizeof(float));
     refresh_quantum(h);
-    h->phase = PH_EMPTY;
+    particles_changed(h);
     return BM_OK;
 }
 
 // ------------------------------------------------------------------ substep
+// Fast mode: re-bin the particles by grid block (swaps the particle buffers).
+static int rebin(bm_mpm_handle h) {
+    h->begin(BM_PHASE_SORT);
+    const int rc = launch_rebin(h->pview(), h->pbuf2, h->pid, h->pid2, h->dom, (int)h->nblocks, h->unit_p, h->bs,
+                                h->stream, &h->n_units);
+    h->end();
+    if (rc != 0) return fail(BM_ERR_HIP, "particle re-binning failed: %s", hipGetErrorString(hipGetLastError()));
+    std::swap(h->pbuf, h->pbuf2);
+    std::swap(h->pid, h->pid2);
+    h->binned = true;
+    h->since_bin = 0;
+    return BM_OK;
+}
+
+static int p2g_fast(bm_mpm_handle h, float dt) {
+    if (h->prepared && dt == h->dt_prepared) {  // deposited by the previous fused G2P
+        h->cur = h->prep;
+        h->prepared = false;
+        h->phase = PH_P2G;
+        return BM_OK;
+    }
+    if (!h->binned)
+        if (int rc = rebin(h)) return rc;
+    GridBuf& g = h->grid[h->cur];
+    h->begin(BM_PHASE_CLEAR);
+    launch_clear(g, (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream);
+    h->end();
+    h->begin(BM_PHASE_P2G);
+    launch_fused(false, h->pview(), h->bs.units, h->n_units, g, g, h->bever, h->dom, h->phys(), dt, dt,
+                 h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant, h->stream);
+    h->end();
+    HIP_TRY(hipGetLastError());
+    h->prepared = false;
+    h->phase = PH_P2G;
+    return BM_OK;
+}
+
+static int g2p_fast(bm_mpm_handle h, float dt) {
+    // the current grid survives the re-binning: only the particle order changes
+    if (!h->binned || h->since_bin >= h->sort_every)
+        if (int rc = rebin(h)) return rc;
+    const int nxt = 1 - h->cur;
+    h->begin(BM_PHASE_CLEAR);
+    launch_clear(h->grid[nxt], (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream);
+    h->end();
+    h->begin(BM_PHASE_G2P_P2G);
+    launch_fused(true, h->pview(), h->bs.units, h->n_units, h->grid[h->cur], h->grid[nxt], h->bever, h->dom,
+                 h->phys(), dt, dt, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant,
+                 h->stream);
+    h->end();
+    HIP_TRY(hipGetLastError());
+    h->prepared = true;
+    h->dt_prepared = dt;
+    h->prep = nxt;
+    h->since_bin += 1;
+    return BM_OK;
+}
+
 int bm_mpm_p2g(bm_mpm_handle h, float dt) {
     (void)dt;  // p2g_single ignores dt (mpm_solver.h:168)
     if (int rc = check_handle(h)) return rc;
+    if (h->fast) return p2g_fast(h, dt);
     GridBuf& g = h->grid[h->cur];
+    h->begin(BM_PHASE_CLEAR);
     launch_clear(g, (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream);
+    h->end();
+    h->begin(BM_PHASE_P2G);
     launch_p2g_ref(h->pview(), g, h->bever, h->dom, h->phys(), h->qscale_inv(), h->d_err, h->stream);
+    h->end();
     HIP_TRY(hipGetLastError());
     h->phase = PH_P2G;
     return BM_OK;
@@ -401,7 +595,9 @@ int bm_mpm_p2g(bm_mpm_handle h, float dt) {
 int bm_mpm_grid_update(bm_mpm_handle h, float dt) {
     if (int rc = check_handle(h)) return rc;
     if (h->phase != PH_P2G) return fail(BM_ERR_STATE, "grid_update needs a preceding p2g");
+    h->begin(BM_PHASE_UPDATE);
     launch_update(h->grid[h->cur], (int)h->nblocks, h->dom, h->phys(), h->qscale(), dt, h->stream);
+    h->end();
     HIP_TRY(hipGetLastError());
     h->phase = PH_UPDATED;
     return BM_OK;
@@ -410,7 +606,10 @@ int bm_mpm_grid_update(bm_mpm_handle h, float dt) {
 int bm_mpm_g2p(bm_mpm_handle h, float dt) {
     if (int rc = check_handle(h)) return rc;
     if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "g2p needs an updated grid");
+    if (h->fast) return g2p_fast(h, dt);
+    h->begin(BM_PHASE_G2P);
     launch_g2p_ref(h->pview(), h->grid[h->cur], h->dom, h->phys(), dt, h->d_err, h->stream);
+    h->end();
     HIP_TRY(hipGetLastError());
     return BM_OK;
 }
@@ -488,7 +687,9 @@ int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, floa
     if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "couple needs an updated grid");
     ColliderDev c = make_collider(h, body);
     if (couple_partials_needed(c) > h->max_partials) return fail(BM_ERR_INVALID_ARG, "collider range too large");
+    h->begin(BM_PHASE_COUPLE);
     launch_couple(method, c, h->grid[h->cur], h->bever, h->dom, dt, h->d_partials, h->d_out7, h->stream);
+    h->end();
     HIP_TRY(hipGetLastError());
     double r[7];
     HIP_TRY(hipMemcpyAsync(r, h->d_out7, sizeof(double) * 7, hipMemcpyDeviceToHost, h->stream));
@@ -574,6 +775,7 @@ int bm_mpm_upload_grid(bm_mpm_handle h, const int32_t org[3], const int32_t dims
         hipFree(d_act);
         if (e != hipSuccess) return fail(BM_ERR_HIP, "grid upload: %s", hipGetErrorString(e));
     }
+    h->prepared = false;
     h->phase = PH_UPDATED;
     return BM_OK;
 }
@@ -609,6 +811,41 @@ int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallback
         unsigned long long f = 0;
         HIP_TRY(hipMemcpy(&f, h->d_fallbacks, sizeof(f), hipMemcpyDeviceToHost));
         *fp64_fallbacks = (int64_t)f;
+    }
+    return BM_OK;
+}
+
+int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_every, int32_t variant) {
+    if (int rc = check_handle(h)) return rc;
+    if (unit_particles > FUSED_UNIT_MAX || (unit_particles > 0 && unit_particles < 16))
+        return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", FUSED_UNIT_MAX);
+    if (variant > 3) return fail(BM_ERR_INVALID_ARG, "unknown variant %d", variant);
+    if (unit_particles > 0 && unit_particles != h->unit_p) {
+        h->unit_p = unit_particles;
+        h->binned = false;  // units must be rebuilt
+    }
+    if (sort_every > 0) h->sort_every = sort_every;
+    if (variant >= 0) h->variant = variant;
+    return BM_OK;
+}
+
+int bm_mpm_set_timing(bm_mpm_handle h, int32_t on) {
+    if (int rc = check_handle(h)) return rc;
+    h->timing = on != 0;
+    return BM_OK;
+}
+
+int bm_mpm_read_timing(bm_mpm_handle h, int32_t reset, double* ms, int64_t* launches) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->harvest();
+    for (int i = 0; i < BM_NUM_PHASES; ++i) {
+        if (ms) ms[i] = h->t_ms[i];
+        if (launches) launches[i] = h->t_calls[i];
+        if (reset) {
+            h->t_ms[i] = 0.0;
+            h->t_calls[i] = 0;
+        }
     }
     return BM_OK;
 }

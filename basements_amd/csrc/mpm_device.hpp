@@ -66,6 +66,10 @@ struct Stencil {
     float w[3][3];
 };
 __device__ __forceinline__ void make_stencil(float x, float y, float z, float inv_dx, Stencil& s) {
+    // Rounded exactly as the reference rounds it in every build mode: the
+    // weights decide the fixed-point mass deposit, which must not depend on
+    // whether the surrounding kernel allows FMA contraction.
+#pragma clang fp contract(off)
     const float p[3] = {x, y, z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -198,8 +202,9 @@ __device__ __forceinline__ m3<float> hencky_from_svd(const m3<float>& U, v3<floa
 }
 
 // Drucker-Prager projection in log-singular-value space (mpm_solver.h:125-162):
-// returns the projected singular values.
-__device__ __forceinline__ v3<float> dp_project(v3<float> s, const Material& M) {
+// returns the projected singular values; *lns receives their logarithms
+// (the Hencky strain of the projected state, reused for the next stress).
+__device__ __forceinline__ v3<float> dp_project(v3<float> s, const Material& M, v3<float>* lns = nullptr) {
     s.x = fmaxf(s.x, 1e-6f);
     s.y = fmaxf(s.y, 1e-6f);
     s.z = fmaxf(s.z, 1e-6f);
@@ -208,15 +213,34 @@ __device__ __forceinline__ v3<float> dp_project(v3<float> s, const Material& M) 
     const float t3 = tr / 3.0f;
     v3<float> eh = mk(e.x - t3, e.y - t3, e.z - t3);
     const float H = sqrtf(dot(eh, eh));
-    if (tr > 0.0f) return mk(1.0f, 1.0f, 1.0f);
+    if (tr > 0.0f) {
+        if (lns) *lns = mk(0.0f, 0.0f, 0.0f);
+        return mk(1.0f, 1.0f, 1.0f);
+    }
     const float limit = -M.alpha * M.kohesion * tr;
     if (H > limit) {
         const float sc = limit / H;
         eh = eh * sc;
         e = mk(eh.x + t3, eh.y + t3, eh.z + t3);
+        if (lns) *lns = e;
         return mk(expf(e.x), expf(e.y), expf(e.z));
     }
+    if (lns) *lns = e;
     return s;
+}
+
+// Kirchhoff stress from U and the Hencky strain ln(sigma) directly.
+__device__ __forceinline__ m3<float> hencky_from_log(const m3<float>& U, v3<float> l, float mu, float lam) {
+    const float tr = l.x + l.y + l.z;
+    const float t0 = 2.0f * mu * l.x + lam * tr, t1 = 2.0f * mu * l.y + lam * tr, t2 = 2.0f * mu * l.z + lam * tr;
+    m3<float> r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float a0 = U.a[i * 3] * t0, a1 = U.a[i * 3 + 1] * t1, a2 = U.a[i * 3 + 2] * t2;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.a[i * 3 + j] = a0 * U.a[j * 3] + a1 * U.a[j * 3 + 1] + a2 * U.a[j * 3 + 2];
+    }
+    return r;
 }
 
 // (U diag(s)) V^T

@@ -1,0 +1,224 @@
+// mpm_fast_kernels.hip -- particle re-binning and the FP64 redo pass of the
+// fast substep (BM_MODE_FAST).
+//
+// Particles are kept ordered by the 4^3-node grid block that holds their
+// stencil base floor(x/dx - 1/2); every few substeps they are re-binned by a
+// counting sort (count per block, exclusive scan, scatter) and the work units
+// (up to unit_p particles of one block) are rebuilt.  The fused substep
+// kernel itself lives in mpm_fused.hip.
+
+#include <hipcub/hipcub.hpp>
+
+#include "mpm_fast_device.hpp"
+
+namespace bm {
+
+// --------------------------------------------------------------- binning
+__global__ void k_fill_i32(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Block of each particle and its rank inside the block.  Consecutive lanes
+// mostly share a block, so the lanes of a wave agree on one counter
+// increment per distinct block (the leader adds the group size) instead of
+// one contended atomic per particle.
+__global__ void __launch_bounds__(256) k_bin(PView P, Domain d, int* bcnt, int* key, int* slot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < P.n;
+    int b = -1;
+    if (live) {
+        const float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
+        const int lx = (int)floorf(x * d.inv_dx - 0.5f) - d.ox;
+        const int ly = (int)floorf(y * d.inv_dx - 0.5f) - d.oy;
+        const int lz = (int)floorf(z * d.inv_dx - 0.5f) - d.oz;
+        // out-of-domain particles go to the nearest block; the substep
+        // kernels detect and report them
+        const int bx = clampi(lx >> 2, 0, d.nbx - 1), by = clampi(ly >> 2, 0, d.nby - 1),
+                  bz = clampi(lz >> 2, 0, d.nbz - 1);
+        b = (bx * d.nby + by) * d.nbz + bz;
+    }
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long pending = __ballot(live);
+    int my_slot = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int lb = __shfl(b, leader);
+        const unsigned long long grp = __ballot(live && b == lb);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&bcnt[lb], __popcll(grp));
+        base = __shfl(base, leader);
+        if (live && b == lb) my_slot = base + __popcll(grp & below);
+        pending &= ~grp;
+    }
+    if (live) {
+        key[i] = b;
+        slot[i] = my_slot;
+    }
+}
+
+__global__ void k_reorder(PView src, float* dst, const int* key, const int* slot, const int* bstart,
+                          const int* pid_src, int* pid_dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= src.n) return;
+    const int64_t j = (int64_t)bstart[key[i]] + slot[i];
+#pragma unroll 2
+    for (int c = 0; c < PCOMP; ++c) dst[c * src.cap + j] = src.p[c * src.cap + i];
+    pid_dst[j] = pid_src[i];
+}
+
+__global__ void k_units_count(const int* bcnt, int nblocks, int unit_p, int* ucnt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nblocks) ucnt[b] = (bcnt[b] + unit_p - 1) / unit_p;
+}
+
+__global__ void k_units_fill(const int* bcnt, const int* bstart, const int* ucnt, const int* ustart, int nblocks,
+                             int unit_p, int4* units, int* n_units) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const int c = bcnt[b], nu = ucnt[b], u0 = ustart[b], s0 = bstart[b];
+    for (int u = 0; u < nu; ++u) {
+        const int cnt = min(unit_p, c - u * unit_p);
+        units[u0 + u] = make_int4(b, s0 + u * unit_p, cnt, 0);
+    }
+    if (b == nblocks - 1) *n_units = u0 + nu;
+}
+
+__global__ void k_iota(int* p, int64_t n, int64_t first) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int)(first + i);
+}
+
+// Particles whose FP32 SVD lost orthogonality: the whole particle update is
+// redone here with the FP64 Jacobi, reading the grid from global memory and
+// depositing with global atomics.  One lane per listed particle.
+template <bool DO_G2P>
+__global__ void __launch_bounds__(64) k_redo(PView P, GridBuf gin, GridBuf gout, uint8_t* bever, Domain d, Phys ph,
+                                             FastArgs fa, int* err, unsigned long long* counters) {
+    const int cnt = *fa.n_redo;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
+        const int64_t i = fa.redo[r];
+        float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
+        m3<float> F;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) F.a[c] = P.p[(PF + c) * P.cap + i];
+        v3<float> vel, lns;
+        m3<float> Cm, Us;
+        const float dx = d.dx, inv_dx = d.inv_dx;
+        if (DO_G2P) {
+            Stencil s;
+            make_stencil(x, y, z, inv_dx, s);
+            const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+            gather(s, dx, 4.0f * inv_dx * inv_dx,
+                   [&](int a, int b, int c) { return gin.mv[node_of(d, lx + a, ly + b, lz + c)]; }, vel, Cm);
+            const float dt = fa.dt_g2p;
+            x += vel.x * dt;
+            y += vel.y * dt;
+            z += vel.z * dt;
+            m3<float> A;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) A.a[c] = ((c % 4 == 0) ? 1.0f : 0.0f) + Cm.a[c] * dt;
+            const m3<float> Ftr = mmul(A, F);
+            m3<float> V;
+            v3<float> sg;
+            svd_guarded(Ftr, Us, sg, V);
+            const Material M{ph.mu, ph.lambda, ph.alpha, ph.kohesion};
+            const v3<float> sn = dp_project(sg, M, &lns);
+            F = recompose(Us, sn, V);
+            if (y < 0.0f) {
+                y = 0.0f;
+                vel.y = 0.0f;
+            }
+            P.p[(PX + 0) * P.cap + i] = x;
+            P.p[(PX + 1) * P.cap + i] = y;
+            P.p[(PX + 2) * P.cap + i] = z;
+            P.p[(PV + 0) * P.cap + i] = vel.x;
+            P.p[(PV + 1) * P.cap + i] = vel.y;
+            P.p[(PV + 2) * P.cap + i] = vel.z;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                P.p[(PF + c) * P.cap + i] = F.a[c];
+                P.p[(PC + c) * P.cap + i] = Cm.a[c];
+            }
+        } else {
+            vel = mk(P.p[(PV + 0) * P.cap + i], P.p[(PV + 1) * P.cap + i], P.p[(PV + 2) * P.cap + i]);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) Cm.a[c] = P.p[(PC + c) * P.cap + i];
+            m3<float> V;
+            v3<float> sg;
+            svd_guarded(F, Us, sg, V);
+            lns = mk(logf(fmaxf(sg.x, 1e-6f)), logf(fmaxf(sg.y, 1e-6f)), logf(fmaxf(sg.z, 1e-6f)));
+        }
+        const float mass = P.p[PM * P.cap + i];
+        const float V0 = P.p[PV0 * P.cap + i];
+        v3<float> mvel;
+        m3<float> Aff;
+        p2g_coeffs(mass, V0, vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
+        Stencil s;
+        make_stencil(x, y, z, inv_dx, s);
+        const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+        if (!stencil_ok(d, lx, ly, lz)) {
+            atomicOr(err, 1);
+            continue;
+        }
+        const double qp = rint((double)mass * fa.qscale_inv);
+        scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+            if (w != 0.0f) deposit_global(gout, bever, d, lx + a, ly + b, lz + c, dq, mom);
+        });
+        atomicAdd(&counters[0], 1ull);
+    }
+}
+
+__global__ void k_zero_i32(int* p) { *p = 0; }
+
+// ================================================================ launchers
+static inline int grid_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+size_t scan_temp_bytes(int n) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int*)nullptr, (int*)nullptr, n);
+    return bytes;
+}
+
+// Re-bin the particles by grid block and rebuild the work units.  Returns the
+// unit count on the host (one stream synchronisation per re-binning).
+int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, const Domain& d, int nblocks,
+                 int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nblocks, 256)), dim3(256), 0, s, bs.bcnt, nblocks, 0);
+    if (P.n > 0)
+        hipLaunchKernelGGL(k_bin, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, d, bs.bcnt, bs.key, bs.slot);
+    size_t tb = bs.temp_bytes;
+    (void)hipcub::DeviceScan::ExclusiveSum(bs.temp, tb, bs.bcnt, bs.bstart, nblocks, s);
+    if (P.n > 0)
+        hipLaunchKernelGGL(k_reorder, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, pdst, bs.key, bs.slot, bs.bstart,
+                           pid_src, pid_dst);
+    hipLaunchKernelGGL(k_units_count, dim3(grid_for(nblocks, 256)), dim3(256), 0, s, bs.bcnt, nblocks, unit_p,
+                       bs.ucnt);
+    tb = bs.temp_bytes;
+    (void)hipcub::DeviceScan::ExclusiveSum(bs.temp, tb, bs.ucnt, bs.ustart, nblocks, s);
+    hipLaunchKernelGGL(k_units_fill, dim3(grid_for(nblocks, 256)), dim3(256), 0, s, bs.bcnt, bs.bstart, bs.ucnt,
+                       bs.ustart, nblocks, unit_p, bs.units, bs.n_units);
+    if (hipMemcpyAsync(n_units_host, bs.n_units, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    return 0;
+}
+
+void launch_iota(int* p, int64_t n, int64_t first, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, first);
+}
+
+void launch_redo(bool do_g2p, const PView& P, const GridBuf& gin, const GridBuf& gout, uint8_t* bever,
+                 const Domain& d, const Phys& ph, const FastArgs& fa, int* err, unsigned long long* counters,
+                 hipStream_t s) {
+    if (do_g2p)
+        hipLaunchKernelGGL(k_redo<true>, dim3(32), dim3(64), 0, s, P, gin, gout, bever, d, ph, fa, err, counters);
+    else
+        hipLaunchKernelGGL(k_redo<false>, dim3(32), dim3(64), 0, s, P, gin, gout, bever, d, ph, fa, err, counters);
+}
+
+void launch_zero_i32(int* p, hipStream_t s) { hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, p); }
+
+}  // namespace bm

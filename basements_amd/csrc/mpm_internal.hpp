@@ -90,4 +90,28 @@ void launch_mass_total(const GridBuf& g, unsigned long long* out, hipStream_t s)
 void launch_scale_v0(const PView& P, float rescale, hipStream_t s);
 void launch_max_mass(const PView& P, unsigned int* out, hipStream_t s);
 
+// ------------------------------------------------------------- fast path
+// Scratch of the particle re-binning (all device memory, sized at create).
+struct BinScratch {
+    int* bcnt;      // particles per block
+    int* bstart;    // exclusive scan of bcnt
+    int* ucnt;      // units per block
+    int* ustart;    // exclusive scan of ucnt
+    int* key;       // block of each particle
+    int* slot;      // rank of each particle inside its block
+    int4* units;    // (block, first particle, count, 0)
+    int* n_units;   // device copy of the unit count
+    void* temp;     // hipcub scan scratch
+    size_t temp_bytes;
+};
+size_t scan_temp_bytes(int n);
+int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, const Domain& d, int nblocks,
+                 int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host);
+void launch_iota(int* p, int64_t n, int64_t first, hipStream_t s);
+void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
+                  const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
+                  double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, int variant,
+                  hipStream_t s);
+constexpr int FUSED_UNIT_MAX = 1024;  // largest unit the fused kernel accepts
+
 }  // namespace bm

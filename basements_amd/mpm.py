@@ -128,7 +128,12 @@ _SIGS = {
     "bm_mpm_grid_mass_total": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "bm_mpm_mass_quantum_exp": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "bm_mpm_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "bm_mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "bm_mpm_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "bm_mpm_read_timing": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
 }
+
+PHASES = ("clear", "p2g", "update", "couple", "g2p", "g2p_p2g", "sort", "other")
 
 
 def header_symbols(path: str = INCLUDE_H) -> list[str]:
@@ -314,6 +319,20 @@ class Solver:
         e = C.c_int32()
         _check(lib().bm_mpm_mass_quantum_exp(self.h, C.byref(e)))
         return e.value
+
+    def set_tuning(self, unit_particles=0, sort_every=0, variant=-1):
+        """Fast-mode launch parameters (bm_mpm_set_tuning); 0 / -1 keep the current value."""
+        _check(lib().bm_mpm_set_tuning(self.h, int(unit_particles), int(sort_every), int(variant)))
+
+    def set_timing(self, on=True):
+        _check(lib().bm_mpm_set_timing(self.h, 1 if on else 0))
+
+    def read_timing(self, reset=True):
+        """{phase: (total_ms, launches)} from HIP events on the handle's stream."""
+        ms = np.zeros(len(PHASES), np.float64)
+        calls = np.zeros(len(PHASES), np.int64)
+        _check(lib().bm_mpm_read_timing(self.h, 1 if reset else 0, _ptr(ms), _ptr(calls)))
+        return {p: (float(ms[i]), int(calls[i])) for i, p in enumerate(PHASES)}
 
     def stats(self):
         a, f = C.c_int64(), C.c_int64()

@@ -1,0 +1,249 @@
+#!/usr/bin/env python3
+"""Throughput of the MLS-MPM soil substep on one MI355X.
+
+Workload (default): the single large scene -- a 1.0 x 0.3 x 1.0 m soil bed at
+8 particles per cell (dx = 1/120 m, 4,147,200 particles) on a 256^3-node grid,
+friction angle from U(25, 40) deg, a cylinder wheel (r 0.1 m, width 0.08 m)
+rolling at 0.2 m/s under a 50 N load through the M1 PIC-rigid coupler, dt =
+1e-4 s.  One step = one coupled substep: P2G, grid update, M1 coupler with the
+reaction returned to the wheel, wheel integration, G2P with plasticity.
+
+Prints ONE JSON line: particle-updates/s (value), the roofline of the
+dominant kernel (HIP events on the handle's stream), and a CPU baseline (the
+reference's own MPMSolver::step built from its sources, one thread, on a
+bounded sample of the same bed).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic bytes of each kernel: per particle and per active node.
+#   P2G reads x v C F m V0 (104 B) and adds m, momentum and force into each
+#   active node (28 B); the update reads those 28 B and writes v (12 B); G2P
+#   reads x F (48 B) and v of each active node (12 B) and writes x v C F (96 B).
+#   Total: 200 B per particle + 80 B per active node per substep.
+BYTES_REF = {
+    "p2g": (104, 28),
+    "update": (0, 40),
+    "g2p": (144, 12),
+}
+# Fast mode folds the stress force into the momentum (56 B per active node
+# per substep instead of 80): P2G deposits m + momentum (16 B), the update
+# reads 16 B and writes 12 B, G2P reads 12 B.  The fused G2P(n)+P2G(n+1)
+# kernel reads x F m V0 (56 B), writes x v C F (96 B) per particle, reads v
+# and deposits m + momentum per node.
+BYTES_FAST = {
+    "p2g": (104, 16),
+    "update": (0, 28),
+    "g2p_p2g": (152, 28),
+}
+
+
+def _args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="wheel_bed", choices=["wheel_bed", "box_drop"])
+    ap.add_argument("--mode", default="fast", choices=["reference", "fast"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of known size (for HBM counter scaling)")
+    return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+
+    n = 256 << 20  # 256 Mi floats = 1 GiB each way
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 5}))
+
+
+class Wheel:
+    """Cylinder wheel: driven at a fixed forward speed and rolling rate,
+    free vertically under its weight (the load) and the soil reaction;
+    semi-implicit Euler as TimeIntegrator::integrate (integrator.h:19-57)."""
+
+    def __init__(self, tool, g=9.81):
+        self.R = tool["radius"]
+        self.L = tool["half_len"]
+        self.mass = tool["load"] / g
+        self.g = g
+        self.x = np.array(tool["center"], np.float64)
+        self.v = np.array([tool["speed"], 0.0, 0.0])
+        self.w = np.array([0.0, 0.0, -tool["speed"] / self.R])
+        self.q = (math.cos(math.pi / 4), math.sin(math.pi / 4), 0.0, 0.0)  # local Y (axle) -> world Z
+        self.friction = tool["friction"]
+
+    def collider(self):
+        from basements_amd import Collider, SHAPE_CYLINDER
+
+        return Collider(center=tuple(self.x), quat=self.q, half_extents=(self.R, self.L, self.R),
+                        lin_vel=tuple(self.v), ang_vel=tuple(self.w), friction=self.friction, mass=self.mass,
+                        shape=SHAPE_CYLINDER)
+
+    def integrate(self, force, dt):
+        fy = force[1] - self.mass * self.g
+        self.v[1] += fy / self.mass * dt
+        self.x += self.v * dt
+
+
+def cpu_baseline(scene, nsteps):
+    """Reference MPMSolver::step (oracle/_ref, built from the reference's own
+    sources) on a quarter of the bed, one thread; falls back to the oracle
+    restatement when that build is absent."""
+    from oracle.pyoracle import OracleSim, Params, RefSim, ref_available
+
+    x = scene.particles["x"]
+    lo = x[:, 0].min()
+    sel = x[:, 0] < lo + 0.25 * (x[:, 0].max() - lo)
+    P = {k: np.asarray(v[sel], np.float64) for k, v in scene.particles.items()}
+    prm = Params(E=scene.E, nu=scene.nu, rho=scene.rho, alpha=scene.alpha, mu_floor=scene.mu_floor,
+                 gravity=scene.gravity)
+    if ref_available():
+        sim, kind = RefSim(scene.dx, params=prm), "reference"
+    else:
+        sim, kind = OracleSim(scene.dx, "f32", params=prm), "port"
+    sim.set_particles(P)
+    sim.step(scene.dt, 1)  # warm-up: allocates the sparse blocks
+    t0 = time.perf_counter()
+    sim.step(scene.dt, nsteps)
+    t = time.perf_counter() - t0
+    n = int(P["x"].shape[0])
+    return {"value": n * nsteps / t, "unit": "particle-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{n} particles (the quarter of the bed with the lowest x), {nsteps} substeps of "
+                      f"MPMSolver::step after 1 warm-up, dt {scene.dt}, single thread",
+            "seconds": t}
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (this build has no multi-GPU path)",
+              file=sys.stderr)
+        return 2
+    if a.copy_calibration:
+        copy_calibration()
+        return 0
+
+    from basements_amd import M0, M1, MODE_FAST, MODE_REFERENCE, Solver
+    from basements_amd.scenes import box_drop, wheel_bed
+
+    scene = wheel_bed(a.seed) if a.config == "wheel_bed" else box_drop(a.seed)
+    mode = MODE_FAST if a.mode == "fast" else MODE_REFERENCE
+    S = Solver(scene.dx, scene.origin, scene.dims, scene.n, mode)
+    S.set_material(scene.E, scene.nu, scene.rho)
+    S.set_plastic_alpha(scene.alpha)
+    S.set_floor_friction(scene.mu_floor)
+    S.set_gravity(scene.gravity)
+    S.upload(scene.particles)
+    S.synchronize()
+
+    dt = scene.dt
+    tool = scene.tool
+    wheel = Wheel(tool) if tool["kind"] == "cylinder" else None
+    method = M1 if tool["method"] == 1 else M0
+
+    if wheel is None:
+        from basements_amd import Collider
+
+        box = Collider(center=tool["center"], half_extents=tool["half"], friction=tool["friction"], mass=1.0)
+
+    def substep():
+        S.p2g(dt)
+        S.grid_update(dt)
+        if wheel is not None:
+            f, _, _ = S.couple(method, wheel.collider(), dt)
+            wheel.integrate(f, dt)
+        else:
+            S.couple(method, box, dt)
+        S.g2p(dt)
+
+    for _ in range(a.warmup):
+        substep()
+    S.synchronize()
+    S.set_timing(True)
+    S.read_timing(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        substep()
+    S.synchronize()
+    t = time.perf_counter() - t0
+    tm = S.read_timing(reset=True)
+    S.set_timing(False)
+
+    # active nodes of the last substep (mass > 0)
+    G = S.download_grid()
+    n_nodes = int((G["mass_q"] != 0).sum())
+    del G
+    st = S.stats()
+    n = scene.n
+    value = n * a.steps / t
+
+    per_kernel = {}
+    node_bytes = 80 if mode == MODE_REFERENCE else 56
+    for k, (bp, bn) in (BYTES_REF if mode == MODE_REFERENCE else BYTES_FAST).items():
+        ms, calls = tm.get(k, (0.0, 0))
+        if calls:
+            avg_s = ms / calls * 1e-3
+            algo = bp * n + bn * n_nodes
+            per_kernel[k] = {"avg_ms": ms / calls, "launches": calls, "algo_bytes": algo,
+                             "achieved_GBs": algo / avg_s / 1e9}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
+    ach = per_kernel[dom]["achieved_GBs"]
+    sub_bytes = 200 * n + node_bytes * n_nodes
+    gpu_ms = sum(ms for ms, _ in tm.values()) / a.steps
+
+    out = {
+        "metric": "particle-updates/s",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded jittered soil bed, no external data)",
+        "config": {"workload": scene.name, "particles": n, "grid": list(scene.dims), "dx": scene.dx,
+                   "dt": dt, "coupler": "M1" if method == M1 else "M0", "mode": a.mode,
+                   "active_nodes": n_nodes, "active_blocks": st["active_blocks"]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "algo_bytes_per_launch": per_kernel[dom]["algo_bytes"]},
+        "substep_roofline": {"algo_bytes": sub_bytes, "bytes_per_node": node_bytes, "gpu_ms": gpu_ms,
+                             "achieved_GBs": sub_bytes / (gpu_ms * 1e-3) / 1e9,
+                             "frac": sub_bytes / (gpu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "kernels": per_kernel,
+        "fp64_svd_fallbacks": st["fp64_fallbacks"],
+    }
+    if a.cpu_baseline:
+        cb = cpu_baseline(scene, a.cpu_steps)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -189,6 +189,35 @@ int bm_mpm_mass_quantum_exp(bm_mpm_handle h, int32_t* e);
  * FP64 fallback since creation. */
 int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallbacks);
 
+/* ---------------------------------------------------------------- tuning */
+/* BM_MODE_FAST launch parameters (what the autotuner searches):
+ * unit_particles: particles per workgroup unit (16..1024, default 512);
+ * sort_every: substeps between particle re-binnings (>= 1, default 16);
+ * variant: bit 0 = plain read-modify-write LDS deposit (else collision-free
+ * LDS atomics), bit 1 = stage G2P node velocities in LDS.  Values <= 0 /
+ * < 0 keep the current setting.  No effect in BM_MODE_REFERENCE. */
+int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_every, int32_t variant);
+
+/* ---------------------------------------------------------------- timing */
+/* Per-phase device time from HIP events recorded on the handle's stream
+ * around every kernel launch while timing is on (the reference times only
+ * wall-clock per coupler step, research/mpm_rigid_coupling/metrics.h:122-132). */
+enum bm_phase {
+    BM_PHASE_CLEAR = 0,   /* zero the blocks of the previous substep */
+    BM_PHASE_P2G = 1,     /* particle-to-grid scatter */
+    BM_PHASE_UPDATE = 2,  /* grid update (gravity, floor, walls) */
+    BM_PHASE_COUPLE = 3,  /* M0 / M1 coupler and its reaction reduction */
+    BM_PHASE_G2P = 4,     /* grid-to-particle gather + plasticity */
+    BM_PHASE_G2P_P2G = 5, /* fused G2P(n) + P2G(n+1) (BM_MODE_FAST) */
+    BM_PHASE_SORT = 6,    /* particle re-binning (BM_MODE_FAST) */
+    BM_PHASE_OTHER = 7,
+    BM_NUM_PHASES = 8
+};
+int bm_mpm_set_timing(bm_mpm_handle h, int32_t on);
+/* Synchronises, then returns the accumulated milliseconds and launch counts
+ * per phase (arrays of BM_NUM_PHASES); reset != 0 clears the accumulators. */
+int bm_mpm_read_timing(bm_mpm_handle h, int32_t reset, double* ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,147 @@
+"""BM_MODE_FAST (fused G2P+P2G, block-binned particles, stress from the
+plasticity SVD, force folded into momentum, fast reciprocal/sqrt/log/exp)
+against the oracle.
+
+Bars:
+  * fixed-point node mass of a P2G from identical particle state: bit-exact;
+  * total grid mass every substep: exactly the particles' quanta;
+  * folded momentum vs the oracle's momentum + dt * force: rtol 2e-5 of the
+    field's largest magnitude;
+  * trajectories: measured against the FP64 oracle, the fast path must stay
+    within 2.5x the distance of the FP32 oracle (the reference's own
+    arithmetic) from that same FP64 run -- i.e. at the FP32 noise floor,
+    which over 20 substeps of these scenes is ~1.6e-4 relative in v;
+  * particles come back in insertion order after any number of re-binnings.
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import OracleSim
+from tests._util import gpu_solver, oracle_for, params_from, random_scene, rel_err
+
+pytestmark = pytest.mark.gpu
+
+DX = 1.0 / 32.0
+ORG = (0, -4, 0)
+DIMS = (24, 16, 24)
+
+
+def _pair(seed, sort_every=16, variant=None, unit=0, **kw):
+    from basements_amd import MODE_FAST, Solver
+
+    P = random_scene(seed, dx=DX, **kw)
+    prm = params_from()
+    g = Solver(DX, ORG, DIMS, P["x"].shape[0], MODE_FAST, sort_every=sort_every)
+    if variant is not None or unit:
+        g.set_tuning(unit, 0, -1 if variant is None else variant)
+    g.set_material(prm.E, prm.nu, prm.rho)
+    g.set_plastic_alpha(prm.alpha)
+    g.set_floor_friction(prm.mu_floor)
+    g.set_gravity(prm.gravity)
+    g.upload(P)
+    o = oracle_for(P, DX, prm, g.mass_quantum_exp())
+    return P, prm, g, o
+
+
+def _fp64_pair(P, prm):
+    Pd = {k: np.asarray(v, np.float64) for k, v in P.items()}
+    o64 = OracleSim(DX, "f64", params=prm)
+    o32 = OracleSim(DX, "f32", params=prm)
+    o64.set_particles(Pd)
+    o32.set_particles(Pd)
+    return o32, o64
+
+
+def _assert_at_noise_floor(Pg, P32, P64, fields="xvFC", factor=2.5):
+    for k in fields:
+        floor = rel_err(P32[k], P64[k])
+        got = rel_err(Pg[k], P64[k])
+        assert got <= factor * floor + 1e-6, (k, got, floor)
+
+
+def test_fast_p2g_mass_bit_exact_and_folded_momentum():
+    P, prm, g, o = _pair(11)
+    dt = 1e-4
+    g.p2g(dt)
+    o.p2g()
+    G = g.download_grid()
+    O = o.grid_dense(g.origin, g.dims)
+    assert np.array_equal(G["mass_q"], O["mass_q"])
+    assert np.array_equal(G["active"], O["active"])
+    assert g.grid_mass_total() == o.grid_mass_q_total()
+    want = O["vel"] + dt * O["force"]
+    err = np.abs(G["vel"] - want).max() / np.abs(want).max()
+    assert err < 2e-5, err
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_fast_steps_at_fp32_noise_floor(variant):
+    P, prm, g, _ = _pair(12, variant=variant, strain=0.01, vel=0.05, cgrad=0.2)
+    o32, o64 = _fp64_pair(P, prm)
+    e = g.mass_quantum_exp()
+    q = int(np.rint(np.ldexp(P["m"].astype(np.float64), -e)).sum())
+    dt = 1e-4
+    for _ in range(20):
+        g.p2g(dt)
+        assert g.grid_mass_total() == q
+        g.grid_update(dt)
+        g.g2p(dt)
+    o32.step(dt, 20)
+    o64.step(dt, 20)
+    _assert_at_noise_floor(g.download(), o32.get_particles(), o64.get_particles())
+
+
+def test_fast_order_survives_rebinning():
+    P, prm, g, _ = _pair(13, sort_every=3, unit=64, strain=0.01, vel=0.05, cgrad=0.2)
+    o32, o64 = _fp64_pair(P, prm)
+    dt = 1e-4
+    g.step(dt, 10)
+    o32.step(dt, 10)
+    o64.step(dt, 10)
+    Pg = g.download()
+    assert np.array_equal(Pg["m"], P["m"]) and np.array_equal(Pg["V0"], P["V0"])
+    _assert_at_noise_floor(Pg, o32.get_particles(), o64.get_particles())
+
+
+def test_fast_matches_reference_mode_over_100_substeps():
+    from basements_amd import MODE_REFERENCE
+
+    P, prm, g, _ = _pair(14, strain=0.0, vel=0.0, cgrad=0.0)
+    r = gpu_solver(P, DX, ORG, DIMS, prm, mode=MODE_REFERENCE)
+    o32, o64 = _fp64_pair(P, prm)
+    g.step(1e-4, 100)
+    r.step(1e-4, 100)
+    o32.step(1e-4, 100)
+    o64.step(1e-4, 100)
+    P64 = o64.get_particles()
+    _assert_at_noise_floor(g.download(), o32.get_particles(), P64, fields="xF")
+    _assert_at_noise_floor(r.download(), o32.get_particles(), P64, fields="xF")
+
+
+def test_fast_dt_change_redeposits():
+    P, prm, g, _ = _pair(15, strain=0.01, vel=0.05, cgrad=0.2)
+    o32, o64 = _fp64_pair(P, prm)
+    for dt in (1e-4, 5e-5, 5e-5, 1e-4):
+        g.step(dt)
+        o32.step(dt)
+        o64.step(dt)
+    _assert_at_noise_floor(g.download(), o32.get_particles(), o64.get_particles())
+
+
+def test_fast_material_change_redeposits():
+    """Changing E after a substep must not reuse the stress already deposited."""
+    P, prm, g, _ = _pair(16, strain=0.01, vel=0.05, cgrad=0.2)
+    o32, o64 = _fp64_pair(P, prm)
+    g.step(1e-4)
+    o32.step(1e-4)
+    o64.step(1e-4)
+    from oracle.pyoracle import Params
+
+    prm2 = Params(E=1.0e6, nu=prm.nu, rho=prm.rho, alpha=prm.alpha, mu_floor=prm.mu_floor, gravity=prm.gravity)
+    g.set_material(prm2.E, prm2.nu, prm2.rho)
+    o32.set_params(prm2)
+    o64.set_params(prm2)
+    g.step(1e-4, 3)
+    o32.step(1e-4, 3)
+    o64.step(1e-4, 3)
+    _assert_at_noise_floor(g.download(), o32.get_particles(), o64.get_particles())
