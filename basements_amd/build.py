@@ -36,6 +36,7 @@ UNITS = [
     ("mpm_ref_kernels.hip", ["-ffp-contract=off"]),
     ("mpm_fast_kernels.hip", []),
     ("mpm_fused.hip", []),
+    ("mpm_split.hip", []),
     ("bm_mpm_api.hip", ["-ffp-contract=off"]),
 ]
 

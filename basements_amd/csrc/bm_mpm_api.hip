@@ -77,13 +77,14 @@ struct bm_mpm_solver {
     int* pid2 = nullptr;
     int* redo = nullptr;    // particles redone with the FP64 SVD (one list per launch)
     int* n_redo = nullptr;
+    float* coef = nullptr;  // deposit coefficients of the two-kernel form (12 per particle)
     BinScratch bs{};
     int n_units = 0;
     bool binned = false;      // units describe the current particle order
     int since_bin = 0;        // substeps since the last re-binning
     int sort_every = 16;
     int unit_p = 512;
-    int variant = 1;          // fused-kernel variant (see bm_mpm_set_tuning): ordered read-modify-write
+    int variant = 4;          // fast-kernel variant (see bm_mpm_set_tuning): two-kernel form
     bool prepared = false;    // grid[prep] already holds the next substep's P2G
     float dt_prepared = 0.0f;
     int prep = 0;
@@ -159,12 +160,16 @@ struct bm_mpm_solver {
     double qscale_inv() const { return std::ldexp(1.0, -q_exp); }
 };
 
-// Mass quantum: 2^(ilogb(max mass) - 39) kg, so a particle of the largest
-// mass is 2^39..2^40 quanta and any particle within 2^16 of it is an exact
-// integer number of quanta.
+// Mass quantum: 2^(ilogb(max mass) - 22) kg, so a particle of the largest
+// mass is 2^22..2^23 - 1 quanta (its float mantissa exactly) and a node
+// deposit resolves 2^-23 of a particle's mass -- FP32 resolution.  Every
+// particle deposits at most 2^23 - 1 quanta, so the sum over any 512
+// particles fits in 32 bits: the fast path's LDS tiles (units of at most 512
+// particles) accumulate node mass in one 32-bit word.  Products w * q_p need
+// at most 24 + 23 bits, so the double-precision rounding of w * q_p is exact.
 static int quantum_exp_for(float max_mass) {
     if (!(max_mass > 0.0f) || !std::isfinite(max_mass)) return 0;
-    return std::ilogb(max_mass) - 39;
+    return std::ilogb(max_mass) - 22;
 }
 
 static int check_handle(bm_mpm_handle h) {
@@ -280,6 +285,7 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
             hipMalloc(&b.units, sizeof(int4) * (pc / 16 + h->nblocks + 1)) != hipSuccess ||
             hipMalloc(&b.n_units, sizeof(int)) != hipSuccess ||
             hipMalloc(&h->redo, sizeof(int) * pc) != hipSuccess || hipMalloc(&h->n_redo, sizeof(int)) != hipSuccess ||
+            hipMalloc(&h->coef, sizeof(float) * 12 * pc) != hipSuccess ||
             hipMalloc(&b.temp, std::max<size_t>(b.temp_bytes, 16)) != hipSuccess)
             return bail(fail(BM_ERR_OOM, "fast-mode scratch"));
     }
@@ -318,6 +324,7 @@ int bm_mpm_destroy(bm_mpm_handle h) {
     hipFree(h->pid2);
     hipFree(h->redo);
     hipFree(h->n_redo);
+    hipFree(h->coef);
     hipFree(h->bs.key);
     hipFree(h->bs.slot);
     hipFree(h->bs.bcnt);
@@ -532,6 +539,29 @@ static int rebin(bm_mpm_handle h) {
     return BM_OK;
 }
 
+// The fast substep's particle kernels: fused G2P+P2G, or the two-kernel form
+// (tuning variant bit 2).  do_g2p = false deposits the current state only.
+static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const GridBuf& gout, float dt_g2p,
+                         float dt_fold) {
+    if (h->variant & 4) {
+        // time the two kernels apart: the caller's phase covers the first
+        const int outer = h->open_phase;
+        launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
+                     dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream,
+                     [](void* p) {
+                         auto* hh = (bm_mpm_handle)p;
+                         if (hh->open_phase < 0) return;
+                         hh->end();
+                         hh->begin(BM_PHASE_P2G);
+                     },
+                     h);
+        (void)outer;
+    } else
+        launch_fused(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
+                     dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant & 3,
+                     h->stream);
+}
+
 static int p2g_fast(bm_mpm_handle h, float dt) {
     if (h->prepared && dt == h->dt_prepared) {  // deposited by the previous fused G2P
         h->cur = h->prep;
@@ -546,8 +576,7 @@ static int p2g_fast(bm_mpm_handle h, float dt) {
     launch_clear(g, (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream);
     h->end();
     h->begin(BM_PHASE_P2G);
-    launch_fused(false, h->pview(), h->bs.units, h->n_units, g, g, h->bever, h->dom, h->phys(), dt, dt,
-                 h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant, h->stream);
+    fast_kernels(h, false, g, g, dt, dt);
     h->end();
     HIP_TRY(hipGetLastError());
     h->prepared = false;
@@ -563,10 +592,8 @@ static int g2p_fast(bm_mpm_handle h, float dt) {
     h->begin(BM_PHASE_CLEAR);
     launch_clear(h->grid[nxt], (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream);
     h->end();
-    h->begin(BM_PHASE_G2P_P2G);
-    launch_fused(true, h->pview(), h->bs.units, h->n_units, h->grid[h->cur], h->grid[nxt], h->bever, h->dom,
-                 h->phys(), dt, dt, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant,
-                 h->stream);
+    h->begin((h->variant & 4) ? BM_PHASE_G2P : BM_PHASE_G2P_P2G);
+    fast_kernels(h, true, h->grid[h->cur], h->grid[nxt], dt, dt);
     h->end();
     HIP_TRY(hipGetLastError());
     h->prepared = true;
@@ -819,7 +846,7 @@ int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_ever
     if (int rc = check_handle(h)) return rc;
     if (unit_particles > FUSED_UNIT_MAX || (unit_particles > 0 && unit_particles < 16))
         return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", FUSED_UNIT_MAX);
-    if (variant > 3) return fail(BM_ERR_INVALID_ARG, "unknown variant %d", variant);
+    if (variant > 7) return fail(BM_ERR_INVALID_ARG, "unknown variant %d", variant);
     if (unit_particles > 0 && unit_particles != h->unit_p) {
         h->unit_p = unit_particles;
         h->binned = false;  // units must be rebuilt

@@ -9,22 +9,41 @@
 //
 // LDS accumulation without same-address collisions: the wave first ranks its
 // particles per stencil-base cell (64 cells per block = 64 lanes), then works
-// in rounds -- in round r lane c holds the r-th particle of cell c.  For a
+// in rounds -- in round r lane c holds ranks 2r and 2r+1 of cell c.  For a
 // fixed stencil offset every lane of a round then addresses a different node,
-// so the scatter either uses collision-free LDS atomics (RMW = false) or plain
-// read-modify-write in program order (RMW = true; one wave, so no other
-// writer exists).  Particles that changed cell during their G2P, drifted out
-// of the block since the last re-binning, or exceed the rank table go through
-// LDS atomics (or global atomics when their stencil left the tile).
-// GTILE stages the G2P node velocities in LDS as well; otherwise the gather
-// reads the grid through the caches.
+// so the scatter either uses collision-free LDS atomics (RMW = false) or a
+// plain read-modify-write in program order (RMW = true; one wave, so no
+// other writer exists).  The two particles of a lane share their 27 nodes
+// and are deposited with one read-modify-write per node.  Particles that
+// changed cell during their G2P, drifted out of the block since the last
+// re-binning, or exceed the rank table go through LDS atomics (or global
+// atomics when their stencil left the tile).  GTILE stages the G2P node
+// velocities in LDS as well; otherwise the gather reads the grid through the
+// caches.
 
 #include "mpm_fast_device.hpp"
 
 namespace bm {
 
-constexpr int RMAX = 24;        // particle ranks per cell held in the round table
+constexpr int RMAX = 16;        // particle ranks per cell held in the round table
 constexpr int UNIT_MAX = 1024;  // largest unit (leftover-list capacity)
+
+// LDS slot of tile node (ix, iy, iz): rows of 8 nodes along z are rotated by
+// 4 on odd y, so the 8 lanes that share a 128-byte LDS cycle in a round
+// (cells (cx, cy..cy+1, 0..3)) address 8 distinct 16-byte bank groups.
+__device__ __forceinline__ int tix(int ix, int iy, int iz) { return ((ix * TE + iy) * TE) + ((iz + 4 * (iy & 1)) & 7); }
+__device__ __forceinline__ void tile_coords(int k, int& ix, int& iy, int& iz) {
+    ix = k >> 6;
+    iy = (k >> 3) & 7;
+    iz = ((k & 7) - 4 * (iy & 1)) & 7;
+}
+
+// The fixed-point node mass is accumulated in LDS as two 32-bit sums: the
+// low 22 bits of each deposit (in the w word of the momentum record) and the
+// rest.  A unit holds at most UNIT_MAX = 1024 particles and each particle
+// deposits once per node, so neither sum can overflow.
+constexpr int MLO_BITS = 22;
+constexpr unsigned MLO_MASK = (1u << MLO_BITS) - 1u;
 
 // floor(x/dx - 1/2) rounded exactly as make_stencil rounds it.
 __device__ __forceinline__ int base_of(float x, float inv_dx) {
@@ -32,12 +51,23 @@ __device__ __forceinline__ int base_of(float x, float inv_dx) {
     return (int)floorf(x * inv_dx - 0.5f);
 }
 
+// What a particle deposits: its stencil at the new position and the folded
+// momentum coefficients.  rx..rz: stencil base relative to the tile origin.
+struct Dep {
+    Stencil s;
+    v3<float> mvel;
+    m3<float> Aff;
+    double qp;
+    int rx, ry, rz;
+    int state;  // 0 nothing to deposit, 1 tile, 2 outside the tile
+};
+
 template <bool DO_G2P, bool RMW, bool GTILE>
 struct Fused {
     struct Smem {
         float4 vt[(DO_G2P && GTILE) ? TNODES : 1];  // G2P node velocities
-        float4 pt[TNODES];                          // deposited momentum (w unused)
-        unsigned long long mt[TNODES];              // deposited fixed-point mass
+        float4 pt[TNODES];                          // deposited momentum; w = low mass bits (uint)
+        unsigned int mh[TNODES];                    // high mass bits
         unsigned short tab[RMAX][64];               // round table: tab[r][cell] = particle
         unsigned short left[UNIT_MAX];              // particles outside the table
         int cnt[64];
@@ -45,14 +75,13 @@ struct Fused {
         int tb[27];
     };
 
-    // Process particle i: G2P (when DO_G2P), plasticity, state write-back and
-    // P2G into the tile.  `ranked`: the lane holds this particle's stencil
-    // cell exclusively in the current round.  Returns the number of
-    // deposits that fell outside the tile.
-    static __device__ __forceinline__ int particle(Smem& S, int64_t i, bool ranked, int cell0, const PView& P,
-                                                   const GridBuf& gin, const GridBuf& gout, uint8_t* bever,
-                                                   const Domain& d, const Phys& ph, const FastArgs& fa, int* err,
-                                                   int tx0, int ty0, int tz0) {
+    // G2P (when DO_G2P), plasticity and write-back of particle i; returns
+    // the particle's deposit.
+    static __device__ __forceinline__ Dep advance(Smem& S, int64_t i, const PView& P, const GridBuf& gin,
+                                                  const Domain& d, const Phys& ph, const FastArgs& fa, int* err,
+                                                  int tx0, int ty0, int tz0) {
+        Dep D;
+        D.state = 0;
         const float dx = d.dx, inv_dx = d.inv_dx;
         float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
         m3<float> F;
@@ -60,20 +89,18 @@ struct Fused {
         for (int c = 0; c < 9; ++c) F.a[c] = P.p[(PF + c) * P.cap + i];
         v3<float> vel, lns;
         m3<float> Cm, Us;
-        int off = 0;
         if (DO_G2P) {
             Stencil s;
             make_stencil(x, y, z, inv_dx, s);
             const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
             if (!stencil_ok(d, lx, ly, lz)) {
                 atomicOr(err, 1);
-                return 0;
+                return D;
             }
             const float kC = 4.0f * inv_dx * inv_dx;
             const int rx = lx - tx0, ry = ly - ty0, rz = lz - tz0;
             if (GTILE && rx >= 0 && ry >= 0 && rz >= 0 && rx <= TE - 3 && ry <= TE - 3 && rz <= TE - 3) {
-                gather(s, dx, kC, [&](int a, int b, int c) { return S.vt[((rx + a) * TE + (ry + b)) * TE + (rz + c)]; },
-                       vel, Cm);
+                gather(s, dx, kC, [&](int a, int b, int c) { return S.vt[tix(rx + a, ry + b, rz + c)]; }, vel, Cm);
             } else {
                 gather(s, dx, kC, [&](int a, int b, int c) { return gin.mv[node_of(d, lx + a, ly + b, lz + c)]; }, vel,
                        Cm);
@@ -90,7 +117,7 @@ struct Fused {
             v3<float> sg;
             if (!svd_fast(Ftr, Us, sg, V)) {
                 fa.redo[atomicAdd(fa.n_redo, 1)] = (int)i;  // state untouched: redone from scratch in FP64
-                return 0;
+                return D;
             }
             const Material M{ph.mu, ph.lambda, ph.alpha, ph.kohesion};
             const v3<float> sn = dp_fast(sg, M, lns);
@@ -118,58 +145,119 @@ struct Fused {
             v3<float> sg;
             if (!svd_fast(F, Us, sg, V)) {
                 fa.redo[atomicAdd(fa.n_redo, 1)] = (int)i;
-                return 0;
+                return D;
             }
             lns = mk(flog(fmaxf(sg.x, 1e-6f)), flog(fmaxf(sg.y, 1e-6f)), flog(fmaxf(sg.z, 1e-6f)));
         }
-
-        // ---- P2G of the (new) state
         const float mass = P.p[PM * P.cap + i];
         const float V0 = P.p[PV0 * P.cap + i];
-        v3<float> mvel;
-        m3<float> Aff;
-        p2g_coeffs(mass, V0, vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
-        Stencil s;
-        make_stencil(x, y, z, inv_dx, s);
-        const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+        p2g_coeffs(mass, V0, vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, D.mvel, D.Aff);
+        make_stencil(x, y, z, inv_dx, D.s);
+        const int lx = D.s.b[0] - d.ox, ly = D.s.b[1] - d.oy, lz = D.s.b[2] - d.oz;
         if (!stencil_ok(d, lx, ly, lz)) {
             atomicOr(err, 1);
-            return 0;
+            return D;
         }
-        const double qp = rint((double)mass * fa.qscale_inv);
-        const int rx = lx - tx0, ry = ly - ty0, rz = lz - tz0;
-        if (!(rx >= 0 && ry >= 0 && rz >= 0 && rx <= TE - 3 && ry <= TE - 3 && rz <= TE - 3)) {
-            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+        D.qp = rint((double)mass * fa.qscale_inv);
+        D.rx = lx - tx0;
+        D.ry = ly - ty0;
+        D.rz = lz - tz0;
+        D.state = (D.rx >= 0 && D.ry >= 0 && D.rz >= 0 && D.rx <= TE - 3 && D.ry <= TE - 3 && D.rz <= TE - 3) ? 1 : 2;
+        return D;
+    }
+
+    static __device__ __forceinline__ bool in_cell(const Dep& D, int cell) {
+        return D.state == 1 && cell >= 0 && D.rx - 1 == (cell >> 4) && D.ry - 1 == ((cell >> 2) & 3) &&
+               D.rz - 1 == (cell & 3);
+    }
+
+    // Ordered read-modify-write of one node record (no other lane of the
+    // round addresses this node for this stencil offset).
+    static __device__ __forceinline__ void rmw(Smem& S, int k, v3<float> mom, long long dq) {
+        float4 q = S.pt[k];
+        q.x += mom.x;
+        q.y += mom.y;
+        q.z += mom.z;
+        q.w = __uint_as_float(__float_as_uint(q.w) + ((unsigned)dq & MLO_MASK));
+        S.pt[k] = q;
+        S.mh[k] += (unsigned)(dq >> MLO_BITS);
+        __asm__ volatile("" ::: "memory");
+    }
+
+    // Deposit of one particle: RMW when the lane owns the stencil's cell in
+    // this round, LDS atomics when it does not, global atomics off the tile.
+    static __device__ __forceinline__ int deposit(Smem& S, const Dep& D, bool own, const GridBuf& gout,
+                                                  uint8_t* bever, const Domain& d, int tx0, int ty0, int tz0) {
+        if (D.state == 0) return 0;
+        const float dx = d.dx;
+        if (D.state == 2) {
+            const int lx = D.rx + tx0, ly = D.ry + ty0, lz = D.rz + tz0;
+            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
                 if (w != 0.0f) deposit_global(gout, bever, d, lx + a, ly + b, lz + c, dq, mom);
             });
             return 1;
         }
-        const bool same_cell = ranked && rx - 1 == (cell0 >> 4) && ry - 1 == ((cell0 >> 2) & 3) && rz - 1 == (cell0 & 3);
-        if (RMW && same_cell) {
-            // this lane owns the stencil of its cell in this round: plain
-            // read-modify-write, kept in program order across the 27 nodes
-            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
-                const int k = ((rx + a) * TE + (ry + b)) * TE + (rz + c);
-                float4 q = S.pt[k];
-                q.x += mom.x;
-                q.y += mom.y;
-                q.z += mom.z;
-                S.pt[k] = q;
-                S.mt[k] += (unsigned long long)dq;
-                __asm__ volatile("" ::: "memory");
+        if (RMW && own) {
+            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+                rmw(S, tix(D.rx + a, D.ry + b, D.rz + c), mom, dq);
             });
         } else {
             // zero weights deposit zero: no per-node branch is needed
-            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
-                const int k = ((rx + a) * TE + (ry + b)) * TE + (rz + c);
-                atomicAdd(&S.mt[k], (unsigned long long)dq);
+            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+                const int k = tix(D.rx + a, D.ry + b, D.rz + c);
                 float* q = (float*)&S.pt[k];
                 atomicAdd(q + 0, mom.x);
                 atomicAdd(q + 1, mom.y);
                 atomicAdd(q + 2, mom.z);
+                atomicAdd((unsigned*)(q + 3), (unsigned)dq & MLO_MASK);
+                atomicAdd(&S.mh[k], (unsigned)(dq >> MLO_BITS));
             });
         }
-        return off;
+        return 0;
+    }
+
+    // Two particles with the same stencil base owned by this lane: one
+    // read-modify-write per node for both.
+    static __device__ __forceinline__ void deposit2(Smem& S, const Dep& A, const Dep& B, float dx) {
+        long long accA = 0, accB = 0;
+        v3<float> mcA, mcB;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float dxA = ((float)(A.s.b[0] + a) - A.s.f[0] - 0.5f) * dx;
+            const float dxB = ((float)(B.s.b[0] + a) - B.s.f[0] - 0.5f) * dx;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const float dyA = ((float)(A.s.b[1] + b) - A.s.f[1] - 0.5f) * dx;
+                const float dyB = ((float)(B.s.b[1] + b) - B.s.f[1] - 0.5f) * dx;
+                const float pxA = A.mvel.x + A.Aff.a[0] * dxA + A.Aff.a[1] * dyA;
+                const float pyA = A.mvel.y + A.Aff.a[3] * dxA + A.Aff.a[4] * dyA;
+                const float pzA = A.mvel.z + A.Aff.a[6] * dxA + A.Aff.a[7] * dyA;
+                const float pxB = B.mvel.x + B.Aff.a[0] * dxB + B.Aff.a[1] * dyB;
+                const float pyB = B.mvel.y + B.Aff.a[3] * dxB + B.Aff.a[4] * dyB;
+                const float pzB = B.mvel.z + B.Aff.a[6] * dxB + B.Aff.a[7] * dyB;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float dzA = ((float)(A.s.b[2] + c) - A.s.f[2] - 0.5f) * dx;
+                    const float dzB = ((float)(B.s.b[2] + c) - B.s.f[2] - 0.5f) * dx;
+                    const float wA = A.s.w[0][a] * A.s.w[1][b] * A.s.w[2][c];
+                    const float wB = B.s.w[0][a] * B.s.w[1][b] * B.s.w[2][c];
+                    const v3<float> mA = mk(wA * (pxA + A.Aff.a[2] * dzA), wA * (pyA + A.Aff.a[5] * dzA),
+                                            wA * (pzA + A.Aff.a[8] * dzA));
+                    const v3<float> mB = mk(wB * (pxB + B.Aff.a[2] * dzB), wB * (pyB + B.Aff.a[5] * dzB),
+                                            wB * (pzB + B.Aff.a[8] * dzB));
+                    if (a == 1 && b == 1 && c == 1) {  // centres take the remainders, last
+                        mcA = mA;
+                        mcB = mB;
+                        continue;
+                    }
+                    const long long qA = mass_quanta(wA, A.qp), qB = mass_quanta(wB, B.qp);
+                    accA += qA;
+                    accB += qB;
+                    rmw(S, tix(A.rx + a, A.ry + b, A.rz + c), mA + mB, qA + qB);
+                }
+            }
+        }
+        rmw(S, tix(A.rx + 1, A.ry + 1, A.rz + 1), mcA + mcB, ((long long)A.qp - accA) + ((long long)B.qp - accB));
     }
 
     static __device__ void run(PView P, const int4* __restrict__ units, GridBuf gin, GridBuf gout, uint8_t* bever,
@@ -188,14 +276,16 @@ struct Fused {
 
         for (int k = lane; k < TNODES; k += 64) {
             if (DO_G2P && GTILE) {
-                const int lx = tx0 + (k >> 6), ly = ty0 + ((k >> 3) & 7), lz = tz0 + (k & 7);
+                int ix, iy, iz;
+                tile_coords(k, ix, iy, iz);
+                const int lx = tx0 + ix, ly = ty0 + iy, lz = tz0 + iz;
                 float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (lx >= 0 && ly >= 0 && lz >= 0 && lx < d.nx && ly < d.ny && lz < d.nz)
                     q = gin.mv[node_of(d, lx, ly, lz)];
                 S.vt[k] = q;
             }
             S.pt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            S.mt[k] = 0ull;
+            S.mh[k] = 0u;
         }
         S.cnt[lane] = 0;
         if (lane < 27) S.tb[lane] = 0;
@@ -221,29 +311,43 @@ struct Fused {
         }
         __syncthreads();
 
-        int mine = min(S.cnt[lane], RMAX);
+        const int mine = min(S.cnt[lane], RMAX);
         int rounds = mine;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
         int off = 0;
-        for (int r = 0; r < rounds; ++r) {
-            if (r < mine)
-                off += particle(S, (int64_t)U.y + S.tab[r][lane], true, lane, P, gin, gout, bever, d, ph, fa, err, tx0,
-                                ty0, tz0);
+        for (int r = 0; r < rounds; r += 2) {
+            Dep A, B;
+            A.state = 0;
+            B.state = 0;
+            if (r < mine) A = advance(S, (int64_t)U.y + S.tab[r][lane], P, gin, d, ph, fa, err, tx0, ty0, tz0);
+            if (r + 1 < mine) B = advance(S, (int64_t)U.y + S.tab[r + 1][lane], P, gin, d, ph, fa, err, tx0, ty0, tz0);
+            const bool ownA = in_cell(A, lane), ownB = in_cell(B, lane);
+            if (RMW && ownA && ownB) {
+                deposit2(S, A, B, d.dx);
+            } else {
+                off += deposit(S, A, ownA, gout, bever, d, tx0, ty0, tz0);
+                off += deposit(S, B, ownB, gout, bever, d, tx0, ty0, tz0);
+            }
         }
         const int nl = S.nleft;
-        for (int k = lane; k < nl; k += 64)
-            off += particle(S, (int64_t)U.y + S.left[k], false, -1, P, gin, gout, bever, d, ph, fa, err, tx0, ty0, tz0);
+        for (int k = lane; k < nl; k += 64) {
+            const Dep D = advance(S, (int64_t)U.y + S.left[k], P, gin, d, ph, fa, err, tx0, ty0, tz0);
+            off += deposit(S, D, false, gout, bever, d, tx0, ty0, tz0);
+        }
         if (off) atomicAdd(&counters[1], (unsigned long long)off);
         __syncthreads();
 
         // ---- flush the tile into the next grid
         for (int k = lane; k < TNODES; k += 64) {
-            const unsigned long long m = S.mt[k];
-            if (m == 0ull) continue;
-            const int lx = tx0 + (k >> 6), ly = ty0 + ((k >> 3) & 7), lz = tz0 + (k & 7);
-            const int64_t nid = node_of(d, lx, ly, lz);
             const float4 q = S.pt[k];
+            const unsigned long long m =
+                ((unsigned long long)S.mh[k] << MLO_BITS) + (unsigned long long)__float_as_uint(q.w);
+            if (m == 0ull) continue;
+            int ix, iy, iz;
+            tile_coords(k, ix, iy, iz);
+            const int lx = tx0 + ix, ly = ty0 + iy, lz = tz0 + iz;
+            const int64_t nid = node_of(d, lx, ly, lz);
             atomicAdd((unsigned long long*)&gout.mq[nid], m);
             float* nm = (float*)&gout.mv[nid];
             atomicAdd(nm + 0, q.x);

@@ -112,6 +112,14 @@ void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, int variant,
                   hipStream_t s);
-constexpr int FUSED_UNIT_MAX = 1024;  // largest unit the fused kernel accepts
+// Largest unit the fast kernels accept: 512 particles of at most 2^23 - 1
+// mass quanta each keep every LDS node-mass sum below 2^32.
+constexpr int FUSED_UNIT_MAX = 512;
+// Two-kernel form (variant bit 2): per-particle G2P writing 12 deposit
+// coefficients per particle into `coef`, then the ranked per-block P2G.
+void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
+                  const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
+                  double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
+                  hipStream_t s, void (*between)(void*) = nullptr, void* ctx = nullptr);
 
 }  // namespace bm

@@ -1,0 +1,382 @@
+// mpm_split.hip -- the two-kernel form of the fast substep (tuning variant
+// bit 2): a per-particle G2P kernel at high occupancy that also writes each
+// particle's deposit coefficients, then a per-block P2G kernel that ranks the
+// particles by their *final* stencil cell and deposits with ordered LDS
+// read-modify-writes.
+//
+// Compared with the fused kernel it moves 112 more bytes per particle
+// (coefficients written and read back, position read twice) but neither
+// kernel holds the other's registers: the G2P kernel runs without LDS, and
+// the P2G kernel without the SVD.
+//
+// Deposit coefficients (12 floats per particle, component-major like the
+// particle arrays): m v (3) and m C + dt K (9, row-major), K = -4/dx^2 V0 tau.
+
+#include "mpm_fast_device.hpp"
+
+namespace bm {
+
+constexpr int NCOEF = 12;
+constexpr int SUNIT_MAX = 512;   // largest unit (see FUSED_UNIT_MAX)
+
+__device__ __forceinline__ int stix(int ix, int iy, int iz) { return ((ix * TE + iy) * TE) + ((iz + 4 * (iy & 1)) & 7); }
+__device__ __forceinline__ void stile_coords(int k, int& ix, int& iy, int& iz) {
+    ix = k >> 6;
+    iy = (k >> 3) & 7;
+    iz = ((k & 7) - 4 * (iy & 1)) & 7;
+}
+
+__device__ __forceinline__ void store_coef(float* coef, int64_t cap, int64_t i, const v3<float>& mvel,
+                                           const m3<float>& Aff) {
+    coef[0 * cap + i] = mvel.x;
+    coef[1 * cap + i] = mvel.y;
+    coef[2 * cap + i] = mvel.z;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) coef[(3 + c) * cap + i] = Aff.a[c];
+}
+
+// G2P + plasticity + write-back + deposit coefficients; one lane per particle.
+template <bool DO_G2P>
+__global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float* coef,
+                                                 int* err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const float dx = d.dx, inv_dx = d.inv_dx;
+    float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
+    m3<float> F;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) F.a[c] = P.p[(PF + c) * P.cap + i];
+    v3<float> vel, lns;
+    m3<float> Cm, Us;
+    if (DO_G2P) {
+        Stencil s;
+        make_stencil(x, y, z, inv_dx, s);
+        const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+        if (!stencil_ok(d, lx, ly, lz)) {
+            atomicOr(err, 1);
+            store_coef(coef, P.cap, i, mk(0.f, 0.f, 0.f), m3<float>{{0, 0, 0, 0, 0, 0, 0, 0, 0}});
+            return;
+        }
+        const NodeAxes na(d, lx, ly, lz);
+        const float4* __restrict__ g = gin.mv;
+        gather_sep(s, dx, 4.0f * inv_dx * inv_dx, [&](int a, int b, int c) { return g[na(a, b, c)]; }, vel, Cm);
+        const float dt = fa.dt_g2p;
+        x += vel.x * dt;
+        y += vel.y * dt;
+        z += vel.z * dt;
+        m3<float> A;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) A.a[c] = ((c % 4 == 0) ? 1.0f : 0.0f) + Cm.a[c] * dt;
+        const m3<float> Ftr = mmul(A, F);
+        m3<float> V;
+        v3<float> sg;
+        if (!svd_fast(Ftr, Us, sg, V)) {
+            fa.redo[atomicAdd(fa.n_redo, 1)] = (int)i;  // redone from scratch in FP64
+            return;
+        }
+        const Material M{ph.mu, ph.lambda, ph.alpha, ph.kohesion};
+        const v3<float> sn = dp_fast(sg, M, lns);
+        F = recompose(Us, sn, V);
+        if (y < 0.0f) {
+            y = 0.0f;
+            vel.y = 0.0f;
+        }
+        P.p[(PX + 0) * P.cap + i] = x;
+        P.p[(PX + 1) * P.cap + i] = y;
+        P.p[(PX + 2) * P.cap + i] = z;
+        P.p[(PV + 0) * P.cap + i] = vel.x;
+        P.p[(PV + 1) * P.cap + i] = vel.y;
+        P.p[(PV + 2) * P.cap + i] = vel.z;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            P.p[(PF + c) * P.cap + i] = F.a[c];
+            P.p[(PC + c) * P.cap + i] = Cm.a[c];
+        }
+    } else {
+        vel = mk(P.p[(PV + 0) * P.cap + i], P.p[(PV + 1) * P.cap + i], P.p[(PV + 2) * P.cap + i]);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Cm.a[c] = P.p[(PC + c) * P.cap + i];
+        m3<float> V;
+        v3<float> sg;
+        if (!svd_fast(F, Us, sg, V)) {
+            fa.redo[atomicAdd(fa.n_redo, 1)] = (int)i;
+            return;
+        }
+        lns = mk(flog(fmaxf(sg.x, 1e-6f)), flog(fmaxf(sg.y, 1e-6f)), flog(fmaxf(sg.z, 1e-6f)));
+    }
+    v3<float> mvel;
+    m3<float> Aff;
+    p2g_coeffs(P.p[PM * P.cap + i], P.p[PV0 * P.cap + i], vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
+    store_coef(coef, P.cap, i, mvel, Aff);
+}
+
+// The FP64 pass for the particles k_advance listed: same outputs.
+template <bool DO_G2P>
+__global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float* coef,
+                                                     unsigned long long* counters) {
+    const int cnt = *fa.n_redo;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
+        const int64_t i = fa.redo[r];
+        const float dx = d.dx, inv_dx = d.inv_dx;
+        float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
+        m3<float> F;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) F.a[c] = P.p[(PF + c) * P.cap + i];
+        v3<float> vel, lns;
+        m3<float> Cm, Us;
+        if (DO_G2P) {
+            Stencil s;
+            make_stencil(x, y, z, inv_dx, s);
+            const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+            gather(s, dx, 4.0f * inv_dx * inv_dx,
+                   [&](int a, int b, int c) { return gin.mv[node_of(d, lx + a, ly + b, lz + c)]; }, vel, Cm);
+            const float dt = fa.dt_g2p;
+            x += vel.x * dt;
+            y += vel.y * dt;
+            z += vel.z * dt;
+            m3<float> A;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) A.a[c] = ((c % 4 == 0) ? 1.0f : 0.0f) + Cm.a[c] * dt;
+            const m3<float> Ftr = mmul(A, F);
+            m3<float> V;
+            v3<float> sg;
+            svd_guarded(Ftr, Us, sg, V);
+            const Material M{ph.mu, ph.lambda, ph.alpha, ph.kohesion};
+            const v3<float> sn = dp_project(sg, M, &lns);
+            F = recompose(Us, sn, V);
+            if (y < 0.0f) {
+                y = 0.0f;
+                vel.y = 0.0f;
+            }
+            P.p[(PX + 0) * P.cap + i] = x;
+            P.p[(PX + 1) * P.cap + i] = y;
+            P.p[(PX + 2) * P.cap + i] = z;
+            P.p[(PV + 0) * P.cap + i] = vel.x;
+            P.p[(PV + 1) * P.cap + i] = vel.y;
+            P.p[(PV + 2) * P.cap + i] = vel.z;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                P.p[(PF + c) * P.cap + i] = F.a[c];
+                P.p[(PC + c) * P.cap + i] = Cm.a[c];
+            }
+        } else {
+            vel = mk(P.p[(PV + 0) * P.cap + i], P.p[(PV + 1) * P.cap + i], P.p[(PV + 2) * P.cap + i]);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) Cm.a[c] = P.p[(PC + c) * P.cap + i];
+            m3<float> V;
+            v3<float> sg;
+            svd_guarded(F, Us, sg, V);
+            lns = mk(logf(fmaxf(sg.x, 1e-6f)), logf(fmaxf(sg.y, 1e-6f)), logf(fmaxf(sg.z, 1e-6f)));
+        }
+        v3<float> mvel;
+        m3<float> Aff;
+        p2g_coeffs(P.p[PM * P.cap + i], P.p[PV0 * P.cap + i], vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
+        store_coef(coef, P.cap, i, mvel, Aff);
+        atomicAdd(&counters[0], 1ull);
+    }
+}
+
+// P2G of one unit (one wave) with per-cell register accumulation.  The
+// unit's particles are counting-sorted by the cell of their final stencil
+// base; lane c then owns cell c of the block, runs through that cell's
+// particles accumulating all 27 node deposits (momentum and mass quanta) in
+// registers -- they share their 27 nodes -- and adds the totals into the
+// LDS tile with one ordered read-modify-write chain.  For a fixed stencil
+// offset the 64 lanes address 64 different nodes, and program order across
+// offsets keeps neighbouring cells' shared nodes consistent.
+struct CellSmem {
+    float4 pt[TNODES];  // momentum xyz; w = mass quanta (uint, < 2^32 for <= 512 particles)
+    unsigned short ord[SUNIT_MAX];
+    unsigned short rank[SUNIT_MAX];
+    unsigned char cell[SUNIT_MAX];
+    unsigned short left[SUNIT_MAX];
+    int cnt[64];
+    int start[64];
+    int nleft;
+    int tb[27];
+};
+
+__global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restrict__ coef,
+                                                 const int4* __restrict__ units, GridBuf gout, uint8_t* bever,
+                                                 Domain d, FastArgs fa, int* err, unsigned long long* counters) {
+    __shared__ CellSmem S;
+    const int per = (gridDim.x + 7) / 8;
+    const int u = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (u >= fa.n_units) return;
+    const int4 U = units[u];
+    const int blk = U.x;
+    const int bz = blk % d.nbz, by = (blk / d.nbz) % d.nby, bx = blk / (d.nbz * d.nby);
+    const int tx0 = bx * 4 - 1, ty0 = by * 4 - 1, tz0 = bz * 4 - 1;
+    const int lane = threadIdx.x;
+    const float dx = d.dx, inv_dx = d.inv_dx;
+
+    for (int k = lane; k < TNODES; k += 64) S.pt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    S.cnt[lane] = 0;
+    if (lane < 27) S.tb[lane] = 0;
+    if (lane == 0) S.nleft = 0;
+    __syncthreads();
+
+    // ---- count the particles per cell of the block
+    for (int pi = lane; pi < U.z; pi += 64) {
+        const int64_t i = (int64_t)U.y + pi;
+        Stencil s;
+        make_stencil(P.p[(PX + 0) * P.cap + i], P.p[(PX + 1) * P.cap + i], P.p[(PX + 2) * P.cap + i], inv_dx, s);
+        const int cx = s.b[0] - d.ox - tx0 - 1, cy = s.b[1] - d.oy - ty0 - 1, cz = s.b[2] - d.oz - tz0 - 1;
+        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < 4 && cy < 4 && cz < 4) {
+            const int c = (cx * 4 + cy) * 4 + cz;
+            S.cell[pi] = (unsigned char)c;
+            S.rank[pi] = (unsigned short)atomicAdd(&S.cnt[c], 1);
+        } else {
+            S.cell[pi] = 255;
+            S.left[atomicAdd(&S.nleft, 1)] = (unsigned short)pi;
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the 64 counts across the wave
+    const int mine = S.cnt[lane];
+    int incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    S.start[lane] = incl - mine;
+    __syncthreads();
+    for (int pi = lane; pi < U.z; pi += 64) {
+        const int c = S.cell[pi];
+        if (c != 255) S.ord[S.start[c] + S.rank[pi]] = (unsigned short)pi;
+    }
+    __syncthreads();
+
+    // ---- per-cell accumulation in registers
+    v3<float> acc[27];
+    unsigned accm[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        acc[k] = mk(0.f, 0.f, 0.f);
+        accm[k] = 0u;
+    }
+    const int s0 = S.start[lane];
+    for (int k = 0; k < mine; ++k) {
+        const int64_t i = (int64_t)U.y + S.ord[s0 + k];
+        Stencil s;
+        make_stencil(P.p[(PX + 0) * P.cap + i], P.p[(PX + 1) * P.cap + i], P.p[(PX + 2) * P.cap + i], inv_dx, s);
+        const v3<float> mvel = mk(coef[0 * P.cap + i], coef[1 * P.cap + i], coef[2 * P.cap + i]);
+        m3<float> Aff;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Aff.a[c] = coef[(3 + c) * P.cap + i];
+        const double qp = rint((double)P.p[PM * P.cap + i] * fa.qscale_inv);
+        scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+            const int n = (a * 3 + b) * 3 + c;
+            acc[n] = acc[n] + mom;
+            accm[n] += (unsigned)dq;
+        });
+    }
+    if (mine > 0) {
+        const int rx = (lane >> 4) + 1, ry = ((lane >> 2) & 3) + 1, rz = (lane & 3) + 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int n = (a * 3 + b) * 3 + c;
+                    const int k = stix(rx + a, ry + b, rz + c);
+                    float4 q = S.pt[k];
+                    q.x += acc[n].x;
+                    q.y += acc[n].y;
+                    q.z += acc[n].z;
+                    q.w = __uint_as_float(__float_as_uint(q.w) + accm[n]);
+                    S.pt[k] = q;
+                    __asm__ volatile("" ::: "memory");
+                }
+    }
+    __syncthreads();
+
+    // ---- particles outside the block's cells (drifted since the last
+    // re-binning): LDS atomics inside the tile, global atomics outside
+    int off = 0;
+    const int nl = S.nleft;
+    for (int k = lane; k < nl; k += 64) {
+        const int64_t i = (int64_t)U.y + S.left[k];
+        Stencil s;
+        make_stencil(P.p[(PX + 0) * P.cap + i], P.p[(PX + 1) * P.cap + i], P.p[(PX + 2) * P.cap + i], inv_dx, s);
+        const v3<float> mvel = mk(coef[0 * P.cap + i], coef[1 * P.cap + i], coef[2 * P.cap + i]);
+        m3<float> Aff;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Aff.a[c] = coef[(3 + c) * P.cap + i];
+        const double qp = rint((double)P.p[PM * P.cap + i] * fa.qscale_inv);
+        const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+        if (!stencil_ok(d, lx, ly, lz)) {
+            atomicOr(err, 1);
+            continue;
+        }
+        const int qx = lx - tx0, qy = ly - ty0, qz = lz - tz0;
+        if (qx >= 0 && qy >= 0 && qz >= 0 && qx <= TE - 3 && qy <= TE - 3 && qz <= TE - 3) {
+            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+                float* q = (float*)&S.pt[stix(qx + a, qy + b, qz + c)];
+                atomicAdd(q + 0, mom.x);
+                atomicAdd(q + 1, mom.y);
+                atomicAdd(q + 2, mom.z);
+                atomicAdd((unsigned*)(q + 3), (unsigned)dq);
+            });
+        } else {
+            ++off;
+            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+                if (w != 0.0f) deposit_global(gout, bever, d, lx + a, ly + b, lz + c, dq, mom);
+            });
+        }
+    }
+    if (off) atomicAdd(&counters[1], (unsigned long long)off);
+    __syncthreads();
+
+    // ---- flush the tile into the next grid
+    for (int k = lane; k < TNODES; k += 64) {
+        const float4 q = S.pt[k];
+        const unsigned m = __float_as_uint(q.w);
+        if (m == 0u) continue;
+        int ix, iy, iz;
+        stile_coords(k, ix, iy, iz);
+        const int lx = tx0 + ix, ly = ty0 + iy, lz = tz0 + iz;
+        const int64_t nid = node_of(d, lx, ly, lz);
+        atomicAdd((unsigned long long*)&gout.mq[nid], (unsigned long long)m);
+        float* nm = (float*)&gout.mv[nid];
+        atomicAdd(nm + 0, q.x);
+        atomicAdd(nm + 1, q.y);
+        atomicAdd(nm + 2, q.z);
+        S.tb[(((lx >> 2) - bx + 1) * 3 + ((ly >> 2) - by + 1)) * 3 + ((lz >> 2) - bz + 1)] = 1;
+    }
+    __syncthreads();
+    if (lane < 27 && S.tb[lane]) {
+        const int qx = lane / 9, qy = (lane / 3) % 3, qz = lane % 3;
+        touch_block_f(gout, bever, ((bx + qx - 1) * d.nby + (by + qy - 1)) * d.nbz + (bz + qz - 1));
+    }
+}
+
+void launch_zero_i32(int* p, hipStream_t s);
+
+// G2P (or the coefficients of the current state when !do_g2p), the FP64
+// pass for the particles it listed, then the ranked P2G.
+void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
+                  const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
+                  double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
+                  hipStream_t s, void (*between)(void*), void* ctx) {
+    if (P.n <= 0) return;
+    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo};
+    launch_zero_i32(n_redo, s);
+    const int g = (int)((P.n + 255) / 256);
+    if (do_g2p) {
+        hipLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, P, gin, d, ph, fa, coef, err);
+        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
+    } else {
+        hipLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, P, gin, d, ph, fa, coef, err);
+        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
+    }
+    if (between) between(ctx);
+    if (n_units > 0) {
+        const int nwg = ((n_units + 7) / 8) * 8;
+        hipLaunchKernelGGL(k_p2g_cell, dim3(nwg), dim3(64), 0, s, P, coef, units, gout, bever, d, fa, err, counters);
+    }
+}
+
+}  // namespace bm

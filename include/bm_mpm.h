@@ -191,11 +191,15 @@ int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallback
 
 /* ---------------------------------------------------------------- tuning */
 /* BM_MODE_FAST launch parameters (what the autotuner searches):
- * unit_particles: particles per workgroup unit (16..1024, default 512);
+ * unit_particles: particles per workgroup unit (16..512, default 512);
  * sort_every: substeps between particle re-binnings (>= 1, default 16);
- * variant: bit 0 = plain read-modify-write LDS deposit (else collision-free
- * LDS atomics), bit 1 = stage G2P node velocities in LDS.  Values <= 0 /
- * < 0 keep the current setting.  No effect in BM_MODE_REFERENCE. */
+ * variant: bit 2 = two-kernel form (per-particle G2P writing deposit
+ * coefficients, then a per-block P2G accumulating each cell's particles in
+ * registers; the default, variant 4); without bit 2 the fused G2P+P2G
+ * kernel runs, with bit 0 = plain read-modify-write LDS deposit (else
+ * collision-free LDS atomics) and bit 1 = G2P node velocities staged in LDS.
+ * Values <= 0 / < 0 keep the current setting.  No effect in
+ * BM_MODE_REFERENCE. */
 int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_every, int32_t variant);
 
 /* ---------------------------------------------------------------- timing */
