@@ -60,6 +60,11 @@ struct bm_mpm_solver {
     double* d_partials = nullptr;
     double* d_out7 = nullptr;
     int max_partials = 0;
+    // device body of the coupled loop
+    bm_body* d_body = nullptr;
+    ColliderDev* d_col = nullptr;
+    int64_t body_blocks = 0;  // scan-range bound of the body's collider
+    bool body_set = false;
 
     // MPMSolver parameters (mpm_solver.h:100-109 defaults)
     float E = 1.0e5f, nu = 0.3f, rho = 1500.0f, alpha = 0.40f, mu_floor = 0.4f;
@@ -206,10 +211,11 @@ static int sync_and_check(bm_mpm_handle h) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     int err = 0;
     HIP_TRY(hipMemcpy(&err, h->d_err, sizeof(int), hipMemcpyDeviceToHost));
-    if (err & 1) {
+    if (err) {
         int zero = 0;
         HIP_TRY(hipMemcpy(h->d_err, &zero, sizeof(int), hipMemcpyHostToDevice));
-        return fail(BM_ERR_OUT_OF_DOMAIN, "a particle stencil left the grid domain");
+        if (err & 1) return fail(BM_ERR_OUT_OF_DOMAIN, "a particle stencil left the grid domain");
+        return fail(BM_ERR_HIP, "device body collider exceeded its scan-range bound");
     }
     return BM_OK;
 }
@@ -295,6 +301,9 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
     if (hipMalloc(&h->d_partials, sizeof(double) * 7 * h->max_partials) != hipSuccess)
         return bail(fail(BM_ERR_OOM, "coupler partials"));
     if (hipMalloc(&h->d_out7, sizeof(double) * 8) != hipSuccess) return bail(fail(BM_ERR_OOM, "coupler out"));
+    if (hipMalloc(&h->d_body, sizeof(bm_body)) != hipSuccess) return bail(fail(BM_ERR_OOM, "device body"));
+    if (hipMalloc(&h->d_col, sizeof(ColliderDev)) != hipSuccess) return bail(fail(BM_ERR_OOM, "device collider"));
+    hipMemset(h->d_out7, 0, sizeof(double) * 8);
     if (hipMalloc(&h->d_err, sizeof(int)) != hipSuccess) return bail(fail(BM_ERR_OOM, "err flag"));
     if (hipMalloc(&h->d_scratch_u64, sizeof(unsigned long long) * 2) != hipSuccess) return bail(fail(BM_ERR_OOM, "scratch"));
     if (hipMalloc(&h->d_fallbacks, 2 * sizeof(unsigned long long)) != hipSuccess) return bail(fail(BM_ERR_OOM, "counter"));
@@ -316,6 +325,8 @@ int bm_mpm_destroy(bm_mpm_handle h) {
     hipFree(h->bever);
     hipFree(h->d_partials);
     hipFree(h->d_out7);
+    hipFree(h->d_body);
+    hipFree(h->d_col);
     hipFree(h->d_err);
     hipFree(h->d_scratch_u64);
     hipFree(h->d_fallbacks);
@@ -544,18 +555,23 @@ static int rebin(bm_mpm_handle h) {
 static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const GridBuf& gout, float dt_g2p,
                          float dt_fold) {
     if (h->variant & 4) {
-        // time the two kernels apart: the caller's phase covers the first
-        const int outer = h->open_phase;
+        // Time each kernel on its own: the caller's phase (G2P, or P2G for a
+        // deposit of the current state) covers the particle pass only, the
+        // cell deposit is P2G, the small bookkeeping launches are OTHER.
+        struct Ctx {
+            bm_mpm_handle h;
+            int adv_phase;
+        } ctx{h, h->open_phase};
+        auto mark = [](void* p, int stage) {
+            auto* c = (Ctx*)p;
+            if (!c->h->timing || c->adv_phase < 0) return;
+            c->h->end();
+            c->h->begin(stage == SPLIT_ADVANCE ? c->adv_phase : stage == SPLIT_REDO ? BM_PHASE_OTHER : BM_PHASE_P2G);
+        };
+        if (ctx.adv_phase >= 0) h->open_phase = BM_PHASE_OTHER;  // the opening event now starts OTHER
         launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
-                     dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream,
-                     [](void* p) {
-                         auto* hh = (bm_mpm_handle)p;
-                         if (hh->open_phase < 0) return;
-                         hh->end();
-                         hh->begin(BM_PHASE_P2G);
-                     },
-                     h);
-        (void)outer;
+                     dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream, mark,
+                     &ctx);
     } else
         launch_fused(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
                      dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant & 3,
@@ -656,52 +672,8 @@ int bm_mpm_step_n(bm_mpm_handle h, float dt, int32_t k) {
 
 // ------------------------------------------------------------------ couplers
 static ColliderDev make_collider(bm_mpm_handle h, const bm_collider* b) {
-    ColliderDev c{};
-    const float w = b->quat[0], x = b->quat[1], y = b->quat[2], z = b->quat[3];
-    // Matrix3::from_quaternion, matrix3.h:432-448
-    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y,
-                wz = w * z;
-    const float R[9] = {1.0f - 2.0f * (yy + zz), 2.0f * (xy - wz),        2.0f * (xz + wy),
-                        2.0f * (xy + wz),        1.0f - 2.0f * (xx + zz), 2.0f * (yz - wx),
-                        2.0f * (xz - wy),        2.0f * (yz + wx),        1.0f - 2.0f * (xx + yy)};
-    for (int i = 0; i < 9; ++i) c.Rl[i] = R[i];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) c.Rw[i * 3 + j] = R[j * 3 + i];
-    for (int a = 0; a < 3; ++a) {
-        c.c[a] = b->center[a];
-        c.he[a] = b->half_extents[a];
-        c.vl[a] = b->lin_vel[a];
-        c.wa[a] = b->ang_vel[a];
-    }
-    c.friction = b->friction;
-    c.shape = b->shape;
-    // M1 per-node body mass, mpm_rigid_coupler_pic.h:104-108
-    const float dx = h->dom.dx;
-    const float vol = 8.0f * b->half_extents[0] * b->half_extents[1] * b->half_extents[2];
-    const int approx = std::max(1, (int)std::round(vol / (dx * dx * dx)));
-    c.mb = b->mass / (float)approx;
-    // |R| he world AABB, mpm_rigid_coupler.cpp:70-76
-    for (int a = 0; a < 3; ++a) {
-        const float hw = std::fabs(R[a * 3 + 0]) * c.he[0] + std::fabs(R[a * 3 + 1]) * c.he[1] +
-                         std::fabs(R[a * 3 + 2]) * c.he[2];
-        c.lo[a] = c.c[a] - hw;
-        c.hi[a] = c.c[a] + hw;
-    }
-    // block range: every block whose node-centre box may overlap the AABB,
-    // widened by one block; the kernel applies the reference's exact cull.
-    const int org[3] = {h->dom.ox, h->dom.oy, h->dom.oz};
-    const int nbv[3] = {h->dom.nbx, h->dom.nby, h->dom.nbz};
-    for (int a = 0; a < 3; ++a) {
-        const double lo_n = std::floor((double)c.lo[a] / dx - 0.5) - org[a];
-        const double hi_n = std::ceil((double)c.hi[a] / dx - 0.5) - org[a];
-        long long blo = (long long)std::floor(lo_n / 4.0) - 1;
-        long long bhi = (long long)std::floor(hi_n / 4.0) + 1;
-        blo = std::max<long long>(blo, 0);
-        bhi = std::min<long long>(bhi, nbv[a] - 1);
-        c.b0[a] = (int)blo;
-        c.nb[a] = (bhi >= blo) ? (int)(bhi - blo + 1) : 0;
-    }
-    return c;
+    return collider_build(b->center, b->quat, b->half_extents, b->lin_vel, b->ang_vel, b->friction, b->mass, b->shape,
+                          h->dom);
 }
 
 int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* out) {
@@ -726,6 +698,77 @@ int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, floa
         out->torque[a] = (float)r[3 + a];
     }
     out->nodes_inside = (int32_t)std::llround(r[6]);
+    return BM_OK;
+}
+
+// ------------------------------------------------------------------ device body
+int bm_mpm_set_body(bm_mpm_handle h, const bm_body* body) {
+    if (int rc = check_handle(h)) return rc;
+    if (!body) return fail(BM_ERR_INVALID_ARG, "null body");
+    if (body->shape != BM_SHAPE_BOX && body->shape != BM_SHAPE_CYLINDER)
+        return fail(BM_ERR_INVALID_ARG, "unknown shape %d", body->shape);
+    for (int a = 0; a < 3; ++a)
+        if (!(body->half_extents[a] >= 0.0f) || !std::isfinite(body->half_extents[a]))
+            return fail(BM_ERR_INVALID_ARG, "half extents must be finite and >= 0");
+    const int64_t blocks = couple_blocks_bound(body->half_extents, h->dom);
+    if ((blocks + 3) / 4 > h->max_partials) return fail(BM_ERR_INVALID_ARG, "collider range too large");
+    HIP_TRY(hipMemcpyAsync(h->d_body, body, sizeof(bm_body), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->body_blocks = blocks;
+    h->body_set = true;
+    return BM_OK;
+}
+
+int bm_mpm_get_body(bm_mpm_handle h, bm_body* body, bm_reaction* reaction) {
+    if (int rc = check_handle(h)) return rc;
+    if (!body) return fail(BM_ERR_INVALID_ARG, "null body");
+    if (!h->body_set) return fail(BM_ERR_STATE, "no device body set");
+    double r[7];
+    HIP_TRY(hipMemcpyAsync(body, h->d_body, sizeof(bm_body), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(r, h->d_out7, sizeof(double) * 7, hipMemcpyDeviceToHost, h->stream));
+    if (int rc = sync_and_check(h)) return rc;
+    if (reaction) {
+        for (int a = 0; a < 3; ++a) {
+            reaction->force[a] = (float)r[a];
+            reaction->torque[a] = (float)r[3 + a];
+        }
+        reaction->nodes_inside = (int32_t)std::llround(r[6]);
+    }
+    return BM_OK;
+}
+
+int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* out) {
+    if (int rc = check_handle(h)) return rc;
+    if (method != BM_COUPLER_M0_DIRICHLET && method != BM_COUPLER_M1_PIC_RIGID)
+        return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
+    if (!h->body_set) return fail(BM_ERR_STATE, "no device body set");
+    if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "couple needs an updated grid");
+    h->begin(BM_PHASE_COUPLE);
+    launch_couple_body(method, h->d_body, h->d_col, h->grid[h->cur], h->bever, h->dom, dt, h->body_blocks,
+                       h->d_partials, h->d_out7, h->d_err, h->stream);
+    h->end();
+    HIP_TRY(hipGetLastError());
+    if (!out) return BM_OK;
+    double r[7];
+    HIP_TRY(hipMemcpyAsync(r, h->d_out7, sizeof(double) * 7, hipMemcpyDeviceToHost, h->stream));
+    if (int rc = sync_and_check(h)) return rc;
+    for (int a = 0; a < 3; ++a) {
+        out->force[a] = (float)r[a];
+        out->torque[a] = (float)r[3 + a];
+    }
+    out->nodes_inside = (int32_t)std::llround(r[6]);
+    return BM_OK;
+}
+
+int bm_mpm_step_coupled(bm_mpm_handle h, int32_t method, float dt, int32_t k) {
+    if (int rc = check_handle(h)) return rc;
+    if (k < 0) return fail(BM_ERR_INVALID_ARG, "k < 0");
+    for (int32_t i = 0; i < k; ++i) {
+        if (int rc = bm_mpm_p2g(h, dt)) return rc;
+        if (int rc = bm_mpm_grid_update(h, dt)) return rc;
+        if (int rc = bm_mpm_couple_body(h, method, dt, nullptr)) return rc;
+        if (int rc = bm_mpm_g2p(h, dt)) return rc;
+    }
     return BM_OK;
 }
 

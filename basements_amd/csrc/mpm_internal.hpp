@@ -69,6 +69,72 @@ struct ColliderDev {
     int nb[3];  // blocks in the range per axis
 };
 
+// RigidColliderState -> ColliderDev, on the host for bm_mpm_couple and on the
+// device for the device body: the same FP32 operations, no contraction, so
+// both give the same bits.
+__host__ __device__ inline ColliderDev collider_build(const float center[3], const float q[4], const float he[3],
+                                                      const float vl[3], const float wa[3], float friction,
+                                                      float mass, int shape, const Domain& d) {
+#pragma clang fp contract(off)
+    ColliderDev c{};
+    const float w = q[0], x = q[1], y = q[2], z = q[3];
+    // Matrix3::from_quaternion, matrix3.h:432-448
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y,
+                wz = w * z;
+    const float R[9] = {1.0f - 2.0f * (yy + zz), 2.0f * (xy - wz),        2.0f * (xz + wy),
+                        2.0f * (xy + wz),        1.0f - 2.0f * (xx + zz), 2.0f * (yz - wx),
+                        2.0f * (xz - wy),        2.0f * (yz + wx),        1.0f - 2.0f * (xx + yy)};
+    for (int i = 0; i < 9; ++i) c.Rl[i] = R[i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) c.Rw[i * 3 + j] = R[j * 3 + i];
+    for (int a = 0; a < 3; ++a) {
+        c.c[a] = center[a];
+        c.he[a] = he[a];
+        c.vl[a] = vl[a];
+        c.wa[a] = wa[a];
+    }
+    c.friction = friction;
+    c.shape = shape;
+    // M1 per-node body mass from the box volume, mpm_rigid_coupler_pic.h:104-108
+    const float dx = d.dx;
+    const float vol = 8.0f * he[0] * he[1] * he[2];
+    const int approx = (int)roundf(vol / (dx * dx * dx));
+    c.mb = mass / (float)(approx > 1 ? approx : 1);
+    // |R| he world AABB, mpm_rigid_coupler.cpp:70-76
+    for (int a = 0; a < 3; ++a) {
+        const float hw = fabsf(R[a * 3 + 0]) * c.he[0] + fabsf(R[a * 3 + 1]) * c.he[1] + fabsf(R[a * 3 + 2]) * c.he[2];
+        c.lo[a] = c.c[a] - hw;
+        c.hi[a] = c.c[a] + hw;
+    }
+    // block range: every block whose node-centre box may overlap the AABB,
+    // widened by one block; the kernel applies the reference's exact cull.
+    const int org[3] = {d.ox, d.oy, d.oz};
+    const int nbv[3] = {d.nbx, d.nby, d.nbz};
+    for (int a = 0; a < 3; ++a) {
+        const double lo_n = floor((double)c.lo[a] / dx - 0.5) - org[a];
+        const double hi_n = ceil((double)c.hi[a] / dx - 0.5) - org[a];
+        long long blo = (long long)floor(lo_n / 4.0) - 1;
+        long long bhi = (long long)floor(hi_n / 4.0) + 1;
+        blo = blo > 0 ? blo : 0;
+        bhi = bhi < nbv[a] - 1 ? bhi : nbv[a] - 1;
+        c.b0[a] = (int)blo;
+        c.nb[a] = (bhi >= blo) ? (int)(bhi - blo + 1) : 0;
+    }
+    return c;
+}
+
+// Blocks of the coupler's scan range for any pose of a body with half
+// extents he: each axis of the |R| he box is at most |he| long each way.
+inline int64_t couple_blocks_bound(const float he[3], const Domain& d) {
+    const double r = sqrt((double)he[0] * he[0] + (double)he[1] * he[1] + (double)he[2] * he[2]);
+    const double nodes = ceil(2.0 * r / (double)d.dx) + 4.0;
+    const int64_t per = (int64_t)(nodes / 4.0) + 5;
+    const int nbv[3] = {d.nbx, d.nby, d.nbz};
+    int64_t tot = 1;
+    for (int a = 0; a < 3; ++a) tot *= per < nbv[a] ? per : nbv[a];
+    return tot;
+}
+
 // ------------------------------------------------------------- launchers
 void launch_clear(const GridBuf& g, int nblocks, hipStream_t s);
 void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Domain& d, const Phys& ph,
@@ -80,6 +146,13 @@ void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phy
 int couple_partials_needed(const ColliderDev& b);
 void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* bever, const Domain& d, float dt,
                    double* partials, double* out7, hipStream_t s);
+// Coupled substep of the device body: applied loads, collider from the body
+// pose, coupler over at most max_blocks blocks (a bound from
+// couple_blocks_bound), reaction into the body, integration.  err bit 2 is
+// set if the collider's range exceeded max_blocks (then nothing is coupled).
+void launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridBuf& g, uint8_t* bever,
+                        const Domain& d, float dt, int64_t max_blocks, double* partials, double* out7, int* err,
+                        hipStream_t s);
 void launch_grid_gather(const GridBuf& g, uint8_t* bever, const Domain& d, const int* org, const int* dims,
                         double qscale, int updated, long long* o_mq, float* o_mass, float* o_vel, float* o_force,
                         uint8_t* o_act, uint8_t* o_alloc, hipStream_t s);
@@ -117,9 +190,12 @@ void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, c
 constexpr int FUSED_UNIT_MAX = 512;
 // Two-kernel form (variant bit 2): per-particle G2P writing 12 deposit
 // coefficients per particle into `coef`, then the ranked per-block P2G.
+// `mark(ctx, stage)` is called on the host right before each stage's launch
+// (the timing hook of the handle).
+enum { SPLIT_ADVANCE = 0, SPLIT_REDO = 1, SPLIT_DEPOSIT = 2 };
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
-                  hipStream_t s, void (*between)(void*) = nullptr, void* ctx = nullptr);
+                  hipStream_t s, void (*mark)(void*, int) = nullptr, void* ctx = nullptr);
 
 }  // namespace bm

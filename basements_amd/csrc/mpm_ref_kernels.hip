@@ -285,8 +285,8 @@ __device__ __forceinline__ v3<float> shape_normal_d(const ColliderDev& b, v3<flo
 // One workgroup of 256 lanes = 4 grid blocks of the scanned range.  Each lane
 // produces one node's reaction contribution (7 values); the workgroup reduces
 // them in a fixed tree order in FP64 and writes one partial per workgroup.
-__global__ void __launch_bounds__(256) k_couple(int method, ColliderDev b, GridBuf g, uint8_t* bever, Domain d,
-                                                float dt, double* partials) {
+__device__ __forceinline__ void couple_wg(int method, const ColliderDev& b, const GridBuf& g, const uint8_t* bever,
+                                          const Domain& d, float dt, double* partials) {
     __shared__ double red[7][256];
     const int t = threadIdx.x;
     const int64_t total_blocks = (int64_t)b.nb[0] * b.nb[1] * b.nb[2];
@@ -397,15 +397,37 @@ __global__ void __launch_bounds__(256) k_couple(int method, ColliderDev b, GridB
     }
     if (t < 7) partials[(int64_t)blockIdx.x * 7 + t] = red[t][0];
 }
+__global__ void __launch_bounds__(256) k_couple(int method, ColliderDev b, GridBuf g, uint8_t* bever, Domain d,
+                                                float dt, double* partials) {
+    couple_wg(method, b, g, bever, d, dt, partials);
+}
+// The device body's coupler: the collider was built on the device.
+__global__ void __launch_bounds__(256) k_couple_dev(int method, const ColliderDev* bp, GridBuf g, uint8_t* bever,
+                                                    Domain d, float dt, double* partials) {
+    const ColliderDev b = *bp;
+    couple_wg(method, b, g, bever, d, dt, partials);
+}
 
-// Sum the per-workgroup partials in index order (deterministic).
+// Sum of the per-workgroup partials in a fixed order (lane t takes partials
+// t, t+64, ...; then a pairwise tree over the 64 lanes), so the reaction is
+// deterministic run to run; the loads of one lane are independent.
 __global__ void __launch_bounds__(64) k_reduce_partials(const double* partials, int nparts, double* out7) {
+    __shared__ double red[7][64];
     const int t = threadIdx.x;
-    if (t < 7) {
-        double s = 0.0;
-        for (int i = 0; i < nparts; ++i) s += partials[(int64_t)i * 7 + t];
-        out7[t] = s;
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = t; i < nparts; i += 64)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) s[c] += partials[(int64_t)i * 7 + c];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) red[c][t] = s[c];
+    __syncthreads();
+    for (int w = 32; w >= 1; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + w];
+        __syncthreads();
     }
+    if (t < 7) out7[t] = red[t][0];
 }
 
 // --------------------------------------------------------------- grid I/O
@@ -514,6 +536,118 @@ int couple_partials_needed(const ColliderDev& b) {
     const int64_t tb = (int64_t)b.nb[0] * b.nb[1] * b.nb[2];
     return (int)((tb + 3) / 4);
 }
+// ------------------------------------------------------------ device body
+// Start of the body's coupled substep (bench_compare_all.cpp:642-658): the
+// applied loads go into the accumulators (RigidBody::apply_force /
+// apply_torque, rigid_body.h:168-182), then the collider is built from the
+// body's pose and velocities.
+__global__ void k_body_begin(bm_body* body, ColliderDev* col, Domain d, long long max_blocks, int* err) {
+    if (threadIdx.x != 0) return;
+    bm_body b = *body;
+    for (int a = 0; a < 3; ++a) {
+        b.force[a] = b.force[a] + b.applied_force[a];
+        b.torque[a] = b.torque[a] + b.applied_torque[a];
+    }
+    ColliderDev c = collider_build(b.position, b.orientation, b.half_extents, b.linear_velocity, b.angular_velocity,
+                                   b.friction, b.mass, b.shape, d);
+    if ((long long)c.nb[0] * c.nb[1] * c.nb[2] > max_blocks) {
+        atomicOr(err, 4);
+        c.nb[0] = c.nb[1] = c.nb[2] = 0;
+    }
+    for (int a = 0; a < 3; ++a) {
+        body->force[a] = b.force[a];
+        body->torque[a] = b.torque[a];
+    }
+    *col = c;
+}
+
+// TimeIntegrator::integrate for one body (integrator.h:19-57): semi-implicit
+// Euler, per-body damping, q += 1/2 (0, w) q dt, normalised, forces cleared.
+__device__ void integrate_body_d(bm_body& b, float dt) {
+    if (!b.integrate) return;
+    float v[3], w[3];
+    for (int a = 0; a < 3; ++a) v[a] = b.linear_velocity[a] + (b.force[a] * b.inv_mass) * dt;
+    const float* I = b.inv_inertia;
+    const float it[3] = {I[0] * b.torque[0] + I[1] * b.torque[1] + I[2] * b.torque[2],
+                         I[3] * b.torque[0] + I[4] * b.torque[1] + I[5] * b.torque[2],
+                         I[6] * b.torque[0] + I[7] * b.torque[1] + I[8] * b.torque[2]};
+    for (int a = 0; a < 3; ++a) w[a] = b.angular_velocity[a] + it[a] * dt;
+    const float ld = 1.0f - b.linear_damping * dt, ad = 1.0f - b.angular_damping * dt;
+    for (int a = 0; a < 3; ++a) {
+        v[a] = v[a] * ld;
+        w[a] = w[a] * ad;
+        b.position[a] = b.position[a] + v[a] * dt;
+        b.linear_velocity[a] = v[a];
+        b.angular_velocity[a] = w[a];
+    }
+    // Quaternion product (0, w) * q, quaternion.h
+    const float ow = b.orientation[0], ox = b.orientation[1], oy = b.orientation[2], oz = b.orientation[3];
+    const float pw = 0.0f * ow - w[0] * ox - w[1] * oy - w[2] * oz;
+    const float px = 0.0f * ox + w[0] * ow + w[1] * oz - w[2] * oy;
+    const float py = 0.0f * oy - w[0] * oz + w[1] * ow + w[2] * ox;
+    const float pz = 0.0f * oz + w[0] * oy - w[1] * ox + w[2] * ow;
+    const float hs = 0.5f * dt;
+    float q[4] = {ow + pw * hs, ox + px * hs, oy + py * hs, oz + pz * hs};
+    const float n = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (n < 1e-6f) {
+        q[0] = 1.0f;
+        q[1] = q[2] = q[3] = 0.0f;
+    } else {
+        const float inv = 1.0f / n;
+        for (int a = 0; a < 4; ++a) q[a] = q[a] * inv;
+    }
+    for (int a = 0; a < 4; ++a) b.orientation[a] = q[a];
+    for (int a = 0; a < 3; ++a) {
+        b.force[a] = 0.0f;
+        b.torque[a] = 0.0f;
+    }
+}
+
+// End of the body's coupled substep (bench_compare_all.cpp:659-663): the
+// partials are summed as by k_reduce_partials, the reaction (rounded to
+// FP32, as CouplingReaction holds it) is added to the accumulators, then the
+// body is integrated.
+__global__ void __launch_bounds__(64) k_body_end(bm_body* body, const double* partials, int nparts, double* out7,
+                                                 float dt) {
+    __shared__ double red[7][64];
+    const int t = threadIdx.x;
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = t; i < nparts; i += 64)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) s[c] += partials[(int64_t)i * 7 + c];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) red[c][t] = s[c];
+    __syncthreads();
+    for (int w = 32; w >= 1; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + w];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    double r[7];
+    for (int c = 0; c < 7; ++c) r[c] = red[c][0];
+    for (int c = 0; c < 7; ++c) out7[c] = r[c];
+    bm_body b = *body;
+    for (int a = 0; a < 3; ++a) {
+        b.force[a] = b.force[a] + (float)r[a];
+        b.torque[a] = b.torque[a] + (float)r[3 + a];
+    }
+    integrate_body_d(b, dt);
+    *body = b;
+}
+
+void launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridBuf& g, uint8_t* bever,
+                        const Domain& d, float dt, int64_t max_blocks, double* partials, double* out7, int* err,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_body_begin, dim3(1), dim3(64), 0, s, body, col, d, (long long)max_blocks, err);
+    const int nparts = (dt > 0.0f) ? (int)((max_blocks + 3) / 4) : 0;  // dt <= 0: zero reaction
+    if (nparts > 0)
+        hipLaunchKernelGGL(k_couple_dev, dim3(nparts), dim3(256), 0, s, method, (const ColliderDev*)col, g, bever, d,
+                           dt, partials);
+    hipLaunchKernelGGL(k_body_end, dim3(1), dim3(64), 0, s, body, (const double*)partials, nparts, out7, dt);
+}
+
 void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* bever, const Domain& d, float dt,
                    double* partials, double* out7, hipStream_t s) {
     const int nparts = couple_partials_needed(b);

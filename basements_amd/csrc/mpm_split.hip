@@ -360,19 +360,22 @@ void launch_zero_i32(int* p, hipStream_t s);
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
-                  hipStream_t s, void (*between)(void*), void* ctx) {
+                  hipStream_t s, void (*mark)(void*, int), void* ctx) {
     if (P.n <= 0) return;
     FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo};
     launch_zero_i32(n_redo, s);
     const int g = (int)((P.n + 255) / 256);
-    if (do_g2p) {
+    if (mark) mark(ctx, SPLIT_ADVANCE);
+    if (do_g2p)
         hipLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, P, gin, d, ph, fa, coef, err);
-        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
-    } else {
+    else
         hipLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, P, gin, d, ph, fa, coef, err);
+    if (mark) mark(ctx, SPLIT_REDO);
+    if (do_g2p)
+        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
+    else
         hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
-    }
-    if (between) between(ctx);
+    if (mark) mark(ctx, SPLIT_DEPOSIT);
     if (n_units > 0) {
         const int nwg = ((n_units + 7) / 8) * 8;
         hipLaunchKernelGGL(k_p2g_cell, dim3(nwg), dim3(64), 0, s, P, coef, units, gout, bever, d, fa, err, counters);

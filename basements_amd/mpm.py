@@ -96,6 +96,66 @@ class Collider:
         return c
 
 
+class BodyC(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("orientation", C.c_float * 4), ("linear_velocity", C.c_float * 3),
+                ("angular_velocity", C.c_float * 3), ("force", C.c_float * 3), ("torque", C.c_float * 3),
+                ("inv_mass", C.c_float), ("inv_inertia", C.c_float * 9), ("linear_damping", C.c_float),
+                ("angular_damping", C.c_float), ("integrate", C.c_int32), ("applied_force", C.c_float * 3),
+                ("applied_torque", C.c_float * 3), ("half_extents", C.c_float * 3), ("friction", C.c_float),
+                ("mass", C.c_float), ("shape", C.c_int32)]
+
+
+@dataclass
+class Body:
+    """The device body of the coupled loop: RigidBody state (rigid_body.h:23-60)
+    plus the collider fields of RigidColliderState (mpm_rigid_coupler.h:72-84).
+    Defaults are RigidBody's (identity inertia, damping 0.01)."""
+
+    position: tuple = (0.0, 0.0, 0.0)
+    orientation: tuple = (1.0, 0.0, 0.0, 0.0)
+    linear_velocity: tuple = (0.0, 0.0, 0.0)
+    angular_velocity: tuple = (0.0, 0.0, 0.0)
+    force: tuple = (0.0, 0.0, 0.0)
+    torque: tuple = (0.0, 0.0, 0.0)
+    inv_mass: float = 1.0
+    inv_inertia: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+    linear_damping: float = 0.01
+    angular_damping: float = 0.01
+    integrate: bool = True
+    applied_force: tuple = (0.0, 0.0, 0.0)
+    applied_torque: tuple = (0.0, 0.0, 0.0)
+    half_extents: tuple = (0.5, 0.5, 0.5)
+    friction: float = 0.5
+    mass: float = 1.0
+    shape: int = SHAPE_BOX
+
+    def to_c(self) -> BodyC:
+        b = BodyC()
+        for f, _ in BodyC._fields_:
+            v = getattr(self, f)
+            if isinstance(v, (tuple, list, np.ndarray)):
+                getattr(b, f)[:] = [float(x) for x in v]
+            elif f in ("integrate", "shape"):
+                setattr(b, f, int(v))
+            else:
+                setattr(b, f, float(v))
+        return b
+
+    @staticmethod
+    def from_c(b: BodyC) -> "Body":
+        kw = {}
+        for f, _ in BodyC._fields_:
+            v = getattr(b, f)
+            kw[f] = tuple(v[:]) if hasattr(v, "__len__") else v
+        kw["integrate"] = bool(kw["integrate"])
+        return Body(**kw)
+
+    def collider(self) -> Collider:
+        return Collider(center=self.position, quat=self.orientation, half_extents=self.half_extents,
+                        lin_vel=self.linear_velocity, ang_vel=self.angular_velocity, friction=self.friction,
+                        mass=self.mass, shape=self.shape)
+
+
 # ------------------------------------------------------------------ library
 _lib = None
 
@@ -122,6 +182,10 @@ _SIGS = {
     "bm_mpm_grid_update": (C.c_int, [C.c_void_p, C.c_float]),
     "bm_mpm_g2p": (C.c_int, [C.c_void_p, C.c_float]),
     "bm_mpm_couple": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ColliderC), C.c_float, C.POINTER(ReactionC)]),
+    "bm_mpm_set_body": (C.c_int, [C.c_void_p, C.POINTER(BodyC)]),
+    "bm_mpm_get_body": (C.c_int, [C.c_void_p, C.POINTER(BodyC), C.POINTER(ReactionC)]),
+    "bm_mpm_couple_body": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.POINTER(ReactionC)]),
+    "bm_mpm_step_coupled": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_int32]),
     "bm_mpm_download_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p]),
     "bm_mpm_upload_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -283,6 +347,30 @@ class Solver:
         c = col.to_c()
         _check(lib().bm_mpm_couple(self.h, int(method), C.byref(c), float(dt), C.byref(r)))
         return np.array(r.force[:], np.float64), np.array(r.torque[:], np.float64), int(r.nodes_inside)
+
+    # device body -------------------------------------------------------
+    def set_body(self, body: Body):
+        b = body.to_c()
+        _check(lib().bm_mpm_set_body(self.h, C.byref(b)))
+
+    def get_body(self):
+        """(Body, (force, torque, nodes_inside) of the last coupling)."""
+        b, r = BodyC(), ReactionC()
+        _check(lib().bm_mpm_get_body(self.h, C.byref(b), C.byref(r)))
+        return Body.from_c(b), (np.array(r.force[:], np.float64), np.array(r.torque[:], np.float64),
+                                int(r.nodes_inside))
+
+    def couple_body(self, method, dt, wait=False):
+        """Couple the device body; with wait the reaction is returned."""
+        if not wait:
+            _check(lib().bm_mpm_couple_body(self.h, int(method), float(dt), None))
+            return None
+        r = ReactionC()
+        _check(lib().bm_mpm_couple_body(self.h, int(method), float(dt), C.byref(r)))
+        return np.array(r.force[:], np.float64), np.array(r.torque[:], np.float64), int(r.nodes_inside)
+
+    def step_coupled(self, method, dt, k=1):
+        _check(lib().bm_mpm_step_coupled(self.h, int(method), float(dt), int(k)))
 
     # grid --------------------------------------------------------------
     def download_grid(self, org=None, dims=None):

@@ -6,7 +6,8 @@ Workload (default): the single large scene -- a 1.0 x 0.3 x 1.0 m soil bed at
 friction angle from U(25, 40) deg, a cylinder wheel (r 0.1 m, width 0.08 m)
 rolling at 0.2 m/s under a 50 N load through the M1 PIC-rigid coupler, dt =
 1e-4 s.  One step = one coupled substep: P2G, grid update, M1 coupler with the
-reaction returned to the wheel, wheel integration, G2P with plasticity.
+reaction added to the wheel (a dynamic body under its 50 N weight), wheel
+integration, G2P with plasticity -- all on the device, no host round trip.
 
 Prints ONE JSON line: particle-updates/s (value), the roofline of the
 dominant kernel (HIP events on the handle's stream), and a CPU baseline (the
@@ -29,11 +30,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
-# Algorithmic bytes of each kernel: per particle and per active node.
-#   P2G reads x v C F m V0 (104 B) and adds m, momentum and force into each
-#   active node (28 B); the update reads those 28 B and writes v (12 B); G2P
-#   reads x F (48 B) and v of each active node (12 B) and writes x v C F (96 B).
-#   Total: 200 B per particle + 80 B per active node per substep.
+# Algorithmic bytes of each timed phase, (per particle, per active node): the
+# inputs and outputs of the reference operation the phase completes.
+#   P2G (p2g_single) reads x v C F m V0 (104 B) and adds m, momentum and
+#   force into each active node (28 B); the update reads those 28 B and
+#   writes v (12 B); G2P (g2p_single + apply_plasticity) reads x F (48 B) and
+#   v of each active node (12 B) and writes x v C F (96 B).
+# The substep as a whole must move 200 B per particle (state read once,
+# written once) + 80 B per active node.
 BYTES_REF = {
     "p2g": (104, 28),
     "update": (0, 40),
@@ -41,14 +45,28 @@ BYTES_REF = {
 }
 # Fast mode folds the stress force into the momentum (56 B per active node
 # per substep instead of 80): P2G deposits m + momentum (16 B), the update
-# reads 16 B and writes 12 B, G2P reads 12 B.  The fused G2P(n)+P2G(n+1)
-# kernel reads x F m V0 (56 B), writes x v C F (96 B) per particle, reads v
-# and deposits m + momentum per node.
+# reads 16 B and writes 12 B, G2P reads 12 B.  Two-kernel form: the particle
+# pass (phase g2p) completes G2P; the cell deposit (phase p2g) completes the
+# next substep's P2G.  The single-kernel form (g2p_p2g) completes both and
+# is credited with the state read once: x F m V0 (56 B) in, x v C F (96 B)
+# out per particle; v in and m + momentum out per node.
 BYTES_FAST = {
     "p2g": (104, 16),
     "update": (0, 28),
+    "g2p": (144, 12),
     "g2p_p2g": (152, 28),
 }
+# Kernel that each phase's roofline describes (names as rocprofv3 lists them).
+KERNEL_OF = {
+    ("fast", "g2p"): "k_advance<true>",
+    ("fast", "p2g"): "k_p2g_cell",
+    ("fast", "update"): "k_update",
+    ("fast", "g2p_p2g"): "k_fused",
+    ("reference", "p2g"): "k_p2g_ref",
+    ("reference", "update"): "k_update",
+    ("reference", "g2p"): "k_g2p_ref",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def _args():
@@ -73,38 +91,32 @@ def copy_calibration():
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(5):
-        b.copy_(a)
+        b.copy_(a)  # the runtime's copy kernel
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)  # an elementwise kernel: vector loads and stores, as the solver's kernels
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 5}))
 
 
-class Wheel:
-    """Cylinder wheel: driven at a fixed forward speed and rolling rate,
-    free vertically under its weight (the load) and the soil reaction;
-    semi-implicit Euler as TimeIntegrator::integrate (integrator.h:19-57)."""
+def make_body(tool, g=9.81):
+    """The scene's rigid body as the reference's scenarios build it: dynamic,
+    undamped, gravity as an applied force, inertia assigned directly so the
+    inverse inertia stays the identity (rigid_body.h:122,
+    scenario_rover_wheel.h:95-133, scenario_box_drop.h:60-67).  The wheel
+    rolls: local Y (axle) along world Z, v = 0.2 m/s forward, w = -v/R."""
+    from basements_amd import SHAPE_BOX, SHAPE_CYLINDER, Body
 
-    def __init__(self, tool, g=9.81):
-        self.R = tool["radius"]
-        self.L = tool["half_len"]
-        self.mass = tool["load"] / g
-        self.g = g
-        self.x = np.array(tool["center"], np.float64)
-        self.v = np.array([tool["speed"], 0.0, 0.0])
-        self.w = np.array([0.0, 0.0, -tool["speed"] / self.R])
-        self.q = (math.cos(math.pi / 4), math.sin(math.pi / 4), 0.0, 0.0)  # local Y (axle) -> world Z
-        self.friction = tool["friction"]
-
-    def collider(self):
-        from basements_amd import Collider, SHAPE_CYLINDER
-
-        return Collider(center=tuple(self.x), quat=self.q, half_extents=(self.R, self.L, self.R),
-                        lin_vel=tuple(self.v), ang_vel=tuple(self.w), friction=self.friction, mass=self.mass,
-                        shape=SHAPE_CYLINDER)
-
-    def integrate(self, force, dt):
-        fy = force[1] - self.mass * self.g
-        self.v[1] += fy / self.mass * dt
-        self.x += self.v * dt
+    if tool["kind"] == "cylinder":
+        m = tool["load"] / g
+        R = tool["radius"]
+        return Body(position=tool["center"], orientation=(math.cos(math.pi / 4), math.sin(math.pi / 4), 0.0, 0.0),
+                    linear_velocity=(tool["speed"], 0.0, 0.0), angular_velocity=(0.0, 0.0, -tool["speed"] / R),
+                    inv_mass=1.0 / m, linear_damping=0.0, angular_damping=0.0, applied_force=(0.0, -g * m, 0.0),
+                    half_extents=(R, tool["half_len"], R), friction=tool["friction"], mass=m, shape=SHAPE_CYLINDER)
+    m = tool.get("mass", 1.0)
+    return Body(position=tool["center"], inv_mass=1.0 / m, linear_damping=0.0, angular_damping=0.0,
+                applied_force=(0.0, -g * m, 0.0), half_extents=tool["half"], friction=tool["friction"], mass=m,
+                shape=SHAPE_BOX)
 
 
 def cpu_baseline(scene, nsteps):
@@ -135,6 +147,23 @@ def cpu_baseline(scene, nsteps):
             "seconds": t}
 
 
+def measured_traffic(workload, mode, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary:
+    rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one run each),
+    each scaled by what the same counter reads for a device copy of known
+    size.  None when that kernel has not been counted on this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get("kernels", {}).get(f"{workload}/{mode}/{kernel}")
+    if not e:
+        return None
+    b = e["fetch_kib"] * 1024 * t["fetch_scale"] + e["write_kib"] * 1024 * t["write_scale"]
+    return {"bytes": b, "source": TRAFFIC_FILE[len(ROOT) + 1:]}
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -160,32 +189,17 @@ def main():
 
     dt = scene.dt
     tool = scene.tool
-    wheel = Wheel(tool) if tool["kind"] == "cylinder" else None
     method = M1 if tool["method"] == 1 else M0
+    # One step = p2g, grid update, coupler with the reaction into the device
+    # body and its integration, g2p (bm_mpm_step_coupled): no host round trip.
+    S.set_body(make_body(tool))
 
-    if wheel is None:
-        from basements_amd import Collider
-
-        box = Collider(center=tool["center"], half_extents=tool["half"], friction=tool["friction"], mass=1.0)
-
-    def substep():
-        S.p2g(dt)
-        S.grid_update(dt)
-        if wheel is not None:
-            f, _, _ = S.couple(method, wheel.collider(), dt)
-            wheel.integrate(f, dt)
-        else:
-            S.couple(method, box, dt)
-        S.g2p(dt)
-
-    for _ in range(a.warmup):
-        substep()
+    S.step_coupled(method, dt, a.warmup)
     S.synchronize()
     S.set_timing(True)
     S.read_timing(reset=True)
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        substep()
+    S.step_coupled(method, dt, a.steps)
     S.synchronize()
     t = time.perf_counter() - t0
     tm = S.read_timing(reset=True)
@@ -208,10 +222,13 @@ def main():
             algo = bp * n + bn * n_nodes
             per_kernel[k] = {"avg_ms": ms / calls, "launches": calls, "algo_bytes": algo,
                              "achieved_GBs": algo / avg_s / 1e9}
+    for k, v in per_kernel.items():
+        v["kernel"] = KERNEL_OF.get((a.mode, k), k)
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
     ach = per_kernel[dom]["achieved_GBs"]
     sub_bytes = 200 * n + node_bytes * n_nodes
     gpu_ms = sum(ms for ms, _ in tm.values()) / a.steps
+    traffic = measured_traffic(scene.name, a.mode, per_kernel[dom]["kernel"])
 
     out = {
         "metric": "particle-updates/s",
@@ -229,9 +246,12 @@ def main():
         "config": {"workload": scene.name, "particles": n, "grid": list(scene.dims), "dx": scene.dx,
                    "dt": dt, "coupler": "M1" if method == M1 else "M0", "mode": a.mode,
                    "active_nodes": n_nodes, "active_blocks": st["active_blocks"]},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "algo_bytes_per_launch": per_kernel[dom]["algo_bytes"]},
+        "roofline": {"bound": "hbm", "kernel": per_kernel[dom]["kernel"], "phase": dom, "achieved": ach,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "algo_bytes_per_launch": per_kernel[dom]["algo_bytes"],
+                     "avg_ms": per_kernel[dom]["avg_ms"]},
         "substep_roofline": {"algo_bytes": sub_bytes, "bytes_per_node": node_bytes, "gpu_ms": gpu_ms,
                              "achieved_GBs": sub_bytes / (gpu_ms * 1e-3) / 1e9,
                              "frac": sub_bytes / (gpu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},

@@ -45,11 +45,12 @@ enum bm_status {
  *     of F), grid update, couplers, G2P with the plasticity SVD -- with the
  *     reference's operation order and FP32 Jacobi; force kept apart from
  *     momentum on the grid.  mpm_solver.h:71-87.
- *   BM_MODE_FAST: the same physics with G2P of substep n fused to P2G of
- *     substep n+1 in one kernel; the Kirchhoff stress of n+1 comes from the
- *     plasticity SVD of n (F_new = U diag(sigma') V^T is its own SVD), the
- *     stress force is folded into the node momentum, particles are kept
- *     binned by grid block, and the SVD has an FP64 fallback. */
+ *   BM_MODE_FAST: the same physics with G2P of substep n and the particle
+ *     half of P2G of substep n+1 in one pass over the particles, then a
+ *     per-cell deposit into the next grid; the Kirchhoff stress of n+1 comes
+ *     from the plasticity SVD of n (F_new = U diag(sigma') V^T is its own
+ *     SVD), the stress force is folded into the node momentum, particles are
+ *     kept binned by grid block, and the SVD has an FP64 fallback. */
 enum bm_mode { BM_MODE_REFERENCE = 0, BM_MODE_FAST = 1 };
 
 /* Coupler method ids; the numbering is the reference's M0/M1
@@ -167,6 +168,46 @@ int bm_mpm_g2p(bm_mpm_handle h, float dt);
  * Synchronous: the reaction is written to host memory.  dt <= 0 gives a zero
  * reaction, as in the reference (mpm_rigid_coupler.cpp:62, _pic.h:92). */
 int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* out);
+
+/* ---------------------------------------------------------------- device body */
+/* The rigid body of the coupled loop, kept in device memory so a coupled
+ * substep needs no host round trip.  Body fields follow RigidBody
+ * (rigid_body.h:23-60); the collider fields complete RigidColliderState
+ * (mpm_rigid_coupler.h:72-84), whose pose and velocities come from the body. */
+typedef struct {
+    float position[3];
+    float orientation[4];     /* quaternion (w, x, y, z) */
+    float linear_velocity[3];
+    float angular_velocity[3];
+    float force[3];           /* accumulators, cleared by every integration */
+    float torque[3];
+    float inv_mass;
+    float inv_inertia[9];     /* inv_inertia_tensor, body space, row-major */
+    float linear_damping;
+    float angular_damping;
+    int32_t integrate;        /* 0: STATIC or sleeping, integration skipped (integrator.h:25) */
+    float applied_force[3];   /* added to force at the start of each coupled substep (gravity * mass) */
+    float applied_torque[3];  /* added to torque likewise (a drive torque) */
+    float half_extents[3];    /* collider shape, as bm_collider */
+    float friction;
+    float mass;               /* RigidColliderState::mass */
+    int32_t shape;
+} bm_body;
+
+/* Set / read the device body.  get also returns the reaction of the last
+ * coupled substep (reaction may be NULL); both synchronize the stream. */
+int bm_mpm_set_body(bm_mpm_handle h, const bm_body* body);
+int bm_mpm_get_body(bm_mpm_handle h, bm_body* body, bm_reaction* reaction);
+/* One coupling of the device body, the body half of the reference's coupled
+ * loop (bench_compare_all.cpp:642-663): force += applied_force, torque +=
+ * applied_torque; reaction = coupler(grid, collider of the body, dt); force +=
+ * reaction.force, torque += reaction.torque; TimeIntegrator::integrate
+ * (integrator.h:19-57).  Between bm_mpm_grid_update and bm_mpm_g2p.  out may
+ * be NULL: then nothing is copied to the host and the call does not wait. */
+int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* out);
+/* k coupled substeps (p2g, grid update, couple_body, g2p) with no host
+ * round trip: the order of mpm_rigid_coupler.h:209-211. */
+int bm_mpm_step_coupled(bm_mpm_handle h, int32_t method, float dt, int32_t k);
 
 /* ---------------------------------------------------------------- grid access */
 /* Dense copy of the node box [org, org + dims) (get_grid(), mpm_solver.h:90).

@@ -1,0 +1,107 @@
+"""The C ABI without a GPU: the library loads, exports every function the
+header declares, its structs have the layout the Python mirror assumes, and
+argument checks that precede any device call answer with the documented
+status codes."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bm_mpm.h")
+
+
+def _lib_or_skip():
+    from basements_amd import lib
+    from basements_amd.mpm import LIB_PATH
+
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("libbm_mpm.so is not built")
+    return lib()
+
+
+def test_library_exports_every_header_symbol():
+    from basements_amd import header_symbols
+
+    L = _lib_or_skip()
+    names = header_symbols(HEADER)
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.bm_abi_version() == 1
+
+
+def test_python_signatures_cover_the_header():
+    from basements_amd import header_symbols
+    from basements_amd.mpm import _SIGS
+
+    assert sorted(_SIGS) == header_symbols(HEADER)
+
+
+def test_argument_errors_without_device():
+    from basements_amd.mpm import Desc
+
+    L = _lib_or_skip()
+    h = C.c_void_p()
+    assert L.bm_mpm_create(None, C.byref(h)) == 1  # BM_ERR_INVALID_ARG
+    assert b"null" in L.bm_last_error()
+    d = Desc()
+    d.dx = -1.0
+    d.grid_dims[:] = [8, 8, 8]
+    assert L.bm_mpm_create(C.byref(d), C.byref(h)) == 1
+    assert b"dx" in L.bm_last_error()
+    d.dx = 0.1
+    d.mode = 5
+    assert L.bm_mpm_create(C.byref(d), C.byref(h)) == 1
+    assert L.bm_mpm_step(None, 1e-4) == 1
+    assert L.bm_mpm_destroy(None) == 0
+
+
+_LAYOUT_C = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "bm_mpm.h"
+#define F(T, m) printf(#T " " #m " %zu\n", offsetof(T, m))
+#define S(T) printf(#T " sizeof %zu\n", sizeof(T))
+int main(void) {
+  S(bm_mpm_desc); F(bm_mpm_desc, dx); F(bm_mpm_desc, grid_origin); F(bm_mpm_desc, grid_dims);
+  F(bm_mpm_desc, max_particles); F(bm_mpm_desc, mode); F(bm_mpm_desc, sort_every); F(bm_mpm_desc, stream);
+  S(bm_particles); F(bm_particles, V0);
+  S(bm_particles_device); F(bm_particles_device, capacity);
+  S(bm_collider); F(bm_collider, quat); F(bm_collider, friction); F(bm_collider, shape);
+  S(bm_reaction); F(bm_reaction, nodes_inside);
+  S(bm_body); F(bm_body, inv_mass); F(bm_body, integrate); F(bm_body, applied_torque); F(bm_body, shape);
+  return 0;
+}
+"""
+
+
+def test_struct_layouts_match_the_python_mirror(tmp_path):
+    from basements_amd import mpm
+
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "layout.c"
+    src.write_text(_LAYOUT_C)
+    exe = tmp_path / "layout"
+    inc = os.path.join(ROOT, "include")
+    r = subprocess.run([cc, "-std=c11", f"-I{inc}", str(src), "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    want = {}
+    for line in out:
+        if line:
+            t, m, v = line.split()
+            want[(t, m)] = int(v)
+    py = {"bm_mpm_desc": mpm.Desc, "bm_particles": mpm.HostParticles, "bm_particles_device": mpm.DeviceParticles,
+          "bm_collider": mpm.ColliderC, "bm_reaction": mpm.ReactionC, "bm_body": mpm.BodyC}
+    for (t, m), v in want.items():
+        if m == "sizeof":
+            assert C.sizeof(py[t]) == v, (t, C.sizeof(py[t]), v)
+        else:
+            assert getattr(py[t], m).offset == v, (t, m, getattr(py[t], m).offset, v)
