@@ -555,18 +555,29 @@ static int rebin(bm_mpm_handle h) {
 static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const GridBuf& gout, float dt_g2p,
                          float dt_fold) {
     if (h->variant & 4) {
-        // Time each kernel on its own: the caller's phase (G2P, or P2G for a
-        // deposit of the current state) covers the particle pass only, the
-        // cell deposit is P2G, the small bookkeeping launches are OTHER.
+        // Time each kernel on its own: the particle pass under the caller's
+        // phase (G2P, or P2G for a deposit of the current state) and the cell
+        // deposit under P2G, each with events recorded by its own dispatch;
+        // the small bookkeeping launches between them are OTHER.
         struct Ctx {
             bm_mpm_handle h;
             int adv_phase;
         } ctx{h, h->open_phase};
-        auto mark = [](void* p, int stage) {
+        auto mark = [](void* p, int stage, hipEvent_t* a, hipEvent_t* b) {
             auto* c = (Ctx*)p;
-            if (!c->h->timing || c->adv_phase < 0) return;
-            c->h->end();
-            c->h->begin(stage == SPLIT_ADVANCE ? c->adv_phase : stage == SPLIT_REDO ? BM_PHASE_OTHER : BM_PHASE_P2G);
+            bm_mpm_handle hh = c->h;
+            if (!hh->timing || c->adv_phase < 0) return;
+            if (stage == SPLIT_REDO) {
+                hh->begin(BM_PHASE_OTHER);
+                return;
+            }
+            hh->end();
+            *a = hh->take_event();
+            *b = hh->take_event();
+            if (*a && *b)
+                hh->pending.push_back(bm_mpm_solver::Timed{stage == SPLIT_ADVANCE ? c->adv_phase : BM_PHASE_P2G, *a, *b});
+            else
+                *a = *b = nullptr;
         };
         if (ctx.adv_phase >= 0) h->open_phase = BM_PHASE_OTHER;  // the opening event now starts OTHER
         launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,

@@ -52,8 +52,8 @@ __host__ __device__ inline int64_t node_of(const Domain& d, int lx, int ly, int 
     return block_of(d, lx, ly, lz) * 64 + ((lx & 3) * 4 + (ly & 3)) * 4 + (lz & 3);
 }
 
-// Collider in the form the coupler kernel wants: rotation matrices built on
-// the host from the quaternion (matrix3.h:432-448), the |R| he world AABB
+// Collider in the form the coupler kernel wants: rotation matrices built
+// from the quaternion (matrix3.h:432-448), the |R| he world AABB
 // (mpm_rigid_coupler.cpp:70-76) and the block range to scan.
 struct ColliderDev {
     float c[3];
@@ -190,12 +190,14 @@ void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, c
 constexpr int FUSED_UNIT_MAX = 512;
 // Two-kernel form (variant bit 2): per-particle G2P writing 12 deposit
 // coefficients per particle into `coef`, then the ranked per-block P2G.
-// `mark(ctx, stage)` is called on the host right before each stage's launch
-// (the timing hook of the handle).
+// `mark(ctx, stage, start, stop)` is called on the host right before each
+// stage's launch (the timing hook of the handle); for the two main kernels it
+// may hand back a pair of events that the dispatch itself records.
 enum { SPLIT_ADVANCE = 0, SPLIT_REDO = 1, SPLIT_DEPOSIT = 2 };
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
-                  hipStream_t s, void (*mark)(void*, int) = nullptr, void* ctx = nullptr);
+                  hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*) = nullptr,
+                  void* ctx = nullptr);
 
 }  // namespace bm

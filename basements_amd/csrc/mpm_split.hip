@@ -12,6 +12,8 @@
 // Deposit coefficients (12 floats per particle, component-major like the
 // particle arrays): m v (3) and m C + dt K (9, row-major), K = -4/dx^2 V0 tau.
 
+#include <hip/hip_ext.h>
+
 #include "mpm_fast_device.hpp"
 
 namespace bm {
@@ -360,25 +362,29 @@ void launch_zero_i32(int* p, hipStream_t s);
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
-                  hipStream_t s, void (*mark)(void*, int), void* ctx) {
+                  hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx) {
     if (P.n <= 0) return;
     FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo};
     launch_zero_i32(n_redo, s);
     const int g = (int)((P.n + 255) / 256);
-    if (mark) mark(ctx, SPLIT_ADVANCE);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (mark) mark(ctx, SPLIT_ADVANCE, &e0, &e1);
+    // with timing events the launch records them around the dispatch itself
     if (do_g2p)
-        hipLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, P, gin, d, ph, fa, coef, err);
+        hipExtLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, coef, err);
     else
-        hipLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, P, gin, d, ph, fa, coef, err);
-    if (mark) mark(ctx, SPLIT_REDO);
+        hipExtLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, coef, err);
+    if (mark) mark(ctx, SPLIT_REDO, nullptr, nullptr);
     if (do_g2p)
         hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
     else
         hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
-    if (mark) mark(ctx, SPLIT_DEPOSIT);
+    e0 = e1 = nullptr;
+    if (mark) mark(ctx, SPLIT_DEPOSIT, &e0, &e1);
     if (n_units > 0) {
         const int nwg = ((n_units + 7) / 8) * 8;
-        hipLaunchKernelGGL(k_p2g_cell, dim3(nwg), dim3(64), 0, s, P, coef, units, gout, bever, d, fa, err, counters);
+        hipExtLaunchKernelGGL(k_p2g_cell, dim3(nwg), dim3(64), 0, s, e0, e1, 0, P, (const float*)coef, units, gout,
+                              bever, d, fa, err, counters);
     }
 }
 

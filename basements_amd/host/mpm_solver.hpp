@@ -1,0 +1,355 @@
+// mpm_solver.hpp -- C++ host mirror of basements::mpm::MPMSolver over the C ABI
+// of libbm_mpm.so (include/bm_mpm.h).
+//
+// Same class, method names and argument meaning as the reference
+// (include/basements/physics/mpm/mpm_solver.h:15-95), so code written against
+// the reference compiles against this header after a namespace change:
+//   initialize(dx), add_particle(pos, vel, mass), set_material(E, nu, rho),
+//   set_gravity(g), set_plastic_alpha(a), set_floor_friction(mu), step(dt),
+//   get_particles(), get_particles_mutable(), get_grid().
+// get_grid() returns a host snapshot with the reference grid's shape:
+// get_blocks() maps block keys (spgrid_cpu.h:92-98) to 4^3-node blocks of
+// GridNode {velocity, force, mass, active} (spgrid_cpu.h:16-24).
+//
+// The work runs on the GPU.  Particles are staged on the host until the first
+// step; the device grid is then a dense node box around the particle cloud
+// plus `margin_cells` on each side (the reference's grid is unbounded).
+// get_particles() copies back lazily; edits through get_particles_mutable()
+// are uploaded before the next step, and once a mutable view has been handed
+// out, step() refreshes it in place as the reference's step() does.  Errors from the C ABI throw
+// basements_amd::mpm::Error (the reference does no checking at all).
+#pragma once
+
+#include <bm_mpm.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace basements_amd {
+namespace math {
+
+struct Vec3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    Vec3() = default;
+    Vec3(float a, float b, float c) : x(a), y(b), z(c) {}
+    explicit Vec3(float s) : x(s), y(s), z(s) {}
+    static Vec3 zero() { return Vec3(); }
+    Vec3 operator+(const Vec3& o) const { return Vec3(x + o.x, y + o.y, z + o.z); }
+    Vec3 operator-(const Vec3& o) const { return Vec3(x - o.x, y - o.y, z - o.z); }
+    Vec3 operator*(float s) const { return Vec3(x * s, y * s, z * s); }
+    float dot(const Vec3& o) const { return (x * o.x + y * o.y) + z * o.z; }
+    float length() const { return std::sqrt(dot(*this)); }
+    float length_squared() const { return dot(*this); }
+};
+
+struct Matrix3 {
+    float m[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    static Matrix3 identity() {
+        Matrix3 r;
+        r.m[0][0] = r.m[1][1] = r.m[2][2] = 1.0f;
+        return r;
+    }
+    static Matrix3 zero() { return Matrix3(); }
+    static Matrix3 scale(const Vec3& d) {
+        Matrix3 r;
+        r.m[0][0] = d.x;
+        r.m[1][1] = d.y;
+        r.m[2][2] = d.z;
+        return r;
+    }
+    float operator()(int i, int j) const { return m[i][j]; }
+    float& operator()(int i, int j) { return m[i][j]; }
+};
+
+}  // namespace math
+
+namespace mpm {
+
+using math::Matrix3;
+using math::Vec3;
+
+constexpr int BLOCK_SIZE = 4;
+
+// struct Particle, particle.h:9-23
+struct Particle {
+    Vec3 position;
+    Vec3 velocity;
+    Matrix3 F = Matrix3::identity();
+    Matrix3 C = Matrix3::zero();
+    float mass = 1.0f;
+    float volume = 1.0f;
+    int material_id = 0;
+};
+
+// GridNode / GridBlock, spgrid_cpu.h:16-35
+struct GridNode {
+    Vec3 velocity;
+    Vec3 force;
+    float mass = 0.0f;
+    bool active = false;
+};
+struct GridBlock {
+    GridNode nodes[BLOCK_SIZE * BLOCK_SIZE * BLOCK_SIZE];
+    int origin_x = 0, origin_y = 0, origin_z = 0;  // node index of the block's first node
+};
+
+// Host snapshot of the device grid in the reference's block form.
+class SPGridView {
+   public:
+    explicit SPGridView(float dx = 0.1f) : dx_(dx) {}
+    // block key packing of spgrid_cpu.h:92-98: (coordinate + 100000) in 21-bit
+    // fields, x lowest
+    static uint64_t get_block_key(int bx, int by, int bz) {
+        const uint64_t ux = (uint64_t)(bx + 100000) & 0x1FFFFF, uy = (uint64_t)(by + 100000) & 0x1FFFFF,
+                       uz = (uint64_t)(bz + 100000) & 0x1FFFFF;
+        return ux | (uy << 21) | (uz << 42);
+    }
+    const std::unordered_map<uint64_t, GridBlock>& get_blocks() const { return blocks_; }
+    float get_dx() const { return dx_; }
+    // node (i, j, k) or nullptr if its block was never allocated
+    const GridNode* get_node(int i, int j, int k) const {
+        const auto it = blocks_.find(get_block_key(floordiv(i), floordiv(j), floordiv(k)));
+        if (it == blocks_.end()) return nullptr;
+        return &it->second.nodes[(mod4(i) * 4 + mod4(j)) * 4 + mod4(k)];
+    }
+
+   private:
+    friend class MPMSolver;
+    static int floordiv(int a) { return a >= 0 ? a / 4 : -((-a + 3) / 4); }
+    static int mod4(int a) { return a - 4 * floordiv(a); }
+    float dx_;
+    std::unordered_map<uint64_t, GridBlock> blocks_;
+};
+
+class Error : public std::runtime_error {
+   public:
+    Error(int code, const std::string& what) : std::runtime_error(what), code(code) {}
+    int code;
+};
+
+inline void check(int rc) {
+    if (rc != BM_OK) throw Error(rc, std::string("libbm_mpm: ") + bm_last_error());
+}
+
+class MPMSolver {
+   public:
+    explicit MPMSolver(int mode = BM_MODE_REFERENCE, int margin_cells = 16) : mode_(mode), margin_(margin_cells) {
+        if (bm_abi_version() != BM_MPM_ABI_VERSION) throw Error(BM_ERR_NOT_SUPPORTED, "libbm_mpm ABI mismatch");
+    }
+    ~MPMSolver() { release(); }
+    MPMSolver(const MPMSolver&) = delete;
+    MPMSolver& operator=(const MPMSolver&) = delete;
+
+    // mpm_solver.h:21
+    void initialize(float dx) {
+        dx_ = dx;
+        materialize();
+        release();
+    }
+    // mpm_solver.h:23-34
+    void add_particle(const Vec3& pos, const Vec3& vel, float mass = 1.0f) {
+        materialize();
+        Particle p;
+        p.position = pos;
+        p.velocity = vel;
+        p.mass = mass;
+        p.volume = rho_ > 1e-12f ? mass / rho_ : 1.0f;
+        host_.push_back(p);
+        dirty_ = true;
+    }
+    // mpm_solver.h:42-51: V0 of existing particles follows a density change
+    void set_material(float E, float nu, float rho) {
+        E_ = E;
+        nu_ = nu;
+        if (rho > 1e-12f && rho_ > 1e-12f && rho != rho_) {
+            materialize();
+            const float s = rho_ / rho;
+            for (auto& p : host_) p.volume *= s;
+            if (!host_.empty()) dirty_ = true;
+        }
+        rho_ = rho;
+    }
+    void set_gravity(const Vec3& g) { g_ = g; }
+    void set_plastic_alpha(float a) { alpha_ = a; }
+    void set_floor_friction(float mu) { mu_floor_ = mu; }
+
+    // mpm_solver.h:62-87
+    void step(float dt) {
+        sync_device();
+        check(bm_mpm_step(h_, dt));
+        host_valid_ = false;
+        if (view_out_) materialize();
+    }
+    // Split phases and the couplers (the reference keeps the phases private).
+    void p2g(float dt) {
+        sync_device();
+        check(bm_mpm_p2g(h_, dt));
+        host_valid_ = false;
+    }
+    void grid_update(float dt) { check(bm_mpm_grid_update(need(), dt)); }
+    void g2p(float dt) {
+        check(bm_mpm_g2p(need(), dt));
+        host_valid_ = false;
+        if (view_out_) materialize();
+    }
+    bm_reaction couple(int method, const bm_collider& c, float dt) {
+        bm_reaction r{};
+        check(bm_mpm_couple(need(), method, &c, dt, &r));
+        return r;
+    }
+
+    const std::vector<Particle>& get_particles() {
+        materialize();
+        return host_;
+    }
+    std::vector<Particle>& get_particles_mutable() {
+        materialize();
+        dirty_ = true;
+        view_out_ = true;
+        return host_;
+    }
+    // mpm_solver.h:90: a snapshot of every allocated block
+    SPGridView get_grid() const {
+        SPGridView v(dx_);
+        if (!h_) return v;
+        const int64_t nn = (int64_t)dims_[0] * dims_[1] * dims_[2];
+        std::vector<float> mass(nn), vel(3 * nn), force(3 * nn);
+        std::vector<uint8_t> act(nn), alloc(nn);
+        check(bm_mpm_download_grid(h_, org_, dims_, nullptr, mass.data(), vel.data(),
+                                   mode_ == BM_MODE_REFERENCE ? force.data() : nullptr, act.data(), alloc.data()));
+        for (int i = 0; i < dims_[0]; ++i)
+            for (int j = 0; j < dims_[1]; ++j)
+                for (int k = 0; k < dims_[2]; ++k) {
+                    const int64_t idx = ((int64_t)i * dims_[1] + j) * dims_[2] + k;
+                    if (!alloc[idx]) continue;
+                    const int gi = org_[0] + i, gj = org_[1] + j, gk = org_[2] + k;
+                    const int bx = SPGridView::floordiv(gi), by = SPGridView::floordiv(gj),
+                              bz = SPGridView::floordiv(gk);
+                    GridBlock& b = v.blocks_[SPGridView::get_block_key(bx, by, bz)];
+                    b.origin_x = bx * BLOCK_SIZE;
+                    b.origin_y = by * BLOCK_SIZE;
+                    b.origin_z = bz * BLOCK_SIZE;
+                    GridNode& n = b.nodes[(SPGridView::mod4(gi) * 4 + SPGridView::mod4(gj)) * 4 + SPGridView::mod4(gk)];
+                    n.mass = mass[idx];
+                    n.velocity = Vec3(vel[3 * idx], vel[3 * idx + 1], vel[3 * idx + 2]);
+                    n.force = Vec3(force[3 * idx], force[3 * idx + 1], force[3 * idx + 2]);
+                    n.active = act[idx] != 0;
+                }
+        return v;
+    }
+    bm_mpm_handle handle() { return need(); }
+
+   private:
+    bm_mpm_handle need() {
+        if (!h_) throw Error(BM_ERR_STATE, "no device state yet: call step() or p2g() first");
+        return h_;
+    }
+    void release() {
+        if (h_) bm_mpm_destroy(h_);
+        h_ = nullptr;
+    }
+    void materialize() {
+        if (host_valid_) return;
+        host_valid_ = true;
+        if (!h_) return;
+        int64_t n = 0;
+        check(bm_mpm_num_particles(h_, &n));
+        std::vector<float> x(3 * n), v(3 * n), F(9 * n), C(9 * n), m(n), V0(n);
+        bm_particles hp{x.data(), v.data(), F.data(), C.data(), m.data(), V0.data()};
+        check(bm_mpm_download_particles(h_, &hp));
+        host_.resize(n);  // in place: references handed out stay valid
+        for (int64_t i = 0; i < n; ++i) {
+            Particle& p = host_[i];
+            p.position = Vec3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+            p.velocity = Vec3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    p.F.m[r][c] = F[9 * i + 3 * r + c];
+                    p.C.m[r][c] = C[9 * i + 3 * r + c];
+                }
+            p.mass = m[i];
+            p.volume = V0[i];
+        }
+    }
+    void sync_device() {
+        materialize();
+        const int64_t n = (int64_t)host_.size();
+        if (!h_ || n > cap_) {
+            release();
+            place_box();
+            bm_mpm_desc d{};
+            d.dx = dx_;
+            for (int a = 0; a < 3; ++a) {
+                d.grid_origin[a] = org_[a];
+                d.grid_dims[a] = dims_[a];
+            }
+            cap_ = std::max<int64_t>(n, 1);
+            d.max_particles = cap_;
+            d.mode = mode_;
+            check(bm_mpm_create(&d, &h_));
+            for (int a = 0; a < 3; ++a) dims_[a] = (dims_[a] + 3) & ~3;
+            dirty_ = true;
+        }
+        check(bm_mpm_set_material(h_, E_, nu_, rho_));
+        check(bm_mpm_set_gravity(h_, g_.x, g_.y, g_.z));
+        check(bm_mpm_set_plastic_alpha(h_, alpha_));
+        check(bm_mpm_set_floor_friction(h_, mu_floor_));
+        if (dirty_ || view_out_) {  // a caller holding a mutable view may have edited it
+            std::vector<float> x(3 * n), v(3 * n), F(9 * n), C(9 * n), m(n), V0(n);
+            for (int64_t i = 0; i < n; ++i) {
+                const Particle& p = host_[i];
+                x[3 * i] = p.position.x, x[3 * i + 1] = p.position.y, x[3 * i + 2] = p.position.z;
+                v[3 * i] = p.velocity.x, v[3 * i + 1] = p.velocity.y, v[3 * i + 2] = p.velocity.z;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) {
+                        F[9 * i + 3 * r + c] = p.F.m[r][c];
+                        C[9 * i + 3 * r + c] = p.C.m[r][c];
+                    }
+                m[i] = p.mass;
+                V0[i] = p.volume;
+            }
+            bm_particles hp{x.data(), v.data(), F.data(), C.data(), m.data(), V0.data()};
+            check(bm_mpm_upload_particles(h_, n, &hp));
+            dirty_ = false;
+        }
+    }
+    // node box: the particles' stencil range plus the margin on every side
+    void place_box() {
+        float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (size_t i = 0; i < host_.size(); ++i) {
+            const float p[3] = {host_[i].position.x, host_[i].position.y, host_[i].position.z};
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = i ? std::min(lo[a], p[a]) : p[a];
+                hi[a] = i ? std::max(hi[a], p[a]) : p[a];
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            const int l = (int)std::floor(lo[a] / dx_ - 0.5f) - margin_;
+            const int u = (int)std::floor(hi[a] / dx_ - 0.5f) + 2 + margin_;
+            org_[a] = l;
+            dims_[a] = u - l + 1;
+        }
+    }
+
+    int mode_;
+    int margin_;
+    float dx_ = 0.1f;  // MPMSolver(): grid(0.1f), mpm_solver.h:17
+    // defaults of mpm_solver.h:100-109
+    float E_ = 1.0e5f, nu_ = 0.3f, rho_ = 1500.0f, alpha_ = 0.40f, mu_floor_ = 0.4f;
+    Vec3 g_{0.0f, -9.81f, 0.0f};
+    std::vector<Particle> host_;
+    bool host_valid_ = true;
+    bool dirty_ = false;
+    bool view_out_ = false;  // a mutable view was handed out
+    bm_mpm_handle h_ = nullptr;
+    int64_t cap_ = 0;
+    int32_t org_[3] = {0, 0, 0}, dims_[3] = {4, 4, 4};
+};
+
+}  // namespace mpm
+}  // namespace basements_amd
