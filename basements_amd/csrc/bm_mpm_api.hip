@@ -59,12 +59,12 @@ struct bm_mpm_solver {
     unsigned long long* d_fallbacks = nullptr;
     double* d_partials = nullptr;
     double* d_out7 = nullptr;
-    int max_partials = 0;
     // device body of the coupled loop
     bm_body* d_body = nullptr;
     ColliderDev* d_col = nullptr;
     int64_t body_blocks = 0;  // scan-range bound of the body's collider
     bool body_set = false;
+    bool body_begun = false;  // the grid update already started the body's substep
 
     // MPMSolver parameters (mpm_solver.h:100-109 defaults)
     float E = 1.0e5f, nu = 0.3f, rho = 1500.0f, alpha = 0.40f, mu_floor = 0.4f;
@@ -297,8 +297,7 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
     }
     if (hipMalloc(&h->bever, h->nblocks) != hipSuccess) return bail(fail(BM_ERR_OOM, "block flags"));
     hipMemset(h->bever, 0, h->nblocks);
-    h->max_partials = (int)((h->nblocks + 3) / 4) + 1;
-    if (hipMalloc(&h->d_partials, sizeof(double) * 7 * h->max_partials) != hipSuccess)
+    if (hipMalloc(&h->d_partials, sizeof(double) * 7 * COUPLE_WGS) != hipSuccess)
         return bail(fail(BM_ERR_OOM, "coupler partials"));
     if (hipMalloc(&h->d_out7, sizeof(double) * 8) != hipSuccess) return bail(fail(BM_ERR_OOM, "coupler out"));
     if (hipMalloc(&h->d_body, sizeof(bm_body)) != hipSuccess) return bail(fail(BM_ERR_OOM, "device body"));
@@ -600,7 +599,7 @@ static int p2g_fast(bm_mpm_handle h, float dt) {
         if (int rc = rebin(h)) return rc;
     GridBuf& g = h->grid[h->cur];
     h->begin(BM_PHASE_CLEAR);
-    launch_clear(g, (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream);
+    launch_clear(g, (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream, h->n_redo, (h->variant & 4) != 0);
     h->end();
     h->begin(BM_PHASE_P2G);
     fast_kernels(h, false, g, g, dt, dt);
@@ -617,7 +616,8 @@ static int g2p_fast(bm_mpm_handle h, float dt) {
         if (int rc = rebin(h)) return rc;
     const int nxt = 1 - h->cur;
     h->begin(BM_PHASE_CLEAR);
-    launch_clear(h->grid[nxt], (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream);
+    launch_clear(h->grid[nxt], (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream, h->n_redo,
+                 (h->variant & 4) != 0);
     h->end();
     h->begin((h->variant & 4) ? BM_PHASE_G2P : BM_PHASE_G2P_P2G);
     fast_kernels(h, true, h->grid[h->cur], h->grid[nxt], dt, dt);
@@ -646,15 +646,26 @@ int bm_mpm_p2g(bm_mpm_handle h, float dt) {
     return BM_OK;
 }
 
-int bm_mpm_grid_update(bm_mpm_handle h, float dt) {
-    if (int rc = check_handle(h)) return rc;
+// with_body: the update kernel also starts the device body's coupled substep
+// (the caller couples the body next).
+static int grid_update_impl(bm_mpm_handle h, float dt, bool with_body) {
     if (h->phase != PH_P2G) return fail(BM_ERR_STATE, "grid_update needs a preceding p2g");
     h->begin(BM_PHASE_UPDATE);
-    launch_update(h->grid[h->cur], (int)h->nblocks, h->dom, h->phys(), h->qscale(), dt, h->stream);
+    if (with_body)
+        launch_update(h->grid[h->cur], (int)h->nblocks, h->dom, h->phys(), h->qscale(), dt, h->stream, h->d_body,
+                      h->d_col, h->body_blocks, h->d_err);
+    else
+        launch_update(h->grid[h->cur], (int)h->nblocks, h->dom, h->phys(), h->qscale(), dt, h->stream);
     h->end();
     HIP_TRY(hipGetLastError());
     h->phase = PH_UPDATED;
+    h->body_begun = with_body;
     return BM_OK;
+}
+
+int bm_mpm_grid_update(bm_mpm_handle h, float dt) {
+    if (int rc = check_handle(h)) return rc;
+    return grid_update_impl(h, dt, false);
 }
 
 int bm_mpm_g2p(bm_mpm_handle h, float dt) {
@@ -696,7 +707,6 @@ int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, floa
     if (!(dt > 0.0f)) return BM_OK;
     if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "couple needs an updated grid");
     ColliderDev c = make_collider(h, body);
-    if (couple_partials_needed(c) > h->max_partials) return fail(BM_ERR_INVALID_ARG, "collider range too large");
     h->begin(BM_PHASE_COUPLE);
     launch_couple(method, c, h->grid[h->cur], h->bever, h->dom, dt, h->d_partials, h->d_out7, h->stream);
     h->end();
@@ -722,7 +732,6 @@ int bm_mpm_set_body(bm_mpm_handle h, const bm_body* body) {
         if (!(body->half_extents[a] >= 0.0f) || !std::isfinite(body->half_extents[a]))
             return fail(BM_ERR_INVALID_ARG, "half extents must be finite and >= 0");
     const int64_t blocks = couple_blocks_bound(body->half_extents, h->dom);
-    if ((blocks + 3) / 4 > h->max_partials) return fail(BM_ERR_INVALID_ARG, "collider range too large");
     HIP_TRY(hipMemcpyAsync(h->d_body, body, sizeof(bm_body), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->body_blocks = blocks;
@@ -756,7 +765,8 @@ int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* o
     if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "couple needs an updated grid");
     h->begin(BM_PHASE_COUPLE);
     launch_couple_body(method, h->d_body, h->d_col, h->grid[h->cur], h->bever, h->dom, dt, h->body_blocks,
-                       h->d_partials, h->d_out7, h->d_err, h->stream);
+                       h->d_partials, h->d_out7, h->d_err, h->body_begun, h->stream);
+    h->body_begun = false;
     h->end();
     HIP_TRY(hipGetLastError());
     if (!out) return BM_OK;
@@ -774,9 +784,12 @@ int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* o
 int bm_mpm_step_coupled(bm_mpm_handle h, int32_t method, float dt, int32_t k) {
     if (int rc = check_handle(h)) return rc;
     if (k < 0) return fail(BM_ERR_INVALID_ARG, "k < 0");
+    if (method != BM_COUPLER_M0_DIRICHLET && method != BM_COUPLER_M1_PIC_RIGID)
+        return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
+    if (!h->body_set) return fail(BM_ERR_STATE, "no device body set");
     for (int32_t i = 0; i < k; ++i) {
         if (int rc = bm_mpm_p2g(h, dt)) return rc;
-        if (int rc = bm_mpm_grid_update(h, dt)) return rc;
+        if (int rc = grid_update_impl(h, dt, true)) return rc;
         if (int rc = bm_mpm_couple_body(h, method, dt, nullptr)) return rc;
         if (int rc = bm_mpm_g2p(h, dt)) return rc;
     }

@@ -260,9 +260,11 @@ __device__ __forceinline__ m3<float> recompose(const m3<float>& U, v3<float> s, 
 // max(1, rint(w * q_p)) quanta, the stencil centre (1,1,1) the remainder, so a
 // particle always deposits exactly q_p quanta and a node is active iff its
 // integer mass is nonzero.
-__device__ __forceinline__ long long mass_quanta(float w, double qp) {
+// Quanta of one node: max(1, rint(w q_p)) for w != 0.  q_p < 2^23 and
+// w <= 1, so the double product is exact and the result fits 32 bits.
+__device__ __forceinline__ int mass_quanta(float w, double qp) {
     if (w == 0.0f) return 0;
-    long long q = (long long)rint((double)w * qp);
+    const int q = (int)rint((double)w * qp);
     return q < 1 ? 1 : q;
 }
 

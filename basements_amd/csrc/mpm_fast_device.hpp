@@ -122,6 +122,7 @@ struct FastArgs {
     double qscale_inv;
     int* redo;      // particles whose SVD needs the FP64 pass
     int* n_redo;
+    int* list_reset;  // block-list length of the output grid, zeroed by the particle pass (two-kernel form)
 };
 
 __device__ __forceinline__ bool stencil_ok(const Domain& d, int lx, int ly, int lz) {
@@ -240,7 +241,7 @@ __device__ __forceinline__ void scatter(const Stencil& s, float dx, double qp, c
                                         const m3<float>& Aff, SinkFn sink) {
     // the stencil centre (1,1,1) is deposited last and takes the remainder,
     // so each other node's quanta are used as soon as they are formed
-    long long acc = 0;
+    int acc = 0;
     v3<float> mom_c;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -260,19 +261,19 @@ __device__ __forceinline__ void scatter(const Stencil& s, float dx, double qp, c
                     mom_c = mom;
                     continue;
                 }
-                const long long dq = mass_quanta(w, qp);
+                const int dq = mass_quanta(w, qp);
                 acc += dq;
                 sink(a, b, c, w, dq, mom);
             }
         }
     }
-    sink(1, 1, 1, s.w[0][1] * s.w[1][1] * s.w[2][1], (long long)qp - acc, mom_c);
+    sink(1, 1, 1, s.w[0][1] * s.w[1][1] * s.w[2][1], (int)qp - acc, mom_c);
 }
 
 __device__ __forceinline__ void deposit_global(const GridBuf& g, uint8_t* bever, const Domain& d, int lx, int ly,
-                                               int lz, long long dq, v3<float> mom) {
+                                               int lz, int dq, v3<float> mom) {
     const int64_t nid = node_of(d, lx, ly, lz);
-    atomicAdd((unsigned long long*)&g.mq[nid], (unsigned long long)dq);
+    atomicAdd((unsigned long long*)&g.mq[nid], (unsigned long long)(long long)dq);
     float* nm = (float*)&g.mv[nid];
     atomicAdd(nm + 0, mom.x);
     atomicAdd(nm + 1, mom.y);

@@ -165,14 +165,12 @@ __global__ void __launch_bounds__(64) k_redo(PView P, GridBuf gin, GridBuf gout,
             continue;
         }
         const double qp = rint((double)mass * fa.qscale_inv);
-        scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+        scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
             if (w != 0.0f) deposit_global(gout, bever, d, lx + a, ly + b, lz + c, dq, mom);
         });
         atomicAdd(&counters[0], 1ull);
     }
 }
-
-__global__ void k_zero_i32(int* p) { *p = 0; }
 
 // ================================================================ launchers
 static inline int grid_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -219,6 +217,5 @@ void launch_redo(bool do_g2p, const PView& P, const GridBuf& gin, const GridBuf&
         hipLaunchKernelGGL(k_redo<false>, dim3(32), dim3(64), 0, s, P, gin, gout, bever, d, ph, fa, err, counters);
 }
 
-void launch_zero_i32(int* p, hipStream_t s) { hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, p); }
 
 }  // namespace bm

@@ -192,18 +192,18 @@ struct Fused {
         const float dx = d.dx;
         if (D.state == 2) {
             const int lx = D.rx + tx0, ly = D.ry + ty0, lz = D.rz + tz0;
-            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
                 if (w != 0.0f) deposit_global(gout, bever, d, lx + a, ly + b, lz + c, dq, mom);
             });
             return 1;
         }
         if (RMW && own) {
-            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
                 rmw(S, tix(D.rx + a, D.ry + b, D.rz + c), mom, dq);
             });
         } else {
             // zero weights deposit zero: no per-node branch is needed
-            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+            scatter(D.s, dx, D.qp, D.mvel, D.Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
                 const int k = tix(D.rx + a, D.ry + b, D.rz + c);
                 float* q = (float*)&S.pt[k];
                 atomicAdd(q + 0, mom.x);
@@ -374,7 +374,6 @@ __global__ void __launch_bounds__(64) k_fused(PView P, const int4* __restrict__ 
 void launch_redo(bool do_g2p, const PView& P, const GridBuf& gin, const GridBuf& gout, uint8_t* bever,
                  const Domain& d, const Phys& ph, const FastArgs& fa, int* err, unsigned long long* counters,
                  hipStream_t s);
-void launch_zero_i32(int* p, hipStream_t s);
 
 template <bool DO_G2P, bool RMW, bool GTILE>
 static void go(int nwg, hipStream_t s, const PView& P, const int4* units, const GridBuf& gin, const GridBuf& gout,
@@ -393,9 +392,9 @@ void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, int variant,
                   hipStream_t s) {
     if (n_units <= 0) return;
-    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo};
+    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, nullptr};
     const int nwg = ((n_units + 7) / 8) * 8;
-    launch_zero_i32(n_redo, s);
+    // n_redo was zeroed by the grid clear that precedes every fast pass
     const bool rmw = variant & 1, gtile = variant & 2;
     if (do_g2p) {
         if (rmw && gtile) go<true, true, true>(nwg, s, P, units, gin, gout, bever, d, ph, fa, err, counters);

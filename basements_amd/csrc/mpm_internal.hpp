@@ -136,23 +136,33 @@ inline int64_t couple_blocks_bound(const float he[3], const Domain& d) {
 }
 
 // ------------------------------------------------------------- launchers
-void launch_clear(const GridBuf& g, int nblocks, hipStream_t s);
+// Zero the blocks on g's list and empty the list; also zeroes *zero_word
+// (a counter of the next kernels) when given.  defer_reset: the list length
+// is left for the next kernel to zero (the fast particle pass does).
+void launch_clear(const GridBuf& g, int nblocks, hipStream_t s, int* zero_word = nullptr, bool defer_reset = false);
 void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Domain& d, const Phys& ph,
                     double qscale_inv, int* err, hipStream_t s);
+// Grid update; with a device body also the start of its coupled substep
+// (applied loads, collider), as k_body_begin.
 void launch_update(const GridBuf& g, int nblocks, const Domain& d, const Phys& ph, double qscale, float dt,
-                   hipStream_t s);
+                   hipStream_t s, bm_body* body = nullptr, ColliderDev* col = nullptr, int64_t max_blocks = 0,
+                   int* err = nullptr);
 void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phys& ph, float dt, int* err,
                     hipStream_t s);
-int couple_partials_needed(const ColliderDev& b);
+// The couplers run a fixed number of workgroups over the scanned blocks, so
+// the FP64 partial sums -- and the reaction -- do not depend on how the scan
+// range was bounded (host-built or device-built collider).
+constexpr int COUPLE_WGS = 128;
 void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* bever, const Domain& d, float dt,
                    double* partials, double* out7, hipStream_t s);
 // Coupled substep of the device body: applied loads, collider from the body
 // pose, coupler over at most max_blocks blocks (a bound from
 // couple_blocks_bound), reaction into the body, integration.  err bit 2 is
 // set if the collider's range exceeded max_blocks (then nothing is coupled).
+// begun = true: the start was already done (fused into launch_update).
 void launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridBuf& g, uint8_t* bever,
                         const Domain& d, float dt, int64_t max_blocks, double* partials, double* out7, int* err,
-                        hipStream_t s);
+                        bool begun, hipStream_t s);
 void launch_grid_gather(const GridBuf& g, uint8_t* bever, const Domain& d, const int* org, const int* dims,
                         double qscale, int updated, long long* o_mq, float* o_mass, float* o_vel, float* o_force,
                         uint8_t* o_act, uint8_t* o_alloc, hipStream_t s);

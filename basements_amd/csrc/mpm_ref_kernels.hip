@@ -17,8 +17,10 @@ namespace bm {
 
 // --------------------------------------------------------------- bookkeeping
 // Zero every node of the blocks listed in g and drop their flags.  One wave
-// per block; grid-stride over the device-side count.
-__global__ void __launch_bounds__(64) k_clear_blocks(GridBuf g) {
+// per block; grid-stride over the device-side count.  The list length is
+// zeroed by a later kernel (k_reset_count or the fast particle pass).
+// zero_word, when given, is a counter of a later kernel zeroed on the way.
+__global__ void __launch_bounds__(64) k_clear_blocks(GridBuf g, int* zero_word) {
     const int cnt = *g.bcount;
     for (int b = blockIdx.x; b < cnt; b += gridDim.x) {
         const int blk = g.blist[b];
@@ -28,6 +30,7 @@ __global__ void __launch_bounds__(64) k_clear_blocks(GridBuf g) {
         if (g.f) g.f[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (threadIdx.x == 0) g.bflag[blk] = 0u;
     }
+    if (zero_word && blockIdx.x == 0 && threadIdx.x == 0) *zero_word = 0;
 }
 
 __global__ void k_reset_count(GridBuf g) { *g.bcount = 0; }
@@ -135,11 +138,17 @@ __global__ void __launch_bounds__(256) k_p2g_ref(PView P, GridBuf g, uint8_t* be
             }
 }
 
+__device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, long long max_blocks, int* err);
+
 // --------------------------------------------------------------- grid update
-// mpm_solver.h:250-291 plus the optional domain walls.  force_on selects
-// whether a separate force buffer is folded in (reference mode) or the force
-// was already folded into the momentum by P2G (fast mode).
-__global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d, Phys ph, double qscale, float dt) {
+// mpm_solver.h:250-291 plus the optional domain walls.  A force buffer is
+// folded in when present (reference mode); in fast mode P2G already folded
+// the force into the momentum.  With a device body, one lane also starts the
+// body's coupled substep (body_begin), which touches no node.
+
+__global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d, Phys ph, double qscale, float dt, bm_body* body,
+                                               ColliderDev* col, long long max_blocks, int* err) {
+    if (body && blockIdx.x == 0 && threadIdx.x == 0) body_begin(body, col, d, max_blocks, err);
     const int cnt = *g.bcount;
     for (int b = blockIdx.x; b < cnt; b += gridDim.x) {
         const int blk = g.blist[b];
@@ -282,17 +291,18 @@ __device__ __forceinline__ v3<float> shape_normal_d(const ColliderDev& b, v3<flo
     return mk(0.f, 0.f, l.z >= 0.f ? 1.f : -1.f);
 }
 
-// One workgroup of 256 lanes = 4 grid blocks of the scanned range.  Each lane
-// produces one node's reaction contribution (7 values); the workgroup reduces
-// them in a fixed tree order in FP64 and writes one partial per workgroup.
+// The scanned range is cut into chunks of 4 grid blocks; workgroup w of the
+// fixed COUPLE_WGS takes chunks w, w + COUPLE_WGS, ...  Lane t handles node
+// t & 63 of block t >> 6 of each chunk and accumulates its reaction
+// contribution (7 values) in FP64; the workgroup reduces them in a fixed tree
+// order and writes one partial per workgroup.
 __device__ __forceinline__ void couple_wg(int method, const ColliderDev& b, const GridBuf& g, const uint8_t* bever,
                                           const Domain& d, float dt, double* partials) {
     __shared__ double red[7][256];
     const int t = threadIdx.x;
     const int64_t total_blocks = (int64_t)b.nb[0] * b.nb[1] * b.nb[2];
-    const int64_t rb = (int64_t)blockIdx.x * 4 + (t >> 6);
     double c7[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (rb < total_blocks) {
+    for (int64_t rb = (int64_t)blockIdx.x * 4 + (t >> 6); rb < total_blocks; rb += (int64_t)gridDim.x * 4) {
         const int rbz = (int)(rb % b.nb[2]);
         const int rby = (int)((rb / b.nb[2]) % b.nb[1]);
         const int rbx = (int)(rb / ((int64_t)b.nb[2] * b.nb[1]));
@@ -349,13 +359,13 @@ __device__ __forceinline__ void couple_wg(int method, const ColliderDev& b, cons
                         g.mv[nid] = q;
                         const v3<float> fo = divg(dp, dt);
                         const v3<float> to = divg(cross(r, dp), dt);
-                        c7[0] = -(double)fo.x;
-                        c7[1] = -(double)fo.y;
-                        c7[2] = -(double)fo.z;
-                        c7[3] = -(double)to.x;
-                        c7[4] = -(double)to.y;
-                        c7[5] = -(double)to.z;
-                        c7[6] = 1.0;
+                        c7[0] += -(double)fo.x;
+                        c7[1] += -(double)fo.y;
+                        c7[2] += -(double)fo.z;
+                        c7[3] += -(double)to.x;
+                        c7[4] += -(double)to.y;
+                        c7[5] += -(double)to.z;
+                        c7[6] += 1.0;
                     } else {
                         // mpm_rigid_coupler_pic.h:145-177
                         v3<float> va;
@@ -373,13 +383,13 @@ __device__ __forceinline__ void couple_wg(int method, const ColliderDev& b, cons
                         g.mv[nid] = q;
                         const v3<float> dF = (va - vb) * (b.mb / dt);
                         const v3<float> to = cross(r, dF);
-                        c7[0] = dF.x;
-                        c7[1] = dF.y;
-                        c7[2] = dF.z;
-                        c7[3] = to.x;
-                        c7[4] = to.y;
-                        c7[5] = to.z;
-                        c7[6] = 1.0;
+                        c7[0] += dF.x;
+                        c7[1] += dF.y;
+                        c7[2] += dF.z;
+                        c7[3] += to.x;
+                        c7[4] += to.y;
+                        c7[5] += to.z;
+                        c7[6] += 1.0;
                     }
                 }
             }
@@ -399,12 +409,6 @@ __device__ __forceinline__ void couple_wg(int method, const ColliderDev& b, cons
 }
 __global__ void __launch_bounds__(256) k_couple(int method, ColliderDev b, GridBuf g, uint8_t* bever, Domain d,
                                                 float dt, double* partials) {
-    couple_wg(method, b, g, bever, d, dt, partials);
-}
-// The device body's coupler: the collider was built on the device.
-__global__ void __launch_bounds__(256) k_couple_dev(int method, const ColliderDev* bp, GridBuf g, uint8_t* bever,
-                                                    Domain d, float dt, double* partials) {
-    const ColliderDev b = *bp;
     couple_wg(method, b, g, bever, d, dt, partials);
 }
 
@@ -512,10 +516,10 @@ __global__ void k_max_mass(PView P, unsigned int* out) {
 // ================================================================ launchers
 static inline int grid_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
-void launch_clear(const GridBuf& g, int nblocks, hipStream_t s) {
+void launch_clear(const GridBuf& g, int nblocks, hipStream_t s, int* zero_word, bool defer_reset) {
     const int grid = nblocks < 8192 ? (nblocks > 0 ? nblocks : 1) : 8192;
-    hipLaunchKernelGGL(k_clear_blocks, dim3(grid), dim3(64), 0, s, g);
-    hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s, g);
+    hipLaunchKernelGGL(k_clear_blocks, dim3(grid), dim3(64), 0, s, g, zero_word);
+    if (!defer_reset) hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s, g);
 }
 void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Domain& d, const Phys& ph,
                     double qscale_inv, int* err, hipStream_t s) {
@@ -523,26 +527,22 @@ void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Doma
     hipLaunchKernelGGL(k_p2g_ref, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, g, bever, d, ph, qscale_inv, err);
 }
 void launch_update(const GridBuf& g, int nblocks, const Domain& d, const Phys& ph, double qscale, float dt,
-                   hipStream_t s) {
+                   hipStream_t s, bm_body* body, ColliderDev* col, int64_t max_blocks, int* err) {
     const int grid = nblocks < 16384 ? (nblocks > 0 ? nblocks : 1) : 16384;
-    hipLaunchKernelGGL(k_update, dim3(grid), dim3(64), 0, s, g, d, ph, qscale, dt);
+    hipLaunchKernelGGL(k_update, dim3(grid), dim3(64), 0, s, g, d, ph, qscale, dt, body, col, (long long)max_blocks,
+                       err);
 }
 void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phys& ph, float dt, int* err,
                     hipStream_t s) {
     if (P.n == 0) return;
     hipLaunchKernelGGL(k_g2p_ref, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, g, d, ph, dt, err);
 }
-int couple_partials_needed(const ColliderDev& b) {
-    const int64_t tb = (int64_t)b.nb[0] * b.nb[1] * b.nb[2];
-    return (int)((tb + 3) / 4);
-}
 // ------------------------------------------------------------ device body
 // Start of the body's coupled substep (bench_compare_all.cpp:642-658): the
 // applied loads go into the accumulators (RigidBody::apply_force /
 // apply_torque, rigid_body.h:168-182), then the collider is built from the
 // body's pose and velocities.
-__global__ void k_body_begin(bm_body* body, ColliderDev* col, Domain d, long long max_blocks, int* err) {
-    if (threadIdx.x != 0) return;
+__device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, long long max_blocks, int* err) {
     bm_body b = *body;
     for (int a = 0; a < 3; ++a) {
         b.force[a] = b.force[a] + b.applied_force[a];
@@ -559,6 +559,9 @@ __global__ void k_body_begin(bm_body* body, ColliderDev* col, Domain d, long lon
         body->torque[a] = b.torque[a];
     }
     *col = c;
+}
+__global__ void k_body_begin(bm_body* body, ColliderDev* col, Domain d, long long max_blocks, int* err) {
+    if (threadIdx.x == 0) body_begin(body, col, d, max_blocks, err);
 }
 
 // TimeIntegrator::integrate for one body (integrator.h:19-57): semi-implicit
@@ -604,19 +607,21 @@ __device__ void integrate_body_d(bm_body& b, float dt) {
 }
 
 // End of the body's coupled substep (bench_compare_all.cpp:659-663): the
-// partials are summed as by k_reduce_partials, the reaction (rounded to
-// FP32, as CouplingReaction holds it) is added to the accumulators, then the
-// body is integrated.
-__global__ void __launch_bounds__(64) k_body_end(bm_body* body, const double* partials, int nparts, double* out7,
-                                                 float dt) {
+// partials are summed as by k_reduce_partials (the first 64 lanes; every lane
+// of the workgroup reaches the barriers), the reaction (rounded to FP32, as
+// CouplingReaction holds it) is added to the accumulators, then the body is
+// integrated.
+__device__ void body_end(bm_body* body, const double* partials, int nparts, double* out7, float dt) {
     __shared__ double red[7][64];
     const int t = threadIdx.x;
-    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int i = t; i < nparts; i += 64)
+    if (t < 64) {
+        double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int i = t; i < nparts; i += 64)
 #pragma unroll
-        for (int c = 0; c < 7; ++c) s[c] += partials[(int64_t)i * 7 + c];
+            for (int c = 0; c < 7; ++c) s[c] += partials[(int64_t)i * 7 + c];
 #pragma unroll
-    for (int c = 0; c < 7; ++c) red[c][t] = s[c];
+        for (int c = 0; c < 7; ++c) red[c][t] = s[c];
+    }
     __syncthreads();
     for (int w = 32; w >= 1; w >>= 1) {
         if (t < w)
@@ -637,23 +642,33 @@ __global__ void __launch_bounds__(64) k_body_end(bm_body* body, const double* pa
     *body = b;
 }
 
+__global__ void __launch_bounds__(64) k_body_end(bm_body* body, const double* partials, int nparts, double* out7,
+                                                 float dt) {
+    body_end(body, partials, nparts, out7, dt);
+}
+
+// The device body's coupler: the collider was built on the device.
+__global__ void __launch_bounds__(256) k_couple_body(int method, const ColliderDev* bp, GridBuf g, uint8_t* bever,
+                                                     Domain d, float dt, double* partials) {
+    const ColliderDev b = *bp;
+    couple_wg(method, b, g, bever, d, dt, partials);
+}
+
 void launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridBuf& g, uint8_t* bever,
                         const Domain& d, float dt, int64_t max_blocks, double* partials, double* out7, int* err,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_body_begin, dim3(1), dim3(64), 0, s, body, col, d, (long long)max_blocks, err);
-    const int nparts = (dt > 0.0f) ? (int)((max_blocks + 3) / 4) : 0;  // dt <= 0: zero reaction
+                        bool begun, hipStream_t s) {
+    if (!begun) hipLaunchKernelGGL(k_body_begin, dim3(1), dim3(64), 0, s, body, col, d, (long long)max_blocks, err);
+    const int nparts = (dt > 0.0f) ? COUPLE_WGS : 0;  // dt <= 0: zero reaction
     if (nparts > 0)
-        hipLaunchKernelGGL(k_couple_dev, dim3(nparts), dim3(256), 0, s, method, (const ColliderDev*)col, g, bever, d,
+        hipLaunchKernelGGL(k_couple_body, dim3(nparts), dim3(256), 0, s, method, (const ColliderDev*)col, g, bever, d,
                            dt, partials);
     hipLaunchKernelGGL(k_body_end, dim3(1), dim3(64), 0, s, body, (const double*)partials, nparts, out7, dt);
 }
 
 void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* bever, const Domain& d, float dt,
                    double* partials, double* out7, hipStream_t s) {
-    const int nparts = couple_partials_needed(b);
-    if (nparts > 0)
-        hipLaunchKernelGGL(k_couple, dim3(nparts), dim3(256), 0, s, method, b, g, bever, d, dt, partials);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partials, nparts, out7);
+    hipLaunchKernelGGL(k_couple, dim3(COUPLE_WGS), dim3(256), 0, s, method, b, g, bever, d, dt, partials);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partials, COUPLE_WGS, out7);
 }
 void launch_grid_gather(const GridBuf& g, uint8_t* bever, const Domain& d, const int* org, const int* dims,
                         double qscale, int updated, long long* o_mq, float* o_mass, float* o_vel, float* o_force,

@@ -37,11 +37,16 @@ __device__ __forceinline__ void store_coef(float* coef, int64_t cap, int64_t i, 
     for (int c = 0; c < 9; ++c) coef[(3 + c) * cap + i] = Aff.a[c];
 }
 
+__global__ void k_set_zero(int* p) { *p = 0; }
+
 // G2P + plasticity + write-back + deposit coefficients; one lane per particle.
 template <bool DO_G2P>
 __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float* coef,
                                                  int* err) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the clear before this pass left the output grid's block list to be
+    // emptied here, before the deposit kernel appends to it
+    if (i == 0 && fa.list_reset) *fa.list_reset = 0;
     if (i >= P.n) return;
     const float dx = d.dx, inv_dx = d.inv_dx;
     float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
@@ -268,7 +273,7 @@ __global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restric
 #pragma unroll
         for (int c = 0; c < 9; ++c) Aff.a[c] = coef[(3 + c) * P.cap + i];
         const double qp = rint((double)P.p[PM * P.cap + i] * fa.qscale_inv);
-        scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+        scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
             const int n = (a * 3 + b) * 3 + c;
             acc[n] = acc[n] + mom;
             accm[n] += (unsigned)dq;
@@ -315,7 +320,7 @@ __global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restric
         }
         const int qx = lx - tx0, qy = ly - ty0, qz = lz - tz0;
         if (qx >= 0 && qy >= 0 && qz >= 0 && qx <= TE - 3 && qy <= TE - 3 && qz <= TE - 3) {
-            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
                 float* q = (float*)&S.pt[stix(qx + a, qy + b, qz + c)];
                 atomicAdd(q + 0, mom.x);
                 atomicAdd(q + 1, mom.y);
@@ -324,7 +329,7 @@ __global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restric
             });
         } else {
             ++off;
-            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, long long dq, v3<float> mom) {
+            scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
                 if (w != 0.0f) deposit_global(gout, bever, d, lx + a, ly + b, lz + c, dq, mom);
             });
         }
@@ -355,7 +360,6 @@ __global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restric
     }
 }
 
-void launch_zero_i32(int* p, hipStream_t s);
 
 // G2P (or the coefficients of the current state when !do_g2p), the FP64
 // pass for the particles it listed, then the ranked P2G.
@@ -363,9 +367,12 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx) {
-    if (P.n <= 0) return;
-    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo};
-    launch_zero_i32(n_redo, s);
+    if (P.n <= 0) {  // no particle pass to empty the output grid's block list
+        hipLaunchKernelGGL(k_set_zero, dim3(1), dim3(1), 0, s, gout.bcount);
+        return;
+    }
+    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, gout.bcount};
+    // n_redo was zeroed by the grid clear that precedes every fast pass
     const int g = (int)((P.n + 255) / 256);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (mark) mark(ctx, SPLIT_ADVANCE, &e0, &e1);

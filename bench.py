@@ -256,6 +256,7 @@ def main():
                              "achieved_GBs": sub_bytes / (gpu_ms * 1e-3) / 1e9,
                              "frac": sub_bytes / (gpu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernels": per_kernel,
+        "phase_ms_per_step": {k: ms / a.steps for k, (ms, calls) in tm.items() if calls},
         "fp64_svd_fallbacks": st["fp64_fallbacks"],
     }
     if a.cpu_baseline:
