@@ -82,7 +82,7 @@ struct bm_mpm_solver {
     int* pid2 = nullptr;
     int* redo = nullptr;    // particles redone with the FP64 SVD (one list per launch)
     int* n_redo = nullptr;
-    float* coef = nullptr;  // deposit coefficients of the two-kernel form (12 per particle)
+    float* coef = nullptr;  // deposit records of the two-kernel form (16 floats per particle)
     BinScratch bs{};
     int n_units = 0;
     bool binned = false;      // units describe the current particle order
@@ -291,7 +291,7 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
             hipMalloc(&b.units, sizeof(int4) * (pc / 16 + h->nblocks + 1)) != hipSuccess ||
             hipMalloc(&b.n_units, sizeof(int)) != hipSuccess ||
             hipMalloc(&h->redo, sizeof(int) * pc) != hipSuccess || hipMalloc(&h->n_redo, sizeof(int)) != hipSuccess ||
-            hipMalloc(&h->coef, sizeof(float) * 12 * pc) != hipSuccess ||
+            hipMalloc(&h->coef, sizeof(float) * 16 * ((pc + 63) / 64 * 64)) != hipSuccess ||
             hipMalloc(&b.temp, std::max<size_t>(b.temp_bytes, 16)) != hipSuccess)
             return bail(fail(BM_ERR_OOM, "fast-mode scratch"));
     }
@@ -581,7 +581,7 @@ static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const
         if (ctx.adv_phase >= 0) h->open_phase = BM_PHASE_OTHER;  // the opening event now starts OTHER
         launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
                      dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream, mark,
-                     &ctx);
+                     &ctx, (h->variant & 8) != 0);
     } else
         launch_fused(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
                      dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant & 3,
@@ -913,7 +913,7 @@ int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_ever
     if (int rc = check_handle(h)) return rc;
     if (unit_particles > FUSED_UNIT_MAX || (unit_particles > 0 && unit_particles < 16))
         return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", FUSED_UNIT_MAX);
-    if (variant > 7) return fail(BM_ERR_INVALID_ARG, "unknown variant %d", variant);
+    if (variant > 15) return fail(BM_ERR_INVALID_ARG, "unknown variant %d", variant);
     if (unit_particles > 0 && unit_particles != h->unit_p) {
         h->unit_p = unit_particles;
         h->binned = false;  // units must be rebuilt

@@ -198,8 +198,8 @@ void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, c
 // Largest unit the fast kernels accept: 512 particles of at most 2^23 - 1
 // mass quanta each keep every LDS node-mass sum below 2^32.
 constexpr int FUSED_UNIT_MAX = 512;
-// Two-kernel form (variant bit 2): per-particle G2P writing 12 deposit
-// coefficients per particle into `coef`, then the ranked per-block P2G.
+// Two-kernel form (variant bit 2): per-particle G2P writing a 16-float
+// deposit record per particle into `coef`, then the per-cell block P2G.
 // `mark(ctx, stage, start, stop)` is called on the host right before each
 // stage's launch (the timing hook of the handle); for the two main kernels it
 // may hand back a pair of events that the dispatch itself records.
@@ -208,6 +208,6 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*) = nullptr,
-                  void* ctx = nullptr);
+                  void* ctx = nullptr, bool two_pass = false);
 
 }  // namespace bm

@@ -4,13 +4,19 @@
 // particles by their *final* stencil cell and deposits with ordered LDS
 // read-modify-writes.
 //
-// Compared with the fused kernel it moves 112 more bytes per particle
-// (coefficients written and read back, position read twice) but neither
-// kernel holds the other's registers: the G2P kernel runs without LDS, and
-// the P2G kernel without the SVD.
+// Compared with the fused kernel it moves 128 more bytes per particle (the
+// deposit record written and read back) but neither kernel holds the other's
+// registers: the G2P kernel runs without LDS, and the P2G kernel without the
+// SVD.
 //
-// Deposit coefficients (12 floats per particle, component-major like the
-// particle arrays): m v (3) and m C + dt K (9, row-major), K = -4/dx^2 V0 tau.
+// Deposit record: 16 floats per particle as four float4 quads
+//   (x, y, z, q_p)  (m v, A00)  (A01, A02, A10, A11)  (A12, A20, A21, A22)
+// stored quad-major within groups of 64 particles (quad q of particle i at
+// float4 index ((i / 64) * 4 + q) * 64 + i % 64): a wave writes each quad as
+// 1 KiB of contiguous memory, and the P2G kernel reads a particle with four
+// wide loads.
+// with q_p the particle's mass quanta (an integer < 2^23, exact in FP32) and
+// A = m C + dt K row-major, K = -4/dx^2 V0 tau.
 
 #include <hip/hip_ext.h>
 
@@ -28,20 +34,38 @@ __device__ __forceinline__ void stile_coords(int k, int& ix, int& iy, int& iz) {
     iz = ((k & 7) - 4 * (iy & 1)) & 7;
 }
 
-__device__ __forceinline__ void store_coef(float* coef, int64_t cap, int64_t i, const v3<float>& mvel,
-                                           const m3<float>& Aff) {
-    coef[0 * cap + i] = mvel.x;
-    coef[1 * cap + i] = mvel.y;
-    coef[2 * cap + i] = mvel.z;
-#pragma unroll
-    for (int c = 0; c < 9; ++c) coef[(3 + c) * cap + i] = Aff.a[c];
+__device__ __forceinline__ int64_t rec_at(int64_t i, int q) { return ((i >> 6) * 4 + q) * 64 + (i & 63); }
+__device__ __forceinline__ void store_rec(float4* rec, int64_t i, float x, float y, float z, float qp,
+                                          const v3<float>& mvel, const m3<float>& A) {
+    rec[rec_at(i, 0)] = make_float4(x, y, z, qp);
+    rec[rec_at(i, 1)] = make_float4(mvel.x, mvel.y, mvel.z, A.a[0]);
+    rec[rec_at(i, 2)] = make_float4(A.a[1], A.a[2], A.a[3], A.a[4]);
+    rec[rec_at(i, 3)] = make_float4(A.a[5], A.a[6], A.a[7], A.a[8]);
 }
+struct DepositRec {
+    float x, y, z;
+    double qp;
+    v3<float> mvel;
+    m3<float> A;
+};
+__device__ __forceinline__ DepositRec load_rec(const float4* __restrict__ rec, int64_t i) {
+    const float4 r0 = rec[rec_at(i, 0)], r1 = rec[rec_at(i, 1)], r2 = rec[rec_at(i, 2)], r3 = rec[rec_at(i, 3)];
+    DepositRec d;
+    d.x = r0.x;
+    d.y = r0.y;
+    d.z = r0.z;
+    d.qp = (double)r0.w;
+    d.mvel = mk(r1.x, r1.y, r1.z);
+    d.A = m3<float>{{r1.w, r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w}};
+    return d;
+}
+__device__ __forceinline__ float quanta_of(float m, double qscale_inv) { return (float)rint((double)m * qscale_inv); }
 
 __global__ void k_set_zero(int* p) { *p = 0; }
 
 // G2P + plasticity + write-back + deposit coefficients; one lane per particle.
 template <bool DO_G2P>
-__global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float* coef,
+__global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float4* rec,
                                                  int* err) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // the clear before this pass left the output grid's block list to be
@@ -61,7 +85,7 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
         const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
         if (!stencil_ok(d, lx, ly, lz)) {
             atomicOr(err, 1);
-            store_coef(coef, P.cap, i, mk(0.f, 0.f, 0.f), m3<float>{{0, 0, 0, 0, 0, 0, 0, 0, 0}});
+            store_rec(rec, i, x, y, z, 0.0f, mk(0.f, 0.f, 0.f), m3<float>{{0, 0, 0, 0, 0, 0, 0, 0, 0}});
             return;
         }
         const NodeAxes na(d, lx, ly, lz);
@@ -113,13 +137,14 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
     }
     v3<float> mvel;
     m3<float> Aff;
-    p2g_coeffs(P.p[PM * P.cap + i], P.p[PV0 * P.cap + i], vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
-    store_coef(coef, P.cap, i, mvel, Aff);
+    const float m = P.p[PM * P.cap + i];
+    p2g_coeffs(m, P.p[PV0 * P.cap + i], vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
+    store_rec(rec, i, x, y, z, quanta_of(m, fa.qscale_inv), mvel, Aff);
 }
 
 // The FP64 pass for the particles k_advance listed: same outputs.
 template <bool DO_G2P>
-__global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float* coef,
+__global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float4* rec,
                                                      unsigned long long* counters) {
     const int cnt = *fa.n_redo;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
@@ -177,8 +202,9 @@ __global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domai
         }
         v3<float> mvel;
         m3<float> Aff;
-        p2g_coeffs(P.p[PM * P.cap + i], P.p[PV0 * P.cap + i], vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
-        store_coef(coef, P.cap, i, mvel, Aff);
+        const float m = P.p[PM * P.cap + i];
+        p2g_coeffs(m, P.p[PV0 * P.cap + i], vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
+        store_rec(rec, i, x, y, z, quanta_of(m, fa.qscale_inv), mvel, Aff);
         atomicAdd(&counters[0], 1ull);
     }
 }
@@ -203,7 +229,104 @@ struct CellSmem {
     int tb[27];
 };
 
-__global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restrict__ coef,
+// Deposits of slabs [A0, A1) of the stencils of one cell's particles,
+// accumulated in registers (node n = ((a - A0) * 3 + b) * 3 + c).  The
+// centre (1, 1, 1) takes the particle's remaining quanta, so when slab 1 is in
+// range the quanta of the other slabs are formed too.  Same arithmetic as
+// scatter() for the nodes it covers.
+template <int A0, int A1>
+__device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, const int4& U, const unsigned short* ord,
+                                            int s0, int mine, float dx, float inv_dx, v3<float>* acc,
+                                            unsigned* accm) {
+#pragma unroll
+    for (int k = 0; k < (A1 - A0) * 9; ++k) {
+        acc[k] = mk(0.f, 0.f, 0.f);
+        accm[k] = 0u;
+    }
+    constexpr bool CENTRE = A0 <= 1 && 1 < A1;
+    for (int k = 0; k < mine; ++k) {
+        const DepositRec r = load_rec(rec, (int64_t)U.y + ord[s0 + k]);
+        Stencil s;
+        make_stencil(r.x, r.y, r.z, inv_dx, s);
+        const v3<float>& mvel = r.mvel;
+        const m3<float>& Aff = r.A;
+        const double qp = r.qp;
+        int rest = 0;
+        if (CENTRE) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (a >= A0 && a < A1) continue;
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) rest += mass_quanta(s.w[0][a] * s.w[1][b] * s.w[2][c], qp);
+            }
+        }
+        v3<float> mom_c = mk(0.f, 0.f, 0.f);
+#pragma unroll
+        for (int a = A0; a < A1; ++a) {
+            const float ddx = ((float)(s.b[0] + a) - s.f[0] - 0.5f) * dx;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const float ddy = ((float)(s.b[1] + b) - s.f[1] - 0.5f) * dx;
+                const float px = mvel.x + Aff.a[0] * ddx + Aff.a[1] * ddy;
+                const float py = mvel.y + Aff.a[3] * ddx + Aff.a[4] * ddy;
+                const float pz = mvel.z + Aff.a[6] * ddx + Aff.a[7] * ddy;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float ddz = ((float)(s.b[2] + c) - s.f[2] - 0.5f) * dx;
+                    const float w = s.w[0][a] * s.w[1][b] * s.w[2][c];
+                    const v3<float> mom =
+                        mk(w * (px + Aff.a[2] * ddz), w * (py + Aff.a[5] * ddz), w * (pz + Aff.a[8] * ddz));
+                    const int n = ((a - A0) * 3 + b) * 3 + c;
+                    if (a == 1 && b == 1 && c == 1) {
+                        mom_c = mom;
+                        continue;
+                    }
+                    const int dq = mass_quanta(w, qp);
+                    rest += dq;
+                    acc[n] = acc[n] + mom;
+                    accm[n] += (unsigned)dq;
+                }
+            }
+        }
+        if (CENTRE) {
+            const int n = ((1 - A0) * 3 + 1) * 3 + 1;
+            acc[n] = acc[n] + mom_c;
+            accm[n] += (unsigned)((int)qp - rest);
+        }
+    }
+}
+
+// Add the register sums of slabs [A0, A1) into the LDS tile: for a fixed
+// stencil offset the 64 lanes (cells) address 64 different nodes, and program
+// order across offsets keeps neighbouring cells' shared nodes consistent.
+template <int A0, int A1>
+__device__ __forceinline__ void chain_slabs(float4* pt, int lane, const v3<float>* acc, const unsigned* accm) {
+    const int rx = (lane >> 4) + 1, ry = ((lane >> 2) & 3) + 1, rz = (lane & 3) + 1;
+#pragma unroll
+    for (int a = A0; a < A1; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int n = ((a - A0) * 3 + b) * 3 + c;
+                const int k = stix(rx + a, ry + b, rz + c);
+                float4 q = pt[k];
+                q.x += acc[n].x;
+                q.y += acc[n].y;
+                q.z += acc[n].z;
+                q.w = __uint_as_float(__float_as_uint(q.w) + accm[n]);
+                pt[k] = q;
+                __asm__ volatile("" ::: "memory");
+            }
+}
+
+// NPASS = 1: all 27 nodes in one pass over the cell's particles.  NPASS = 2:
+// slabs 0-1, then slab 2 in a second pass (fewer live registers, more waves
+// per SIMD, the particle data read twice).
+template <int NPASS>
+__global__ void __launch_bounds__(64) k_p2g_cell(const float4* __restrict__ rec,
                                                  const int4* __restrict__ units, GridBuf gout, uint8_t* bever,
                                                  Domain d, FastArgs fa, int* err, unsigned long long* counters) {
     __shared__ CellSmem S;
@@ -225,9 +348,9 @@ __global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restric
 
     // ---- count the particles per cell of the block
     for (int pi = lane; pi < U.z; pi += 64) {
-        const int64_t i = (int64_t)U.y + pi;
+        const float4 r0 = rec[rec_at((int64_t)U.y + pi, 0)];
         Stencil s;
-        make_stencil(P.p[(PX + 0) * P.cap + i], P.p[(PX + 1) * P.cap + i], P.p[(PX + 2) * P.cap + i], inv_dx, s);
+        make_stencil(r0.x, r0.y, r0.z, inv_dx, s);
         const int cx = s.b[0] - d.ox - tx0 - 1, cy = s.b[1] - d.oy - ty0 - 1, cz = s.b[2] - d.oz - tz0 - 1;
         if (cx >= 0 && cy >= 0 && cz >= 0 && cx < 4 && cy < 4 && cz < 4) {
             const int c = (cx * 4 + cy) * 4 + cz;
@@ -256,47 +379,19 @@ __global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restric
     __syncthreads();
 
     // ---- per-cell accumulation in registers
-    v3<float> acc[27];
-    unsigned accm[27];
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        acc[k] = mk(0.f, 0.f, 0.f);
-        accm[k] = 0u;
-    }
     const int s0 = S.start[lane];
-    for (int k = 0; k < mine; ++k) {
-        const int64_t i = (int64_t)U.y + S.ord[s0 + k];
-        Stencil s;
-        make_stencil(P.p[(PX + 0) * P.cap + i], P.p[(PX + 1) * P.cap + i], P.p[(PX + 2) * P.cap + i], inv_dx, s);
-        const v3<float> mvel = mk(coef[0 * P.cap + i], coef[1 * P.cap + i], coef[2 * P.cap + i]);
-        m3<float> Aff;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) Aff.a[c] = coef[(3 + c) * P.cap + i];
-        const double qp = rint((double)P.p[PM * P.cap + i] * fa.qscale_inv);
-        scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
-            const int n = (a * 3 + b) * 3 + c;
-            acc[n] = acc[n] + mom;
-            accm[n] += (unsigned)dq;
-        });
-    }
-    if (mine > 0) {
-        const int rx = (lane >> 4) + 1, ry = ((lane >> 2) & 3) + 1, rz = (lane & 3) + 1;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const int n = (a * 3 + b) * 3 + c;
-                    const int k = stix(rx + a, ry + b, rz + c);
-                    float4 q = S.pt[k];
-                    q.x += acc[n].x;
-                    q.y += acc[n].y;
-                    q.z += acc[n].z;
-                    q.w = __uint_as_float(__float_as_uint(q.w) + accm[n]);
-                    S.pt[k] = q;
-                    __asm__ volatile("" ::: "memory");
-                }
+    if (NPASS == 1) {
+        v3<float> acc[27];
+        unsigned accm[27];
+        accum_slabs<0, 3>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm);
+        if (mine > 0) chain_slabs<0, 3>(S.pt, lane, acc, accm);
+    } else {
+        v3<float> acc[18];
+        unsigned accm[18];
+        accum_slabs<0, 2>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm);
+        if (mine > 0) chain_slabs<0, 2>(S.pt, lane, acc, accm);
+        accum_slabs<2, 3>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm);
+        if (mine > 0) chain_slabs<2, 3>(S.pt, lane, acc, accm);
     }
     __syncthreads();
 
@@ -305,14 +400,12 @@ __global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restric
     int off = 0;
     const int nl = S.nleft;
     for (int k = lane; k < nl; k += 64) {
-        const int64_t i = (int64_t)U.y + S.left[k];
+        const DepositRec r = load_rec(rec, (int64_t)U.y + S.left[k]);
         Stencil s;
-        make_stencil(P.p[(PX + 0) * P.cap + i], P.p[(PX + 1) * P.cap + i], P.p[(PX + 2) * P.cap + i], inv_dx, s);
-        const v3<float> mvel = mk(coef[0 * P.cap + i], coef[1 * P.cap + i], coef[2 * P.cap + i]);
-        m3<float> Aff;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) Aff.a[c] = coef[(3 + c) * P.cap + i];
-        const double qp = rint((double)P.p[PM * P.cap + i] * fa.qscale_inv);
+        make_stencil(r.x, r.y, r.z, inv_dx, s);
+        const v3<float>& mvel = r.mvel;
+        const m3<float>& Aff = r.A;
+        const double qp = r.qp;
         const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
         if (!stencil_ok(d, lx, ly, lz)) {
             atomicOr(err, 1);
@@ -366,11 +459,12 @@ __global__ void __launch_bounds__(64) k_p2g_cell(PView P, const float* __restric
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
-                  hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx) {
+                  hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx, bool two_pass) {
     if (P.n <= 0) {  // no particle pass to empty the output grid's block list
         hipLaunchKernelGGL(k_set_zero, dim3(1), dim3(1), 0, s, gout.bcount);
         return;
     }
+    float4* rec = (float4*)coef;  // 16 floats per particle (whole groups of 64: see rec_at)
     FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, gout.bcount};
     // n_redo was zeroed by the grid clear that precedes every fast pass
     const int g = (int)((P.n + 255) / 256);
@@ -378,20 +472,24 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
     if (mark) mark(ctx, SPLIT_ADVANCE, &e0, &e1);
     // with timing events the launch records them around the dispatch itself
     if (do_g2p)
-        hipExtLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, coef, err);
+        hipExtLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, rec, err);
     else
-        hipExtLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, coef, err);
+        hipExtLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, rec, err);
     if (mark) mark(ctx, SPLIT_REDO, nullptr, nullptr);
     if (do_g2p)
-        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
+        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters);
     else
-        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, coef, counters);
+        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters);
     e0 = e1 = nullptr;
     if (mark) mark(ctx, SPLIT_DEPOSIT, &e0, &e1);
     if (n_units > 0) {
         const int nwg = ((n_units + 7) / 8) * 8;
-        hipExtLaunchKernelGGL(k_p2g_cell, dim3(nwg), dim3(64), 0, s, e0, e1, 0, P, (const float*)coef, units, gout,
-                              bever, d, fa, err, counters);
+        if (two_pass)
+            hipExtLaunchKernelGGL(k_p2g_cell<2>, dim3(nwg), dim3(64), 0, s, e0, e1, 0, (const float4*)rec, units, gout,
+                                  bever, d, fa, err, counters);
+        else
+            hipExtLaunchKernelGGL(k_p2g_cell<1>, dim3(nwg), dim3(64), 0, s, e0, e1, 0, (const float4*)rec, units, gout,
+                                  bever, d, fa, err, counters);
     }
 }
 
