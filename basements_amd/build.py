@@ -38,6 +38,7 @@ UNITS = [
     ("mpm_fused.hip", []),
     ("mpm_split.hip", []),
     ("bm_mpm_api.hip", ["-ffp-contract=off"]),
+    ("bm_batch.hip", []),
 ]
 
 

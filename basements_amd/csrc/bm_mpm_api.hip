@@ -21,7 +21,7 @@ using namespace bm;
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_last_error;
 
-static int fail(int code, const char* fmt, ...) {
+int bm::fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -30,14 +30,6 @@ static int fail(int code, const char* fmt, ...) {
     g_last_error = buf;
     return code;
 }
-
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return fail(BM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                        __LINE__);                                                                 \
-    } while (0)
 
 enum Phase { PH_EMPTY = 0, PH_P2G = 1, PH_UPDATED = 2 };
 
@@ -249,6 +241,7 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
     d.nbz = d.nz / 4;
     d.dx = desc->dx;
     d.inv_dx = 1.0f / desc->dx;  // SPGridCPU(dx): inv_dx = 1/dx (spgrid_cpu.h:182)
+    d.b0 = 0;
     h->nblocks = (int64_t)d.nbx * d.nby * d.nbz;
     h->nnodes = h->nblocks * 64;
     if (h->nblocks > (int64_t)1 << 30) {

@@ -123,6 +123,7 @@ struct FastArgs {
     int* redo;      // particles whose SVD needs the FP64 pass
     int* n_redo;
     int* list_reset;  // block-list length of the output grid, zeroed by the particle pass (two-kernel form)
+    const int* n_units_dev;  // unit count on the device (batches: no host read-back), else n_units
 };
 
 __device__ __forceinline__ bool stencil_ok(const Domain& d, int lx, int ly, int lz) {
@@ -226,7 +227,7 @@ struct NodeAxes {
     __device__ __forceinline__ NodeAxes(const Domain& d, int lx, int ly, int lz) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            X[k] = (((lx + k) >> 2) * d.nby * d.nbz) * 64 + ((lx + k) & 3) * 16;
+            X[k] = (int)(d.b0 * 64) + (((lx + k) >> 2) * d.nby * d.nbz) * 64 + ((lx + k) & 3) * 16;
             Y[k] = (((ly + k) >> 2) * d.nbz) * 64 + ((ly + k) & 3) * 4;
             Z[k] = ((lz + k) >> 2) * 64 + ((lz + k) & 3);
         }

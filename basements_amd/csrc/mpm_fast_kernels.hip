@@ -25,9 +25,15 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // mostly share a block, so the lanes of a wave agree on one counter
 // increment per distinct block (the leader adds the group size) instead of
 // one contended atomic per particle.
-__global__ void __launch_bounds__(256) k_bin(PView P, Domain d, int* bcnt, int* key, int* slot) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < P.n;
+__global__ void __launch_bounds__(256) k_bin(PView P, Domain d, int* bcnt, int* key, int* slot, SceneView sv) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = i < P.n;
+    if (sv.chunks) {  // one scene per workgroup
+        const int4 ch = sv.chunks[blockIdx.x];
+        i = ch.y + threadIdx.x;
+        live = (int)threadIdx.x < ch.z;
+        d = sv.sc[ch.x].d;
+    }
     int b = -1;
     if (live) {
         const float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
@@ -38,7 +44,7 @@ __global__ void __launch_bounds__(256) k_bin(PView P, Domain d, int* bcnt, int* 
         // kernels detect and report them
         const int bx = clampi(lx >> 2, 0, d.nbx - 1), by = clampi(ly >> 2, 0, d.nby - 1),
                   bz = clampi(lz >> 2, 0, d.nbz - 1);
-        b = (bx * d.nby + by) * d.nbz + bz;
+        b = (int)(d.b0 + ((int64_t)bx * d.nby + by) * d.nbz + bz);
     }
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -76,13 +82,14 @@ __global__ void k_units_count(const int* bcnt, int nblocks, int unit_p, int* ucn
 }
 
 __global__ void k_units_fill(const int* bcnt, const int* bstart, const int* ucnt, const int* ustart, int nblocks,
-                             int unit_p, int4* units, int* n_units) {
+                             int unit_p, int4* units, int* n_units, const int* block_scene) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nblocks) return;
     const int c = bcnt[b], nu = ucnt[b], u0 = ustart[b], s0 = bstart[b];
+    const int scene = block_scene ? block_scene[b] : 0;
     for (int u = 0; u < nu; ++u) {
         const int cnt = min(unit_p, c - u * unit_p);
-        units[u0 + u] = make_int4(b, s0 + u * unit_p, cnt, 0);
+        units[u0 + u] = make_int4(b, s0 + u * unit_p, cnt, scene);
     }
     if (b == nblocks - 1) *n_units = u0 + nu;
 }
@@ -184,10 +191,14 @@ size_t scan_temp_bytes(int n) {
 // Re-bin the particles by grid block and rebuild the work units.  Returns the
 // unit count on the host (one stream synchronisation per re-binning).
 int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, const Domain& d, int nblocks,
-                 int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host) {
+                 int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host, SceneView sv, int n_chunks) {
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nblocks, 256)), dim3(256), 0, s, bs.bcnt, nblocks, 0);
-    if (P.n > 0)
-        hipLaunchKernelGGL(k_bin, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, d, bs.bcnt, bs.key, bs.slot);
+    if (sv.chunks) {
+        if (n_chunks > 0)
+            hipLaunchKernelGGL(k_bin, dim3(n_chunks), dim3(256), 0, s, P, d, bs.bcnt, bs.key, bs.slot, sv);
+    } else if (P.n > 0) {
+        hipLaunchKernelGGL(k_bin, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, d, bs.bcnt, bs.key, bs.slot, sv);
+    }
     size_t tb = bs.temp_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(bs.temp, tb, bs.bcnt, bs.bstart, nblocks, s);
     if (P.n > 0)
@@ -198,7 +209,8 @@ int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, 
     tb = bs.temp_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(bs.temp, tb, bs.ucnt, bs.ustart, nblocks, s);
     hipLaunchKernelGGL(k_units_fill, dim3(grid_for(nblocks, 256)), dim3(256), 0, s, bs.bcnt, bs.bstart, bs.ucnt,
-                       bs.ustart, nblocks, unit_p, bs.units, bs.n_units);
+                       bs.ustart, nblocks, unit_p, bs.units, bs.n_units, sv.block_scene);
+    if (!n_units_host) return 0;  // the deposit kernel reads the count on the device
     if (hipMemcpyAsync(n_units_host, bs.n_units, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
     return 0;

@@ -392,7 +392,7 @@ void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, int variant,
                   hipStream_t s) {
     if (n_units <= 0) return;
-    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, nullptr};
+    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, nullptr, nullptr};
     const int nwg = ((n_units + 7) / 8) * 8;
     // n_redo was zeroed by the grid clear that precedes every fast pass
     const bool rmw = variant & 1, gtile = variant & 2;

@@ -9,6 +9,16 @@
 
 namespace bm {
 
+// Record the last-error string of the C ABI and return code.
+int fail(int code, const char* fmt, ...);
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(BM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                 \
+    } while (0)
+
 // One node buffer of the dense block-linear grid.  Node (i,j,k) of the domain
 // lives in block ((i>>2)*NBY + (j>>2))*NBZ + (k>>2), slot ((i&3)*4+(j&3))*4+(k&3)
 // -- the reference's in-block order (spgrid_cpu.h:205-218).
@@ -26,6 +36,7 @@ struct Domain {
     int nx, ny, nz;      // nodes per axis (multiples of 4)
     int nbx, nby, nbz;   // blocks per axis
     float dx, inv_dx;
+    int64_t b0;          // first block of this box in the grid arrays (0 for a lone scene)
 };
 
 // Particle SoA view: component c of particle i at p[c * cap + i].
@@ -46,7 +57,14 @@ struct Phys {
 };
 
 __host__ __device__ inline int64_t block_of(const Domain& d, int lx, int ly, int lz) {
-    return ((int64_t)(lx >> 2) * d.nby + (ly >> 2)) * d.nbz + (lz >> 2);
+    return d.b0 + ((int64_t)(lx >> 2) * d.nby + (ly >> 2)) * d.nbz + (lz >> 2);
+}
+// Block coordinates of a block of the grid arrays inside its box.
+__host__ __device__ inline void block_coords(const Domain& d, int64_t blk, int& bx, int& by, int& bz) {
+    const int64_t l = blk - d.b0;
+    bz = (int)(l % d.nbz);
+    by = (int)((l / d.nbz) % d.nby);
+    bx = (int)(l / ((int64_t)d.nbz * d.nby));
 }
 __host__ __device__ inline int64_t node_of(const Domain& d, int lx, int ly, int lz) {
     return block_of(d, lx, ly, lz) * 64 + ((lx & 3) * 4 + (ly & 3)) * 4 + (lz & 3);
@@ -135,6 +153,22 @@ inline int64_t couple_blocks_bound(const float he[3], const Domain& d) {
     return tot;
 }
 
+// One scene of a batch: what the kernels need that differs per scene.
+struct SceneDev {
+    Domain d;  // the scene's node box; d.b0 = its first block in the shared grid arrays
+    Phys ph;
+    double qscale, qscale_inv;  // mass quantum and its inverse
+    int64_t p0, n;              // the scene's particle range
+};
+// Scene lookup of a launch over a batch.  All null for a lone scene: then
+// the Domain / Phys passed by value apply.
+struct SceneView {
+    const SceneDev* sc;      // scene table
+    const int* block_scene;  // scene of each block of the grid arrays
+    const int4* chunks;      // (scene, first particle, count, 0): one per workgroup of the particle kernels
+    int n_scenes;
+};
+
 // ------------------------------------------------------------- launchers
 // Zero the blocks on g's list and empty the list; also zeroes *zero_word
 // (a counter of the next kernels) when given.  defer_reset: the list length
@@ -144,9 +178,11 @@ void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Doma
                     double qscale_inv, int* err, hipStream_t s);
 // Grid update; with a device body also the start of its coupled substep
 // (applied loads, collider), as k_body_begin.
+// Over a batch (sv.sc set) each block takes its scene's box and physics, and
+// body[s] / col[s] are the bodies of the scenes with has_body[s] != 0.
 void launch_update(const GridBuf& g, int nblocks, const Domain& d, const Phys& ph, double qscale, float dt,
                    hipStream_t s, bm_body* body = nullptr, ColliderDev* col = nullptr, int64_t max_blocks = 0,
-                   int* err = nullptr);
+                   int* err = nullptr, SceneView sv = SceneView{}, const int* has_body = nullptr);
 void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phys& ph, float dt, int* err,
                     hipStream_t s);
 // The couplers run a fixed number of workgroups over the scanned blocks, so
@@ -163,6 +199,12 @@ void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* 
 void launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridBuf& g, uint8_t* bever,
                         const Domain& d, float dt, int64_t max_blocks, double* partials, double* out7, int* err,
                         bool begun, hipStream_t s);
+// The device bodies of a batch (scene s: bodies[s], cols[s] built by the
+// update, methods[s]); partials hold COUPLE_WGS * 7 doubles and out7 8
+// doubles per scene.
+void launch_couple_batch(const int* methods, const int* has_body, bm_body* bodies, const ColliderDev* cols,
+                         const GridBuf& g, uint8_t* bever, SceneView sv, float dt, double* partials, double* out7,
+                         hipStream_t s);
 void launch_grid_gather(const GridBuf& g, uint8_t* bever, const Domain& d, const int* org, const int* dims,
                         double qscale, int updated, long long* o_mq, float* o_mass, float* o_vel, float* o_force,
                         uint8_t* o_act, uint8_t* o_alloc, hipStream_t s);
@@ -188,8 +230,11 @@ struct BinScratch {
     size_t temp_bytes;
 };
 size_t scan_temp_bytes(int n);
+// Over a batch (sv.chunks set) k_bin runs one workgroup per chunk in its
+// scene's box.  n_units_host = nullptr skips the read-back (no host sync).
 int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, const Domain& d, int nblocks,
-                 int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host);
+                 int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host, SceneView sv = SceneView{},
+                 int n_chunks = 0);
 void launch_iota(int* p, int64_t n, int64_t first, hipStream_t s);
 void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
@@ -204,10 +249,18 @@ constexpr int FUSED_UNIT_MAX = 512;
 // stage's launch (the timing hook of the handle); for the two main kernels it
 // may hand back a pair of events that the dispatch itself records.
 enum { SPLIT_ADVANCE = 0, SPLIT_REDO = 1, SPLIT_DEPOSIT = 2 };
+// A batch: the particle kernels run one workgroup per chunk, and the deposit
+// kernel reads the unit count on the device with n_units_bound workgroups.
+struct SplitBatch {
+    SceneView sv{};
+    int n_chunks = 0;
+    const int* n_units_dev = nullptr;
+    int n_units_bound = 0;
+};
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*) = nullptr,
-                  void* ctx = nullptr, bool two_pass = false);
+                  void* ctx = nullptr, bool two_pass = false, const SplitBatch& batch = SplitBatch{});
 
 }  // namespace bm

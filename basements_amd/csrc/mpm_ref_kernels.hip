@@ -146,12 +146,29 @@ __device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, lon
 // the force into the momentum.  With a device body, one lane also starts the
 // body's coupled substep (body_begin), which touches no node.
 
-__global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d, Phys ph, double qscale, float dt, bm_body* body,
-                                               ColliderDev* col, long long max_blocks, int* err) {
-    if (body && blockIdx.x == 0 && threadIdx.x == 0) body_begin(body, col, d, max_blocks, err);
+__global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d0, Phys ph0, double qscale0, float dt, bm_body* body,
+                                               ColliderDev* col, long long max_blocks, int* err, SceneView sv,
+                                               const int* has_body) {
+    if (body && threadIdx.x == 0) {
+        if (!sv.sc) {
+            if (blockIdx.x == 0) body_begin(body, col, d0, max_blocks, err);
+        } else {  // one body per scene that has one (the launch has at least n_scenes workgroups)
+            const int s = blockIdx.x;
+            if (s < sv.n_scenes && has_body[s]) body_begin(body + s, col + s, sv.sc[s].d, max_blocks, err);
+        }
+    }
     const int cnt = *g.bcount;
     for (int b = blockIdx.x; b < cnt; b += gridDim.x) {
         const int blk = g.blist[b];
+        Domain d = d0;
+        Phys ph = ph0;
+        double qscale = qscale0;
+        if (sv.block_scene) {
+            const SceneDev& S = sv.sc[sv.block_scene[blk]];
+            d = S.d;
+            ph = S.ph;
+            qscale = S.qscale;
+        }
         const int64_t nid = (int64_t)blk * 64 + threadIdx.x;
         const long long mq = g.mq[nid];
         float4 q = g.mv[nid];
@@ -166,9 +183,8 @@ __global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d, Phys ph, dou
             }
             v3<float> v = divg(mom, m);
             v = v + mk(ph.gx * dt, ph.gy * dt, ph.gz * dt);
-            const int bz = blk % d.nbz;
-            const int by = (blk / d.nbz) % d.nby;
-            const int bx = blk / (d.nbz * d.nby);
+            int bx, by, bz;
+            block_coords(d, blk, bx, by, bz);
             const int t = threadIdx.x;
             const int gj = d.oy + by * 4 + ((t >> 2) & 3);
             const float node_y = ((float)gj + 0.5f) * d.dx;
@@ -307,7 +323,7 @@ __device__ __forceinline__ void couple_wg(int method, const ColliderDev& b, cons
         const int rby = (int)((rb / b.nb[2]) % b.nb[1]);
         const int rbx = (int)(rb / ((int64_t)b.nb[2] * b.nb[1]));
         const int bx = b.b0[0] + rbx, by = b.b0[1] + rby, bz = b.b0[2] + rbz;
-        const int blk = (bx * d.nby + by) * d.nbz + bz;
+        const int blk = (int)(d.b0 + ((int64_t)bx * d.nby + by) * d.nbz + bz);
         // global node index of the block origin
         const int ox = d.ox + bx * 4, oy = d.oy + by * 4, oz = d.oz + bz * 4;
         // block cull, mpm_rigid_coupler.cpp:82-91
@@ -527,10 +543,12 @@ void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Doma
     hipLaunchKernelGGL(k_p2g_ref, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, g, bever, d, ph, qscale_inv, err);
 }
 void launch_update(const GridBuf& g, int nblocks, const Domain& d, const Phys& ph, double qscale, float dt,
-                   hipStream_t s, bm_body* body, ColliderDev* col, int64_t max_blocks, int* err) {
-    const int grid = nblocks < 16384 ? (nblocks > 0 ? nblocks : 1) : 16384;
+                   hipStream_t s, bm_body* body, ColliderDev* col, int64_t max_blocks, int* err, SceneView sv,
+                   const int* has_body) {
+    int grid = nblocks < 16384 ? (nblocks > 0 ? nblocks : 1) : 16384;
+    if (grid < sv.n_scenes) grid = sv.n_scenes;
     hipLaunchKernelGGL(k_update, dim3(grid), dim3(64), 0, s, g, d, ph, qscale, dt, body, col, (long long)max_blocks,
-                       err);
+                       err, sv, has_body);
 }
 void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phys& ph, float dt, int* err,
                     hipStream_t s) {
@@ -569,19 +587,27 @@ __global__ void k_body_begin(bm_body* body, ColliderDev* col, Domain d, long lon
 __device__ void integrate_body_d(bm_body& b, float dt) {
     if (!b.integrate) return;
     float v[3], w[3];
-    for (int a = 0; a < 3; ++a) v[a] = b.linear_velocity[a] + (b.force[a] * b.inv_mass) * dt;
-    const float* I = b.inv_inertia;
-    const float it[3] = {I[0] * b.torque[0] + I[1] * b.torque[1] + I[2] * b.torque[2],
-                         I[3] * b.torque[0] + I[4] * b.torque[1] + I[5] * b.torque[2],
-                         I[6] * b.torque[0] + I[7] * b.torque[1] + I[8] * b.torque[2]};
-    for (int a = 0; a < 3; ++a) w[a] = b.angular_velocity[a] + it[a] * dt;
-    const float ld = 1.0f - b.linear_damping * dt, ad = 1.0f - b.angular_damping * dt;
-    for (int a = 0; a < 3; ++a) {
-        v[a] = v[a] * ld;
-        w[a] = w[a] * ad;
-        b.position[a] = b.position[a] + v[a] * dt;
-        b.linear_velocity[a] = v[a];
-        b.angular_velocity[a] = w[a];
+    if (b.integrate == BM_BODY_KINEMATIC) {  // moves at its own velocities; the reaction is reported only
+        for (int a = 0; a < 3; ++a) {
+            v[a] = b.linear_velocity[a];
+            w[a] = b.angular_velocity[a];
+            b.position[a] = b.position[a] + v[a] * dt;
+        }
+    } else {
+        for (int a = 0; a < 3; ++a) v[a] = b.linear_velocity[a] + (b.force[a] * b.inv_mass) * dt;
+        const float* I = b.inv_inertia;
+        const float it[3] = {I[0] * b.torque[0] + I[1] * b.torque[1] + I[2] * b.torque[2],
+                             I[3] * b.torque[0] + I[4] * b.torque[1] + I[5] * b.torque[2],
+                             I[6] * b.torque[0] + I[7] * b.torque[1] + I[8] * b.torque[2]};
+        for (int a = 0; a < 3; ++a) w[a] = b.angular_velocity[a] + it[a] * dt;
+        const float ld = 1.0f - b.linear_damping * dt, ad = 1.0f - b.angular_damping * dt;
+        for (int a = 0; a < 3; ++a) {
+            v[a] = v[a] * ld;
+            w[a] = w[a] * ad;
+            b.position[a] = b.position[a] + v[a] * dt;
+            b.linear_velocity[a] = v[a];
+            b.angular_velocity[a] = w[a];
+        }
     }
     // Quaternion product (0, w) * q, quaternion.h
     const float ow = b.orientation[0], ox = b.orientation[1], oy = b.orientation[2], oz = b.orientation[3];
@@ -652,6 +678,32 @@ __global__ void __launch_bounds__(256) k_couple_body(int method, const ColliderD
                                                      Domain d, float dt, double* partials) {
     const ColliderDev b = *bp;
     couple_wg(method, b, g, bever, d, dt, partials);
+}
+
+// The couplers of a batch: blockIdx.y is the scene; scenes without a body
+// return at once.  Partials and reaction per scene.
+__global__ void __launch_bounds__(256) k_couple_batch(const int* methods, const int* has_body, const ColliderDev* cols,
+                                                      GridBuf g, uint8_t* bever, SceneView sv, float dt,
+                                                      double* partials) {
+    const int s = blockIdx.y;
+    if (!has_body[s]) return;
+    const ColliderDev b = cols[s];
+    couple_wg(methods[s], b, g, bever, sv.sc[s].d, dt, partials + (int64_t)s * COUPLE_WGS * 7);
+}
+__global__ void __launch_bounds__(64) k_body_end_batch(bm_body* bodies, const int* has_body, const double* partials,
+                                                       double* out7, float dt) {
+    const int s = blockIdx.x;
+    if (!has_body[s]) return;
+    body_end(bodies + s, partials + (int64_t)s * COUPLE_WGS * 7, COUPLE_WGS, out7 + (int64_t)s * 8, dt);
+}
+void launch_couple_batch(const int* methods, const int* has_body, bm_body* bodies, const ColliderDev* cols,
+                         const GridBuf& g, uint8_t* bever, SceneView sv, float dt, double* partials, double* out7,
+                         hipStream_t s) {
+    if (sv.n_scenes <= 0 || !(dt > 0.0f)) return;
+    hipLaunchKernelGGL(k_couple_batch, dim3(COUPLE_WGS, sv.n_scenes), dim3(256), 0, s, methods, has_body, cols, g,
+                       bever, sv, dt, partials);
+    hipLaunchKernelGGL(k_body_end_batch, dim3(sv.n_scenes), dim3(64), 0, s, bodies, has_body,
+                       (const double*)partials, out7, dt);
 }
 
 void launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridBuf& g, uint8_t* bever,

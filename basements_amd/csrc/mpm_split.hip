@@ -66,12 +66,22 @@ __global__ void k_set_zero(int* p) { *p = 0; }
 // G2P + plasticity + write-back + deposit coefficients; one lane per particle.
 template <bool DO_G2P>
 __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float4* rec,
-                                                 int* err) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                 int* err, SceneView sv) {
     // the clear before this pass left the output grid's block list to be
     // emptied here, before the deposit kernel appends to it
-    if (i == 0 && fa.list_reset) *fa.list_reset = 0;
-    if (i >= P.n) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && fa.list_reset) *fa.list_reset = 0;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sv.chunks) {  // a batch: one scene per workgroup
+        const int4 ch = sv.chunks[blockIdx.x];
+        if ((int)threadIdx.x >= ch.z) return;
+        i = ch.y + threadIdx.x;
+        const SceneDev& S = sv.sc[ch.x];
+        d = S.d;
+        ph = S.ph;
+        fa.qscale_inv = S.qscale_inv;
+    } else if (i >= P.n) {
+        return;
+    }
     const float dx = d.dx, inv_dx = d.inv_dx;
     float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
     m3<float> F;
@@ -144,11 +154,24 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
 
 // The FP64 pass for the particles k_advance listed: same outputs.
 template <bool DO_G2P>
-__global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float4* rec,
-                                                     unsigned long long* counters) {
+__global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domain d0, Phys ph0, FastArgs fa, float4* rec,
+                                                     unsigned long long* counters, SceneView sv) {
     const int cnt = *fa.n_redo;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
         const int64_t i = fa.redo[r];
+        Domain d = d0;
+        Phys ph = ph0;
+        if (sv.sc) {  // the scene whose particle range holds i
+            int lo = 0, hi = sv.n_scenes - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sv.sc[mid].p0 <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            d = sv.sc[lo].d;
+            ph = sv.sc[lo].ph;
+            fa.qscale_inv = sv.sc[lo].qscale_inv;
+        }
         const float dx = d.dx, inv_dx = d.inv_dx;
         float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
         m3<float> F;
@@ -328,14 +351,17 @@ __device__ __forceinline__ void chain_slabs(float4* pt, int lane, const v3<float
 template <int NPASS>
 __global__ void __launch_bounds__(64) k_p2g_cell(const float4* __restrict__ rec,
                                                  const int4* __restrict__ units, GridBuf gout, uint8_t* bever,
-                                                 Domain d, FastArgs fa, int* err, unsigned long long* counters) {
+                                                 Domain d, FastArgs fa, int* err, unsigned long long* counters,
+                                                 SceneView sv) {
     __shared__ CellSmem S;
     const int per = (gridDim.x + 7) / 8;
     const int u = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (u >= fa.n_units) return;
+    if (u >= (fa.n_units_dev ? *fa.n_units_dev : fa.n_units)) return;
     const int4 U = units[u];
+    if (sv.sc) d = sv.sc[U.w].d;  // a batch: the unit's scene
     const int blk = U.x;
-    const int bz = blk % d.nbz, by = (blk / d.nbz) % d.nby, bx = blk / (d.nbz * d.nby);
+    int bx, by, bz;
+    block_coords(d, blk, bx, by, bz);
     const int tx0 = bx * 4 - 1, ty0 = by * 4 - 1, tz0 = bz * 4 - 1;
     const int lane = threadIdx.x;
     const float dx = d.dx, inv_dx = d.inv_dx;
@@ -449,7 +475,7 @@ __global__ void __launch_bounds__(64) k_p2g_cell(const float4* __restrict__ rec,
     __syncthreads();
     if (lane < 27 && S.tb[lane]) {
         const int qx = lane / 9, qy = (lane / 3) % 3, qz = lane % 3;
-        touch_block_f(gout, bever, ((bx + qx - 1) * d.nby + (by + qy - 1)) * d.nbz + (bz + qz - 1));
+        touch_block_f(gout, bever, (int)(d.b0 + ((int64_t)(bx + qx - 1) * d.nby + (by + qy - 1)) * d.nbz + (bz + qz - 1)));
     }
 }
 
@@ -459,37 +485,50 @@ __global__ void __launch_bounds__(64) k_p2g_cell(const float4* __restrict__ rec,
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
-                  hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx, bool two_pass) {
-    if (P.n <= 0) {  // no particle pass to empty the output grid's block list
+                  hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx, bool two_pass,
+                  const SplitBatch& batch) {
+    const bool batched = batch.sv.chunks != nullptr;
+    if (batched ? batch.n_chunks <= 0 : P.n <= 0) {  // no particle pass to empty the output grid's block list
         hipLaunchKernelGGL(k_set_zero, dim3(1), dim3(1), 0, s, gout.bcount);
         return;
     }
     float4* rec = (float4*)coef;  // 16 floats per particle (whole groups of 64: see rec_at)
-    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, gout.bcount};
+    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, gout.bcount, batch.n_units_dev};
     // n_redo was zeroed by the grid clear that precedes every fast pass
-    const int g = (int)((P.n + 255) / 256);
+    const int g = batched ? batch.n_chunks : (int)((P.n + 255) / 256);
+    const SceneView sv = batch.sv;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (mark) mark(ctx, SPLIT_ADVANCE, &e0, &e1);
     // with timing events the launch records them around the dispatch itself
-    if (do_g2p)
-        hipExtLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, rec, err);
-    else
-        hipExtLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, rec, err);
+    if (e0) {
+        if (do_g2p)
+            hipExtLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, rec, err,
+                                  sv);
+        else
+            hipExtLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, rec, err,
+                                  sv);
+    } else if (do_g2p) {
+        hipLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, P, gin, d, ph, fa, rec, err, sv);
+    } else {
+        hipLaunchKernelGGL(k_advance<false>, dim3(g), dim3(256), 0, s, P, gin, d, ph, fa, rec, err, sv);
+    }
     if (mark) mark(ctx, SPLIT_REDO, nullptr, nullptr);
     if (do_g2p)
-        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters);
+        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters, sv);
     else
-        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters);
+        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters, sv);
     e0 = e1 = nullptr;
     if (mark) mark(ctx, SPLIT_DEPOSIT, &e0, &e1);
-    if (n_units > 0) {
-        const int nwg = ((n_units + 7) / 8) * 8;
-        if (two_pass)
-            hipExtLaunchKernelGGL(k_p2g_cell<2>, dim3(nwg), dim3(64), 0, s, e0, e1, 0, (const float4*)rec, units, gout,
-                                  bever, d, fa, err, counters);
+    const int nu = batch.n_units_dev ? batch.n_units_bound : n_units;
+    if (nu > 0) {
+        const int nwg = ((nu + 7) / 8) * 8;
+        auto kern = two_pass ? k_p2g_cell<2> : k_p2g_cell<1>;
+        if (e0)
+            hipExtLaunchKernelGGL(kern, dim3(nwg), dim3(64), 0, s, e0, e1, 0, (const float4*)rec, units, gout, bever,
+                                  d, fa, err, counters, sv);
         else
-            hipExtLaunchKernelGGL(k_p2g_cell<1>, dim3(nwg), dim3(64), 0, s, e0, e1, 0, (const float4*)rec, units, gout,
-                                  bever, d, fa, err, counters);
+            hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), 0, s, (const float4*)rec, units, gout, bever, d, fa, err,
+                               counters, sv);
     }
 }
 

@@ -35,6 +35,7 @@ M0 = 0
 M1 = 1
 SHAPE_BOX = 0
 SHAPE_CYLINDER = 1
+BODY_STATIC, BODY_DYNAMIC, BODY_KINEMATIC = 0, 1, 2
 
 
 class BMError(RuntimeError):
@@ -121,7 +122,7 @@ class Body:
     inv_inertia: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
     linear_damping: float = 0.01
     angular_damping: float = 0.01
-    integrate: bool = True
+    integrate: int = 1  # BODY_STATIC, BODY_DYNAMIC or BODY_KINEMATIC
     applied_force: tuple = (0.0, 0.0, 0.0)
     applied_torque: tuple = (0.0, 0.0, 0.0)
     half_extents: tuple = (0.5, 0.5, 0.5)
@@ -147,13 +148,25 @@ class Body:
         for f, _ in BodyC._fields_:
             v = getattr(b, f)
             kw[f] = tuple(v[:]) if hasattr(v, "__len__") else v
-        kw["integrate"] = bool(kw["integrate"])
+        kw["integrate"] = int(kw["integrate"])
         return Body(**kw)
 
     def collider(self) -> Collider:
         return Collider(center=self.position, quat=self.orientation, half_extents=self.half_extents,
                         lin_vel=self.linear_velocity, ang_vel=self.angular_velocity, friction=self.friction,
                         mass=self.mass, shape=self.shape)
+
+
+class BatchDescC(C.Structure):
+    _fields_ = [("max_particles", C.c_int64), ("max_blocks", C.c_int64), ("max_scenes", C.c_int32),
+                ("stream", C.c_void_p)]
+
+
+class SceneDescC(C.Structure):
+    _fields_ = [("dx", C.c_float), ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_int32 * 3),
+                ("young_modulus", C.c_float), ("poisson_ratio", C.c_float), ("bulk_density", C.c_float),
+                ("plastic_alpha", C.c_float), ("floor_friction", C.c_float), ("gravity", C.c_float * 3),
+                ("walls", C.c_int32), ("coupler", C.c_int32)]
 
 
 # ------------------------------------------------------------------ library
@@ -186,6 +199,19 @@ _SIGS = {
     "bm_mpm_get_body": (C.c_int, [C.c_void_p, C.POINTER(BodyC), C.POINTER(ReactionC)]),
     "bm_mpm_couple_body": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.POINTER(ReactionC)]),
     "bm_mpm_step_coupled": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_int32]),
+    "bm_batch_create": (C.c_int, [C.POINTER(BatchDescC), C.POINTER(C.c_void_p)]),
+    "bm_batch_destroy": (C.c_int, [C.c_void_p]),
+    "bm_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "bm_batch_add_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneDescC), C.c_int64, C.POINTER(HostParticles),
+                                     C.POINTER(BodyC), C.POINTER(C.c_int32)]),
+    "bm_batch_remove_scene": (C.c_int, [C.c_void_p, C.c_int32]),
+    "bm_batch_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int32]),
+    "bm_batch_download_scene": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(HostParticles)]),
+    "bm_batch_get_body": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BodyC), C.POINTER(ReactionC)]),
+    "bm_batch_usage": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "bm_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "bm_batch_autotune": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "bm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "bm_mpm_download_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p]),
     "bm_mpm_upload_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -535,3 +561,112 @@ class MPMSolver:
         if self._dirty:
             d.upload(P)
             self._dirty = False
+
+
+# ------------------------------------------------------------------ Batch
+@dataclass
+class SceneDesc:
+    """One scene of a batch (bm_scene_desc): its node box, material, floor,
+    gravity and the coupler of its body."""
+
+    dx: float
+    origin: tuple
+    dims: tuple
+    E: float = 1.0e5
+    nu: float = 0.3
+    rho: float = 1500.0
+    alpha: float = 0.40
+    mu_floor: float = 0.4
+    gravity: tuple = (0.0, -9.81, 0.0)
+    walls: int = 0
+    coupler: int = M0
+
+    def to_c(self) -> SceneDescC:
+        d = SceneDescC()
+        d.dx = float(self.dx)
+        d.grid_origin[:] = [int(v) for v in self.origin]
+        d.grid_dims[:] = [int(v) for v in self.dims]
+        d.young_modulus, d.poisson_ratio, d.bulk_density = float(self.E), float(self.nu), float(self.rho)
+        d.plastic_alpha, d.floor_friction = float(self.alpha), float(self.mu_floor)
+        d.gravity[:] = [float(v) for v in self.gravity]
+        d.walls, d.coupler = int(self.walls), int(self.coupler)
+        return d
+
+
+class Batch:
+    """Many scenes in shared device arrays, stepped together (bm_batch_*)."""
+
+    def __init__(self, max_particles, max_blocks, max_scenes, stream=None):
+        d = BatchDescC()
+        d.max_particles, d.max_blocks, d.max_scenes, d.stream = int(max_particles), int(max_blocks), \
+            int(max_scenes), stream
+        h = C.c_void_p()
+        _check(lib().bm_batch_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.n = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, desc: SceneDesc, P, body: Body | None = None) -> int:
+        """Add a scene (full particle state as Solver.upload); returns its id.
+        Raises BMError(BM_ERR_OOM) when the batch has no room."""
+        n = int(np.asarray(P["x"]).reshape(-1, 3).shape[0])
+        arrs = {k: _f32(P[k], -1) for k in ("x", "v", "F", "C", "m", "V0")}
+        hp = HostParticles(*[arrs[k].ctypes.data_as(C.POINTER(C.c_float)) for k in ("x", "v", "F", "C", "m", "V0")])
+        sd = desc.to_c()
+        sid = C.c_int32()
+        bc = body.to_c() if body is not None else None
+        _check(lib().bm_batch_add_scene(self.h, C.byref(sd), n, C.byref(hp), C.byref(bc) if bc is not None else None,
+                                        C.byref(sid)))
+        self.n[sid.value] = n
+        return sid.value
+
+    def remove_scene(self, sid):
+        _check(lib().bm_batch_remove_scene(self.h, int(sid)))
+        self.n.pop(int(sid), None)
+
+    def step(self, dt, k=1):
+        _check(lib().bm_batch_step(self.h, float(dt), int(k)))
+
+    def synchronize(self):
+        _check(lib().bm_batch_synchronize(self.h))
+
+    def download_scene(self, sid):
+        n = self.n[int(sid)]
+        out = {"x": np.zeros((n, 3), np.float32), "v": np.zeros((n, 3), np.float32),
+               "F": np.zeros((n, 3, 3), np.float32), "C": np.zeros((n, 3, 3), np.float32),
+               "m": np.zeros(n, np.float32), "V0": np.zeros(n, np.float32)}
+        hp = HostParticles(*[out[k].ctypes.data_as(C.POINTER(C.c_float)) for k in ("x", "v", "F", "C", "m", "V0")])
+        _check(lib().bm_batch_download_scene(self.h, int(sid), C.byref(hp)))
+        return out
+
+    def get_body(self, sid):
+        b, r = BodyC(), ReactionC()
+        _check(lib().bm_batch_get_body(self.h, int(sid), C.byref(b), C.byref(r)))
+        return Body.from_c(b), (np.array(r.force[:], np.float64), np.array(r.torque[:], np.float64),
+                                int(r.nodes_inside))
+
+    def usage(self):
+        s, p, b = C.c_int32(), C.c_int64(), C.c_int64()
+        _check(lib().bm_batch_usage(self.h, C.byref(s), C.byref(p), C.byref(b)))
+        return s.value, p.value, b.value
+
+    def set_tuning(self, unit_particles=0, sort_every=0):
+        _check(lib().bm_batch_set_tuning(self.h, int(unit_particles), int(sort_every)))
+
+    def autotune(self, dt, substeps=8):
+        u, s = C.c_int32(), C.c_int32()
+        _check(lib().bm_batch_autotune(self.h, float(dt), int(substeps), C.byref(u), C.byref(s)))
+        return u.value, s.value
+
+    def set_graph(self, on=True):
+        _check(lib().bm_batch_set_graph(self.h, 1 if on else 0))

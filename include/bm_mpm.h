@@ -174,6 +174,12 @@ int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, floa
  * substep needs no host round trip.  Body fields follow RigidBody
  * (rigid_body.h:23-60); the collider fields complete RigidColliderState
  * (mpm_rigid_coupler.h:72-84), whose pose and velocities come from the body. */
+/* BM_BODY_STATIC: STATIC or sleeping, integration skipped (integrator.h:25);
+ * BM_BODY_DYNAMIC: TimeIntegrator::integrate;
+ * BM_BODY_KINEMATIC: moves at its own linear / angular velocity whatever
+ *   the reaction (a dragged tool; no counterpart in the reference). */
+enum bm_body_motion { BM_BODY_STATIC = 0, BM_BODY_DYNAMIC = 1, BM_BODY_KINEMATIC = 2 };
+
 typedef struct {
     float position[3];
     float orientation[4];     /* quaternion (w, x, y, z) */
@@ -185,7 +191,7 @@ typedef struct {
     float inv_inertia[9];     /* inv_inertia_tensor, body space, row-major */
     float linear_damping;
     float angular_damping;
-    int32_t integrate;        /* 0: STATIC or sleeping, integration skipped (integrator.h:25) */
+    int32_t integrate;        /* enum bm_body_motion */
     float applied_force[3];   /* added to force at the start of each coupled substep (gravity * mass) */
     float applied_torque[3];  /* added to torque likewise (a drive torque) */
     float half_extents[3];    /* collider shape, as bm_collider */
@@ -238,7 +244,9 @@ int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallback
  * coefficients, then a per-block P2G accumulating each cell's particles in
  * registers; the default, variant 4); without bit 2 the fused G2P+P2G
  * kernel runs, with bit 0 = plain read-modify-write LDS deposit (else
- * collision-free LDS atomics) and bit 1 = G2P node velocities staged in LDS.
+ * collision-free LDS atomics) and bit 1 = G2P node velocities staged in LDS;
+ * bit 3 (with bit 2) = the per-cell deposit in two passes over each cell's
+ * particles (fewer registers, particle data read twice).
  * Values <= 0 / < 0 keep the current setting.  No effect in
  * BM_MODE_REFERENCE. */
 int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_every, int32_t variant);
@@ -262,6 +270,64 @@ int bm_mpm_set_timing(bm_mpm_handle h, int32_t on);
 /* Synchronises, then returns the accumulated milliseconds and launch counts
  * per phase (arrays of BM_NUM_PHASES); reset != 0 clears the accumulators. */
 int bm_mpm_read_timing(bm_mpm_handle h, int32_t reset, double* ms, int64_t* launches);
+
+/* ---------------------------------------------------------------- batches */
+/* Many independent scenes advanced together (no counterpart in the
+ * reference, whose solver holds one scene).  The scenes share device arrays:
+ * particles of scene s occupy one contiguous range, its node box one range of
+ * grid blocks, and every kernel of a substep runs once for all scenes, so
+ * small scenes fill the GPU together.  Each scene has its own dx, box,
+ * material, mass quantum and optionally a device body coupled by M0 or M1.
+ * The physics is BM_MODE_FAST's, scene by scene.  Scenes can be added and
+ * removed between steps (the remaining scenes are packed; their state is
+ * kept).  With graphs on, the substeps are captured once per batch layout
+ * and replayed (the launches of 2 x sort_every substeps per replay). */
+typedef struct bm_batch* bm_batch_handle;
+
+typedef struct {
+    int64_t max_particles;  /* particles of all scenes together */
+    int64_t max_blocks;     /* 4^3-node grid blocks of all scene boxes together */
+    int32_t max_scenes;
+    void* stream;           /* hipStream_t; NULL = the batch creates its own */
+} bm_batch_desc;
+
+typedef struct {
+    float dx;
+    int32_t grid_origin[3];  /* the scene's node box, as bm_mpm_desc */
+    int32_t grid_dims[3];
+    float young_modulus, poisson_ratio, bulk_density;
+    float plastic_alpha;
+    float floor_friction;
+    float gravity[3];
+    int32_t walls;           /* as bm_mpm_set_walls */
+    int32_t coupler;         /* BM_COUPLER_* for the scene's body; ignored without a body */
+} bm_scene_desc;
+
+int bm_batch_create(const bm_batch_desc* desc, bm_batch_handle* out);
+int bm_batch_destroy(bm_batch_handle b);
+int bm_batch_synchronize(bm_batch_handle b);
+/* Add a scene with n particles (full state, as bm_mpm_upload_particles) and
+ * an optional device body (NULL: none).  BM_ERR_OOM when the batch lacks
+ * particle, block or scene room -- the caller's admission signal. */
+int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* desc, int64_t n, const bm_particles* host,
+                       const bm_body* body, int32_t* scene_id);
+int bm_batch_remove_scene(bm_batch_handle b, int32_t scene_id);
+/* k coupled substeps of every scene (p2g, grid update, the scene's body
+ * coupling, g2p). */
+int bm_batch_step(bm_batch_handle b, float dt, int32_t k);
+int bm_batch_download_scene(bm_batch_handle b, int32_t scene_id, bm_particles* host);
+int bm_batch_get_body(bm_batch_handle b, int32_t scene_id, bm_body* body, bm_reaction* reaction);
+/* Scenes, particles and grid blocks in use. */
+int bm_batch_usage(bm_batch_handle b, int32_t* scenes, int64_t* particles, int64_t* blocks);
+/* Launch parameters as bm_mpm_set_tuning (unit_particles, sort_every). */
+int bm_batch_set_tuning(bm_batch_handle b, int32_t unit_particles, int32_t sort_every);
+/* Time `substeps` substeps for each candidate (unit 256 / 384 / 512 x
+ * sort_every 8 / 16 / 32), keep the fastest and report it.  The scenes'
+ * state is restored afterwards; needs device memory for one copy of the
+ * particles. */
+int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* unit_particles, int32_t* sort_every);
+/* Capture the substeps in a HIP graph and replay it (on != 0). */
+int bm_batch_set_graph(bm_batch_handle b, int32_t on);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,8 @@ int main(void) {
   S(bm_collider); F(bm_collider, quat); F(bm_collider, friction); F(bm_collider, shape);
   S(bm_reaction); F(bm_reaction, nodes_inside);
   S(bm_body); F(bm_body, inv_mass); F(bm_body, integrate); F(bm_body, applied_torque); F(bm_body, shape);
+  S(bm_batch_desc); F(bm_batch_desc, max_scenes); F(bm_batch_desc, stream);
+  S(bm_scene_desc); F(bm_scene_desc, young_modulus); F(bm_scene_desc, gravity); F(bm_scene_desc, coupler);
   return 0;
 }
 """
@@ -99,7 +101,8 @@ def test_struct_layouts_match_the_python_mirror(tmp_path):
             t, m, v = line.split()
             want[(t, m)] = int(v)
     py = {"bm_mpm_desc": mpm.Desc, "bm_particles": mpm.HostParticles, "bm_particles_device": mpm.DeviceParticles,
-          "bm_collider": mpm.ColliderC, "bm_reaction": mpm.ReactionC, "bm_body": mpm.BodyC}
+          "bm_collider": mpm.ColliderC, "bm_reaction": mpm.ReactionC, "bm_body": mpm.BodyC,
+          "bm_batch_desc": mpm.BatchDescC, "bm_scene_desc": mpm.SceneDescC}
     for (t, m), v in want.items():
         if m == "sizeof":
             assert C.sizeof(py[t]) == v, (t, C.sizeof(py[t]), v)
