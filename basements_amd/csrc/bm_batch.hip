@@ -98,7 +98,7 @@ struct bm_batch {
     bool prepared = false, binned = false;
     float dt_prepared = 0.0f;
     int since_bin = 0;
-    int unit_p = 512, sort_every = 16;
+    int unit_p = 512, sort_every = 16, deposit_groups = 4096;
 
     // Captured substeps.  A replay starts and ends in the state it was
     // captured in (which particle buffer is current, which grid, re-binning
@@ -220,6 +220,7 @@ static void batch_pass(bm_batch_handle b, bool do_g2p, const GridBuf& gin, const
     sb.n_chunks = b->n_chunks;
     sb.n_units_dev = b->bs.n_units;
     sb.n_units_bound = b->n_units_bound();
+    sb.deposit_groups = b->deposit_groups;
     // the clear before every pass also zeroes the redo counter; the pass
     // empties the output grid's block list
     launch_clear(gout, (int)std::min<int64_t>(b->B, 1 << 30), b->stream, b->n_redo, true);
@@ -597,8 +598,9 @@ int bm_batch_usage(bm_batch_handle b, int32_t* scenes, int64_t* particles, int64
     return BM_OK;
 }
 
-int bm_batch_set_tuning(bm_batch_handle b, int32_t unit_particles, int32_t sort_every) {
+int bm_batch_set_tuning(bm_batch_handle b, int32_t unit_particles, int32_t sort_every, int32_t deposit_groups) {
     if (int rc = check_batch(b)) return rc;
+    if (deposit_groups > 0) b->deposit_groups = (deposit_groups + 7) / 8 * 8;
     if (unit_particles > 0) {
         if (unit_particles < 16 || unit_particles > FUSED_UNIT_MAX)
             return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", FUSED_UNIT_MAX);
@@ -616,7 +618,8 @@ int bm_batch_set_graph(bm_batch_handle b, int32_t on) {
     return BM_OK;
 }
 
-int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* unit_out, int32_t* sort_out) {
+int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* unit_out, int32_t* sort_out,
+                      int32_t* groups_out) {
     if (int rc = check_batch(b)) return rc;
     if (substeps <= 0 || !(dt > 0.0f)) return fail(BM_ERR_INVALID_ARG, "substeps and dt must be > 0");
     if (b->scenes.empty()) return fail(BM_ERR_STATE, "no scenes");
@@ -654,39 +657,59 @@ int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* un
         return BM_OK;
     };
     int rc = save();
-    const int units[3] = {256, 384, 512}, sorts[3] = {8, 16, 32};
-    float best = 1e30f;
-    int bu = b->unit_p, bsrt = b->sort_every;
+    struct Cand {
+        int unit, sort, groups;
+        float ms;
+    };
+    std::vector<Cand> cands;
+    for (int u : {384, 512})
+        for (int so : {8, 16, 32})
+            for (int g : {2048, 4096, 8192}) cands.push_back({u, so, g, 0.0f});
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == BM_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = fail(BM_ERR_HIP, "event creation");
     const bool graph = b->use_graph;
     b->use_graph = false;
-    for (int iu = 0; rc == BM_OK && iu < 3; ++iu)
-        for (int is = 0; rc == BM_OK && is < 3; ++is) {
-            if ((rc = restore())) break;
-            b->unit_p = units[iu];
-            b->sort_every = sorts[is];
-            for (int w = 0; w < 2; ++w) batch_substep(b, dt);
-            hipEventRecord(e0, b->stream);
-            for (int i = 0; i < substeps; ++i) batch_substep(b, dt);
-            hipEventRecord(e1, b->stream);
-            if (hipEventSynchronize(e1) != hipSuccess) {
-                rc = fail(BM_ERR_HIP, "autotune timing");
-                break;
-            }
-            float ms = 0.0f;
-            hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) {
-                best = ms;
-                bu = units[iu];
-                bsrt = sorts[is];
-            }
+    // time one candidate from the saved state over a whole number of
+    // re-binning cycles (each re-binning falls inside the window)
+    auto trial = [&](Cand& c) -> int {
+        if (int r = restore()) return r;
+        b->unit_p = c.unit;
+        b->sort_every = c.sort;
+        b->deposit_groups = c.groups;
+        const int window = (std::max(substeps, c.sort) + c.sort - 1) / c.sort * c.sort;
+        for (int w = 0; w < 2; ++w) batch_substep(b, dt);
+        hipEventRecord(e0, b->stream);
+        for (int i = 0; i < window; ++i) batch_substep(b, dt);
+        hipEventRecord(e1, b->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) return fail(BM_ERR_HIP, "autotune timing");
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        c.ms += ms / (float)window;
+        return BM_OK;
+    };
+    // clocks settle over the first trials: one untimed pass, then a forward
+    // and a backward pass so that drift weighs on every candidate alike
+    if (rc == BM_OK) {
+        Cand warm = cands[cands.size() / 2];
+        rc = trial(warm);
+    }
+    for (size_t i = 0; rc == BM_OK && i < cands.size(); ++i) rc = trial(cands[i]);
+    for (size_t i = cands.size(); rc == BM_OK && i-- > 0;) rc = trial(cands[i]);
+    int bu = b->unit_p, bsrt = b->sort_every, bg = b->deposit_groups;
+    float best = 1e30f;
+    for (const Cand& c : cands)
+        if (rc == BM_OK && c.ms < best) {
+            best = c.ms;
+            bu = c.unit;
+            bsrt = c.sort;
+            bg = c.groups;
         }
     b->use_graph = graph;
     if (rc == BM_OK) rc = restore();
     b->unit_p = bu;
     b->sort_every = bsrt;
+    b->deposit_groups = bg;
     b->graph_valid = false;
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
@@ -698,6 +721,7 @@ int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* un
     if (rc == BM_OK) {
         if (unit_out) *unit_out = bu;
         if (sort_out) *sort_out = bsrt;
+        if (groups_out) *groups_out = bg;
     }
     return rc;
 }

@@ -256,6 +256,7 @@ struct SplitBatch {
     int n_chunks = 0;
     const int* n_units_dev = nullptr;
     int n_units_bound = 0;
+    int deposit_groups = 4096;  // fixed grid of the deposit when the count is on the device
 };
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,

@@ -20,6 +20,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "mpm_fast_device.hpp"
 
 namespace bm {
@@ -345,18 +347,14 @@ __device__ __forceinline__ void chain_slabs(float4* pt, int lane, const v3<float
             }
 }
 
-// NPASS = 1: all 27 nodes in one pass over the cell's particles.  NPASS = 2:
-// slabs 0-1, then slab 2 in a second pass (fewer live registers, more waves
-// per SIMD, the particle data read twice).
+// One unit's deposit.  NPASS = 1: all 27 nodes in one pass over the cell's
+// particles.  NPASS = 2: slabs 0-1, then slab 2 in a second pass (fewer live
+// registers, more waves per SIMD, the particle data read twice).
 template <int NPASS>
-__global__ void __launch_bounds__(64) k_p2g_cell(const float4* __restrict__ rec,
-                                                 const int4* __restrict__ units, GridBuf gout, uint8_t* bever,
-                                                 Domain d, FastArgs fa, int* err, unsigned long long* counters,
-                                                 SceneView sv) {
-    __shared__ CellSmem S;
-    const int per = (gridDim.x + 7) / 8;
-    const int u = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (u >= (fa.n_units_dev ? *fa.n_units_dev : fa.n_units)) return;
+__device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __restrict__ rec,
+                                         const int4* __restrict__ units, const GridBuf& gout, uint8_t* bever,
+                                         Domain d, const FastArgs& fa, int* err, unsigned long long* counters,
+                                         const SceneView& sv) {
     const int4 U = units[u];
     if (sv.sc) d = sv.sc[U.w].d;  // a batch: the unit's scene
     const int blk = U.x;
@@ -479,6 +477,30 @@ __global__ void __launch_bounds__(64) k_p2g_cell(const float4* __restrict__ rec,
     }
 }
 
+template <int NPASS>
+__global__ void __launch_bounds__(64) k_p2g_cell(const float4* __restrict__ rec,
+                                                 const int4* __restrict__ units, GridBuf gout, uint8_t* bever,
+                                                 Domain d, FastArgs fa, int* err, unsigned long long* counters,
+                                                 SceneView sv) {
+    __shared__ CellSmem S;
+    if (!fa.n_units_dev) {  // one workgroup per unit; consecutive units on one XCD
+        const int per = (gridDim.x + 7) / 8;
+        const int u = (blockIdx.x % 8) * per + blockIdx.x / 8;
+        if (u >= fa.n_units) return;
+        p2g_unit<NPASS>(u, S, rec, units, gout, bever, d, fa, err, counters, sv);
+        return;
+    }
+    // The unit count is known on the device only (a batch): a fixed grid (a
+    // multiple of 8 workgroups) walks the units; XCD x takes the x-th eighth.
+    const int nu = *fa.n_units_dev;
+    const int x = blockIdx.x % 8, j = blockIdx.x / 8, nj = gridDim.x / 8;
+    const int lo = (int)((int64_t)nu * x / 8), hi = (int)((int64_t)nu * (x + 1) / 8);
+    for (int u = lo + j; u < hi; u += nj) {
+        p2g_unit<NPASS>(u, S, rec, units, gout, bever, d, fa, err, counters, sv);
+        __syncthreads();  // the next unit reuses the LDS
+    }
+}
+
 
 // G2P (or the coefficients of the current state when !do_g2p), the FP64
 // pass for the particles it listed, then the ranked P2G.
@@ -519,7 +541,8 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
         hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters, sv);
     e0 = e1 = nullptr;
     if (mark) mark(ctx, SPLIT_DEPOSIT, &e0, &e1);
-    const int nu = batch.n_units_dev ? batch.n_units_bound : n_units;
+    // a batch walks the device-counted units with a fixed grid
+    const int nu = batch.n_units_dev ? std::min(batch.n_units_bound, batch.deposit_groups) : n_units;
     if (nu > 0) {
         const int nwg = ((nu + 7) / 8) * 8;
         auto kern = two_pass ? k_p2g_cell<2> : k_p2g_cell<1>;

@@ -209,8 +209,9 @@ _SIGS = {
     "bm_batch_download_scene": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(HostParticles)]),
     "bm_batch_get_body": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BodyC), C.POINTER(ReactionC)]),
     "bm_batch_usage": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
-    "bm_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
-    "bm_batch_autotune": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "bm_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "bm_batch_autotune": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int32)]),
     "bm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "bm_mpm_download_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -660,13 +661,14 @@ class Batch:
         _check(lib().bm_batch_usage(self.h, C.byref(s), C.byref(p), C.byref(b)))
         return s.value, p.value, b.value
 
-    def set_tuning(self, unit_particles=0, sort_every=0):
-        _check(lib().bm_batch_set_tuning(self.h, int(unit_particles), int(sort_every)))
+    def set_tuning(self, unit_particles=0, sort_every=0, deposit_groups=0):
+        _check(lib().bm_batch_set_tuning(self.h, int(unit_particles), int(sort_every), int(deposit_groups)))
 
-    def autotune(self, dt, substeps=8):
-        u, s = C.c_int32(), C.c_int32()
-        _check(lib().bm_batch_autotune(self.h, float(dt), int(substeps), C.byref(u), C.byref(s)))
-        return u.value, s.value
+    def autotune(self, dt, substeps=32):
+        """(unit_particles, sort_every, deposit_groups) of the fastest candidate."""
+        u, s, g = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().bm_batch_autotune(self.h, float(dt), int(substeps), C.byref(u), C.byref(s), C.byref(g)))
+        return u.value, s.value, g.value
 
     def set_graph(self, on=True):
         _check(lib().bm_batch_set_graph(self.h, 1 if on else 0))

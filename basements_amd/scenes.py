@@ -123,3 +123,64 @@ def wheel_bed(seed=0, friction_deg=None, cells=(120, 36, 120), scale=1):
             "speed": 0.2, "load": 50.0, "friction": 0.5, "method": 1}
     return Scene("wheel_bed", dx, P, origin=origin, dims=dims, alpha=dp_alpha(friction_deg), tool=tool,
                  substeps=2000)
+
+
+def tool_scene(seed, n_target, grid_edge=None, name="tool"):
+    """One soil-tool scene of the batched workload: a soil bed of about
+    n_target particles (8 per cell, width = 2 x height) in a cubic box of
+    grid_edge nodes (32..128; the smallest that holds the bed when None), a box
+    or cylinder tool sunk into the bed top and dragged along x at a speed
+    from U(0.05, 0.5) m/s (kinematic), friction angle from U(25, 40) deg,
+    E log-uniform in [1e5, 1e6] Pa.  dt = 1e-4 s."""
+    r = _rng(1_000_003 + seed)
+    cells = max(8.0, n_target / 8.0)
+    H = max(3, int(round((cells / 4.0) ** (1.0 / 3.0))))
+    W = max(4, int(round(math.sqrt(cells / H))))
+    edges = (32, 48, 64, 96, 128)
+    G = grid_edge or next((e for e in edges if e >= W + 16), 128)
+    dx = 1.0 / G
+    lo = ((G - W) // 2, 4, (G - W) // 2)
+    hi = (lo[0] + W, 4 + H, lo[2] + W)
+    P = soil_block(dx, lo, hi, seed=seed, rho=1600.0)
+    E = float(10.0 ** r.uniform(5.0, 6.0))
+    phi = float(r.uniform(25.0, 40.0))
+    speed = float(r.uniform(0.05, 0.5))
+    kind = "box" if r.random() < 0.5 else "cylinder"
+    top = hi[1] * dx
+    size = max(3, W // 6) * dx  # the tool spans a sixth of the bed
+    cx = (lo[0] + 0.25 * W) * dx
+    cz = (lo[2] + 0.5 * W) * dx
+    if kind == "box":
+        he = (0.5 * size, 0.5 * size, 0.5 * size)
+        tool = {"kind": "box", "half": he, "center": (cx, top + 0.5 * size - 2 * dx, cz), "speed": speed,
+                "friction": 0.5, "mass": 1.0, "method": 0}
+    else:
+        R = 0.5 * size
+        tool = {"kind": "cylinder", "radius": R, "half_len": 0.5 * size,
+                "center": (cx, top + R - 2 * dx, cz), "speed": speed, "friction": 0.5, "load": 10.0, "method": 1}
+    return Scene(name, dx, P, origin=(0, 0, 0), dims=(G, G, G), E=E, rho=1600.0, alpha=dp_alpha(phi), tool=tool,
+                 substeps=500)
+
+
+def batch_scenes(seed=0, count=64, n_lo=2e4, n_hi=5e5):
+    """The batched workload: `count` tool scenes with particle counts
+    log-uniform in [n_lo, n_hi]."""
+    r = _rng(seed)
+    ns = np.exp(r.uniform(math.log(n_lo), math.log(n_hi), size=count))
+    return [tool_scene(seed * 1000 + i, float(n), name=f"tool{i}") for i, n in enumerate(ns)]
+
+
+def tool_body(tool):
+    """The kinematic body of a tool scene (dragged along x at its speed)."""
+    from basements_amd import BODY_KINEMATIC, SHAPE_BOX, SHAPE_CYLINDER, Body
+
+    if tool["kind"] == "cylinder":
+        R = tool["radius"]
+        return Body(position=tool["center"], orientation=(math.cos(math.pi / 4), math.sin(math.pi / 4), 0.0, 0.0),
+                    linear_velocity=(tool["speed"], 0.0, 0.0), angular_velocity=(0.0, 0.0, -tool["speed"] / R),
+                    inv_mass=1.0 / tool["load"], linear_damping=0.0, angular_damping=0.0,
+                    half_extents=(R, tool["half_len"], R), friction=tool["friction"], mass=tool["load"],
+                    shape=SHAPE_CYLINDER, integrate=BODY_KINEMATIC)
+    return Body(position=tool["center"], linear_velocity=(tool["speed"], 0.0, 0.0), inv_mass=1.0 / tool["mass"],
+                linear_damping=0.0, angular_damping=0.0, half_extents=tool["half"], friction=tool["friction"],
+                mass=tool["mass"], shape=SHAPE_BOX, integrate=BODY_KINEMATIC)

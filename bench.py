@@ -74,7 +74,13 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="wheel_bed", choices=["wheel_bed", "box_drop"])
+    ap.add_argument("--config", default="wheel_bed", choices=["wheel_bed", "box_drop", "batch"],
+                    help="wheel_bed (the default workload), box_drop, or batch: 64 uneven tool scenes stepped "
+                         "together against the same scenes one at a time on the single-scene path")
+    ap.add_argument("--scenes", type=int, default=64, help="scenes of --config batch")
+    ap.add_argument("--graph", type=int, default=1, help="--config batch: replay captured substeps")
+    ap.add_argument("--serial", type=int, default=1, help="--config batch: also time the scenes one at a time")
+    ap.add_argument("--tuning", default="", help="--config batch: unit,sort_every,deposit_groups instead of autotune")
     ap.add_argument("--mode", default="fast", choices=["reference", "fast"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", type=int, default=1)
@@ -164,6 +170,75 @@ def measured_traffic(workload, mode, kernel):
     return {"bytes": b, "source": TRAFFIC_FILE[len(ROOT) + 1:]}
 
 
+def bench_batch(a):
+    """Batched uneven scenes: aggregate particle-updates/s of all scenes in one
+    batch (autotuned, graph-replayed) against running the same scenes one at a
+    time, each on its own single-scene handle, for the same substeps."""
+    from basements_amd import M0, M1, MODE_FAST, Batch, SceneDesc, Solver
+    from basements_amd.scenes import batch_scenes, tool_body
+
+    scenes = batch_scenes(a.seed, a.scenes)
+    dt = 1e-4
+    n_total = sum(s.n for s in scenes)
+    blocks = sum((-(-s.dims[0] // 4)) * (-(-s.dims[1] // 4)) * (-(-s.dims[2] // 4)) for s in scenes)
+
+    def desc(s):
+        return SceneDesc(dx=s.dx, origin=s.origin, dims=s.dims, E=s.E, nu=s.nu, rho=s.rho, alpha=s.alpha,
+                         mu_floor=s.mu_floor, gravity=s.gravity, coupler=M1 if s.tool["method"] == 1 else M0)
+
+    B = Batch(max_particles=n_total + 64, max_blocks=blocks, max_scenes=len(scenes))
+    ids = [B.add_scene(desc(s), s.particles, tool_body(s.tool)) for s in scenes]
+    if a.tuning:
+        unit, sort_every, groups = (int(t) for t in a.tuning.split(","))
+        B.set_tuning(unit, sort_every, groups)
+    else:
+        unit, sort_every, groups = B.autotune(dt, substeps=32)
+    B.set_graph(bool(a.graph))
+    # warm-up long enough to capture the replayed substeps (two re-binning cycles)
+    warmup = max(a.warmup, 3 * sort_every)
+    B.step(dt, warmup)
+    B.synchronize()
+    t0 = time.perf_counter()
+    B.step(dt, a.steps)
+    B.synchronize()
+    t_batch = time.perf_counter() - t0
+    del B
+
+    # the same scenes one at a time on the plain path
+    t_serial = 0.0
+    for s in scenes if a.serial else []:
+        S = Solver(s.dx, s.origin, s.dims, s.n, MODE_FAST)
+        S.set_material(s.E, s.nu, s.rho)
+        S.set_plastic_alpha(s.alpha)
+        S.set_floor_friction(s.mu_floor)
+        S.set_gravity(s.gravity)
+        S.upload(s.particles)
+        S.set_body(tool_body(s.tool))
+        method = M1 if s.tool["method"] == 1 else M0
+        S.step_coupled(method, dt, a.warmup)
+        S.synchronize()
+        t0 = time.perf_counter()
+        S.step_coupled(method, dt, a.steps)
+        S.synchronize()
+        t_serial += time.perf_counter() - t0
+        S.close()
+    value = n_total * a.steps / t_batch
+    serial = n_total * a.steps / t_serial if t_serial > 0 else None
+    out = {"metric": "particle-updates/s (batched scenes, aggregate)", "value": value,
+           "unit": "particle-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": warmup,
+           "ms_per_step": t_batch / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32", "data": "synthetic (seeded soil beds with dragged tools, no external data)",
+           "config": {"workload": "batch", "scenes": len(scenes), "particles": n_total, "grid_blocks": blocks,
+                      "particles_min": min(s.n for s in scenes), "particles_max": max(s.n for s in scenes),
+                      "grids": sorted({s.dims[0] for s in scenes}), "dt": dt, "graph": bool(a.graph),
+                      "tuned_unit_particles": unit, "tuned_sort_every": sort_every,
+                      "tuned_deposit_groups": groups},
+           "serial_value": serial, "serial_ms_per_step": t_serial / a.steps * 1e3 if serial else None,
+           "speedup_vs_serial": value / serial if serial else None}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -173,6 +248,8 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return 0
+    if a.config == "batch":
+        return bench_batch(a)
 
     from basements_amd import M0, M1, MODE_FAST, MODE_REFERENCE, Solver
     from basements_amd.scenes import box_drop, wheel_bed

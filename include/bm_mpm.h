@@ -319,13 +319,17 @@ int bm_batch_download_scene(bm_batch_handle b, int32_t scene_id, bm_particles* h
 int bm_batch_get_body(bm_batch_handle b, int32_t scene_id, bm_body* body, bm_reaction* reaction);
 /* Scenes, particles and grid blocks in use. */
 int bm_batch_usage(bm_batch_handle b, int32_t* scenes, int64_t* particles, int64_t* blocks);
-/* Launch parameters as bm_mpm_set_tuning (unit_particles, sort_every). */
-int bm_batch_set_tuning(bm_batch_handle b, int32_t unit_particles, int32_t sort_every);
-/* Time `substeps` substeps for each candidate (unit 256 / 384 / 512 x
- * sort_every 8 / 16 / 32), keep the fastest and report it.  The scenes'
+/* Launch parameters: unit_particles and sort_every as bm_mpm_set_tuning;
+ * deposit_groups = workgroups of the cell deposit, which walk the units the
+ * device counted (default 4096).  Values <= 0 keep the current setting. */
+int bm_batch_set_tuning(bm_batch_handle b, int32_t unit_particles, int32_t sort_every, int32_t deposit_groups);
+/* Time each candidate (unit 384 / 512 x sort_every 8 / 16 / 32 x deposit
+ * groups 2048 / 4096 / 8192) over at least `substeps` substeps (a whole
+ * number of re-binning cycles), keep the fastest and report it.  The scenes'
  * state is restored afterwards; needs device memory for one copy of the
  * particles. */
-int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* unit_particles, int32_t* sort_every);
+int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* unit_particles, int32_t* sort_every,
+                      int32_t* deposit_groups);
 /* Capture the substeps in a HIP graph and replay it (on != 0). */
 int bm_batch_set_graph(bm_batch_handle b, int32_t on);
 

@@ -127,8 +127,8 @@ def test_graph_replays_match_eager_steps():
     scenes = _scenes()
     E, eids = _batch(scenes, graph=False)
     G, gids = _batch(scenes, graph=True)
-    E.set_tuning(0, 4)  # short re-binning cycle: 2 x 4 = 8 substeps per replay
-    G.set_tuning(0, 4)
+    E.set_tuning(0, 4, 0)  # short re-binning cycle: 2 x 4 = 8 substeps per replay
+    G.set_tuning(0, 4, 0)
     for rnd in range(3):
         E.step(DT, 12)
         G.step(DT, 12)
@@ -153,8 +153,8 @@ def test_autotune_keeps_the_scenes_state():
     scenes = _scenes()
     A, aids = _batch(scenes)
     T, tids = _batch(scenes)
-    unit, sort_every = T.autotune(DT, substeps=4)
-    assert unit in (256, 384, 512) and sort_every in (8, 16, 32)
+    unit, sort_every, groups = T.autotune(DT, substeps=8)
+    assert unit in (384, 512) and sort_every in (8, 16, 32) and groups in (2048, 4096, 8192)
     A.step(DT, 10)
     T.step(DT, 10)
     for a, b in zip(aids, tids):
