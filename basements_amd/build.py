@@ -39,6 +39,7 @@ UNITS = [
     ("mpm_split.hip", []),
     ("bm_mpm_api.hip", ["-ffp-contract=off"]),
     ("bm_batch.hip", []),
+    ("bm_serve.hip", []),
 ]
 
 

@@ -33,6 +33,7 @@ struct SceneHost {
     int64_t p0, n, nb;
     int q_exp;
     bool has_body;
+    int err = 0;  // device error bits raised by this scene (sticky)
 };
 
 __global__ void k_fill_scene(int* block_scene, int64_t b0, int64_t nb, int s) {
@@ -191,16 +192,38 @@ static int upload_tables(bm_batch_handle b) {
     return BM_OK;
 }
 
-static int batch_check_err(bm_batch_handle b) {
+// Move the device error words into the scenes' sticky flags (the per-scene
+// words are indexed by the current table order, so this runs before any
+// change of that order).
+static int batch_harvest_err(bm_batch_handle b) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     int err = 0;
     HIP_TRY(hipMemcpy(&err, b->d_err, sizeof(int), hipMemcpyDeviceToHost));
     if (err) {
-        int zero = 0;
-        HIP_TRY(hipMemcpy(b->d_err, &zero, sizeof(int), hipMemcpyHostToDevice));
-        if (err & 1) return fail(BM_ERR_OUT_OF_DOMAIN, "a particle stencil left its scene's grid box");
-        return fail(BM_ERR_HIP, "device error flag %d", err);
+        const int ns = (int)b->scenes.size();
+        std::vector<int> w((size_t)ns + 1);
+        HIP_TRY(hipMemcpy(w.data(), b->d_err, sizeof(int) * (ns + 1), hipMemcpyDeviceToHost));
+        for (int s = 0; s < ns; ++s) b->scenes[s].err |= w[(size_t)s + 1];
+        HIP_TRY(hipMemset(b->d_err, 0, sizeof(int) * (b->capS + 1)));
     }
+    return BM_OK;
+}
+
+static int scene_err_code(int bits) {
+    if (bits & 1) return BM_ERR_OUT_OF_DOMAIN;
+    return bits ? BM_ERR_HIP : BM_OK;
+}
+
+static int scene_fail(const SceneHost& S) {
+    if (S.err & 1) return fail(BM_ERR_OUT_OF_DOMAIN, "scene %d: a particle stencil left the scene's grid box", S.id);
+    return fail(BM_ERR_HIP, "scene %d: device error flag %d", S.id, S.err);
+}
+
+// Harvest, then fail if scene s (or, with s < 0, any scene) has raised an error.
+static int batch_check_err(bm_batch_handle b, int s = -1) {
+    if (int rc = batch_harvest_err(b)) return rc;
+    for (int t = 0; t < (int)b->scenes.size(); ++t)
+        if ((s < 0 || t == s) && b->scenes[t].err) return scene_fail(b->scenes[t]);
     return BM_OK;
 }
 
@@ -285,7 +308,31 @@ static int graph_for_state(bm_batch_handle b, float dt, hipGraphExec_t* out) {
     return BM_OK;
 }
 
+// Device bytes that bm_batch_create allocates per particle, per grid block
+// and per scene of capacity (the unit list and chunk table rounded up).
+static void batch_unit_bytes(int64_t& per_particle, int64_t& per_block, int64_t& per_scene) {
+    per_particle = (int64_t)(sizeof(float) * PCOMP * 2      // particle state, double-buffered for re-binning
+                             + sizeof(int) * 2              // insertion ids, double-buffered
+                             + sizeof(float) * 16           // deposit records
+                             + sizeof(int) * 3              // FP64 redo list, bin key, bin slot
+                             + 2);                          // deposit units and chunk table
+    per_block = (int64_t)(2 + sizeof(int)                   // touched flags, block -> scene
+                          + 2 * (sizeof(long long) * 64 + sizeof(float4) * 64 + sizeof(uint32_t) + sizeof(int32_t))
+                          + sizeof(int) * 4 + sizeof(int4)  // bin counts and starts, units
+                          + 8);                             // scan scratch
+    per_scene = (int64_t)(sizeof(SceneDev) + sizeof(bm_body) + sizeof(ColliderDev) + sizeof(int) * 3 +
+                          sizeof(double) * (7 * COUPLE_WGS + 8) + sizeof(int4));
+}
+
 extern "C" {
+
+int bm_batch_footprint(int64_t particles, int64_t blocks, int32_t scenes, int64_t* bytes) {
+    if (!bytes || particles < 0 || blocks < 0 || scenes < 0) return fail(BM_ERR_INVALID_ARG, "bad footprint query");
+    int64_t pp, pb, ps;
+    batch_unit_bytes(pp, pb, ps);
+    *bytes = pp * particles + pb * blocks + ps * scenes + 256;
+    return BM_OK;
+}
 
 int bm_batch_create(const bm_batch_desc* desc, bm_batch_handle* out) {
     if (!desc || !out) return fail(BM_ERR_INVALID_ARG, "null argument");
@@ -317,7 +364,7 @@ int bm_batch_create(const bm_batch_desc* desc, bm_batch_handle* out) {
               A((void**)&b->pid, sizeof(int) * cp) && A((void**)&b->pid2, sizeof(int) * cp) &&
               A((void**)&b->coef, sizeof(float) * 16 * cp) && A((void**)&b->redo, sizeof(int) * cp) &&
               A((void**)&b->n_redo, sizeof(int)) && A((void**)&b->bever, cb) && A((void**)&b->bever_tmp, cb) &&
-              A((void**)&b->d_err, sizeof(int)) && A((void**)&b->d_counters, 2 * sizeof(unsigned long long)) &&
+              A((void**)&b->d_err, sizeof(int) * (b->capS + 1)) && A((void**)&b->d_counters, 2 * sizeof(unsigned long long)) &&
               A((void**)&b->block_scene, sizeof(int) * cb) && A((void**)&b->chunks, sizeof(int4) * b->cap_chunks) &&
               A((void**)&b->d_sc, sizeof(SceneDev) * b->capS) && A((void**)&b->d_bodies, sizeof(bm_body) * b->capS) &&
               A((void**)&b->d_cols, sizeof(ColliderDev) * b->capS) && A((void**)&b->d_has_body, sizeof(int) * b->capS) &&
@@ -346,7 +393,7 @@ int bm_batch_create(const bm_batch_desc* desc, bm_batch_handle* out) {
         ok = A(&bs.temp, bs.temp_bytes);
     }
     if (!ok) return bail(fail(BM_ERR_OOM, "batch allocation"));
-    if (hipMemset(b->bever, 0, cb) != hipSuccess || hipMemset(b->d_err, 0, sizeof(int)) != hipSuccess ||
+    if (hipMemset(b->bever, 0, cb) != hipSuccess || hipMemset(b->d_err, 0, sizeof(int) * (b->capS + 1)) != hipSuccess ||
         hipMemset(b->d_counters, 0, 2 * sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(b->d_out7, 0, sizeof(double) * 8 * b->capS) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess)
@@ -457,7 +504,7 @@ int bm_batch_remove_scene(bm_batch_handle b, int32_t id) {
     if (int rc = check_batch(b)) return rc;
     const int s = b->find(id);
     if (s < 0) return fail(BM_ERR_INVALID_ARG, "no scene %d", id);
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (int rc = batch_harvest_err(b)) return rc;  // before the table order changes
     const SceneHost R = b->scenes[s];
     const int64_t tailP = b->P - (R.p0 + R.n), tailB = b->B - (R.d.b0 + R.nb);
     const size_t pitch = sizeof(float) * b->capP;
@@ -544,7 +591,7 @@ int bm_batch_download_scene(bm_batch_handle b, int32_t id, bm_particles* p) {
     if (!p) return fail(BM_ERR_INVALID_ARG, "null output");
     const int s = b->find(id);
     if (s < 0) return fail(BM_ERR_INVALID_ARG, "no scene %d", id);
-    if (int rc = batch_check_err(b)) return rc;
+    if (int rc = batch_check_err(b, s)) return rc;
     const SceneHost& S = b->scenes[s];
     const int64_t n = S.n;
     if (n == 0) return BM_OK;
@@ -576,7 +623,7 @@ int bm_batch_get_body(bm_batch_handle b, int32_t id, bm_body* body, bm_reaction*
     const int s = b->find(id);
     if (s < 0) return fail(BM_ERR_INVALID_ARG, "no scene %d", id);
     if (!b->scenes[s].has_body) return fail(BM_ERR_STATE, "scene %d has no body", id);
-    if (int rc = batch_check_err(b)) return rc;
+    if (int rc = batch_check_err(b, s)) return rc;
     double o[7];
     HIP_TRY(hipMemcpy(body, b->d_bodies + s, sizeof(bm_body), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(o, b->d_out7 + (int64_t)s * 8, sizeof(double) * 7, hipMemcpyDeviceToHost));
@@ -587,6 +634,16 @@ int bm_batch_get_body(bm_batch_handle b, int32_t id, bm_body* body, bm_reaction*
         }
         r->nodes_inside = (int32_t)std::llround(o[6]);
     }
+    return BM_OK;
+}
+
+int bm_batch_scene_error(bm_batch_handle b, int32_t id, int32_t* code) {
+    if (int rc = check_batch(b)) return rc;
+    if (!code) return fail(BM_ERR_INVALID_ARG, "null output");
+    const int s = b->find(id);
+    if (s < 0) return fail(BM_ERR_INVALID_ARG, "no scene %d", id);
+    if (int rc = batch_harvest_err(b)) return rc;
+    *code = scene_err_code(b->scenes[s].err);
     return BM_OK;
 }
 

@@ -169,6 +169,13 @@ struct SceneView {
     int n_scenes;
 };
 
+// Device error flags: err[0] collects every code; a batch also keeps one word
+// per scene at err[1 + scene] so that a failing scene names itself.
+__device__ __forceinline__ void flag_error(int* err, int scene, int code) {
+    atomicOr(err, code);
+    if (scene >= 0) atomicOr(err + 1 + scene, code);
+}
+
 // ------------------------------------------------------------- launchers
 // Zero the blocks on g's list and empty the list; also zeroes *zero_word
 // (a counter of the next kernels) when given.  defer_reset: the list length

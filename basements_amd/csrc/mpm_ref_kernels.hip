@@ -138,7 +138,8 @@ __global__ void __launch_bounds__(256) k_p2g_ref(PView P, GridBuf g, uint8_t* be
             }
 }
 
-__device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, long long max_blocks, int* err);
+__device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, long long max_blocks, int* err,
+                           int scene = -1);
 
 // --------------------------------------------------------------- grid update
 // mpm_solver.h:250-291 plus the optional domain walls.  A force buffer is
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d0, Phys ph0, d
             if (blockIdx.x == 0) body_begin(body, col, d0, max_blocks, err);
         } else {  // one body per scene that has one (the launch has at least n_scenes workgroups)
             const int s = blockIdx.x;
-            if (s < sv.n_scenes && has_body[s]) body_begin(body + s, col + s, sv.sc[s].d, max_blocks, err);
+            if (s < sv.n_scenes && has_body[s]) body_begin(body + s, col + s, sv.sc[s].d, max_blocks, err, s);
         }
     }
     const int cnt = *g.bcount;
@@ -560,7 +561,8 @@ void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phy
 // applied loads go into the accumulators (RigidBody::apply_force /
 // apply_torque, rigid_body.h:168-182), then the collider is built from the
 // body's pose and velocities.
-__device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, long long max_blocks, int* err) {
+__device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, long long max_blocks, int* err,
+                           int scene) {
     bm_body b = *body;
     for (int a = 0; a < 3; ++a) {
         b.force[a] = b.force[a] + b.applied_force[a];
@@ -569,7 +571,7 @@ __device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, lon
     ColliderDev c = collider_build(b.position, b.orientation, b.half_extents, b.linear_velocity, b.angular_velocity,
                                    b.friction, b.mass, b.shape, d);
     if ((long long)c.nb[0] * c.nb[1] * c.nb[2] > max_blocks) {
-        atomicOr(err, 4);
+        flag_error(err, scene, 4);
         c.nb[0] = c.nb[1] = c.nb[2] = 0;
     }
     for (int a = 0; a < 3; ++a) {

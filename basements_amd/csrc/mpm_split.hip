@@ -73,10 +73,12 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
     // emptied here, before the deposit kernel appends to it
     if (blockIdx.x == 0 && threadIdx.x == 0 && fa.list_reset) *fa.list_reset = 0;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int scene = -1;
     if (sv.chunks) {  // a batch: one scene per workgroup
         const int4 ch = sv.chunks[blockIdx.x];
         if ((int)threadIdx.x >= ch.z) return;
         i = ch.y + threadIdx.x;
+        scene = ch.x;
         const SceneDev& S = sv.sc[ch.x];
         d = S.d;
         ph = S.ph;
@@ -96,7 +98,7 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
         make_stencil(x, y, z, inv_dx, s);
         const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
         if (!stencil_ok(d, lx, ly, lz)) {
-            atomicOr(err, 1);
+            flag_error(err, scene, 1);
             store_rec(rec, i, x, y, z, 0.0f, mk(0.f, 0.f, 0.f), m3<float>{{0, 0, 0, 0, 0, 0, 0, 0, 0}});
             return;
         }
@@ -432,7 +434,7 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
         const double qp = r.qp;
         const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
         if (!stencil_ok(d, lx, ly, lz)) {
-            atomicOr(err, 1);
+            flag_error(err, sv.sc ? U.w : -1, 1);
             continue;
         }
         const int qx = lx - tx0, qy = ly - ty0, qz = lz - tz0;

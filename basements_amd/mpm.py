@@ -169,6 +169,25 @@ class SceneDescC(C.Structure):
                 ("walls", C.c_int32), ("coupler", C.c_int32)]
 
 
+class ServerDescC(C.Structure):
+    _fields_ = [("batch", BatchDescC), ("hbm_budget", C.c_int64), ("dt", C.c_float), ("max_chunk", C.c_int32),
+                ("max_bypass", C.c_int32)]
+
+
+class RequestC(C.Structure):
+    _fields_ = [("scene", SceneDescC), ("n", C.c_int64), ("input", C.POINTER(HostParticles)),
+                ("body", C.POINTER(BodyC)), ("output", C.POINTER(HostParticles)), ("substeps", C.c_int32),
+                ("arrival", C.c_double)]
+
+
+class RequestInfoC(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("substeps_done", C.c_int32), ("pad", C.c_int32),
+                ("arrival", C.c_double), ("admitted", C.c_double), ("finished", C.c_double), ("body", BodyC),
+                ("reaction", ReactionC)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
+
 # ------------------------------------------------------------------ library
 _lib = None
 
@@ -209,9 +228,21 @@ _SIGS = {
     "bm_batch_download_scene": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(HostParticles)]),
     "bm_batch_get_body": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BodyC), C.POINTER(ReactionC)]),
     "bm_batch_usage": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "bm_batch_scene_error": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "bm_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "bm_batch_autotune": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                     C.POINTER(C.c_int32)]),
+    "bm_batch_footprint": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]),
+    "bm_server_create": (C.c_int, [C.POINTER(ServerDescC), C.POINTER(C.c_void_p)]),
+    "bm_server_destroy": (C.c_int, [C.c_void_p]),
+    "bm_server_clock": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "bm_server_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "bm_server_submit": (C.c_int, [C.c_void_p, C.POINTER(RequestC), C.POINTER(C.c_int64)]),
+    "bm_server_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]),
+    "bm_server_info": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestInfoC)]),
+    "bm_server_cancel": (C.c_int, [C.c_void_p, C.c_int64]),
+    "bm_server_load": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int64)]),
     "bm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "bm_mpm_download_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -608,7 +639,8 @@ class Batch:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().bm_batch_destroy(self.h)
+            if not getattr(self, "_borrowed", False):
+                lib().bm_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -661,6 +693,12 @@ class Batch:
         _check(lib().bm_batch_usage(self.h, C.byref(s), C.byref(p), C.byref(b)))
         return s.value, p.value, b.value
 
+    def scene_error(self, sid):
+        """BM_OK (0) or the error code the scene's download would raise."""
+        c = C.c_int32()
+        _check(lib().bm_batch_scene_error(self.h, int(sid), C.byref(c)))
+        return c.value
+
     def set_tuning(self, unit_particles=0, sort_every=0, deposit_groups=0):
         _check(lib().bm_batch_set_tuning(self.h, int(unit_particles), int(sort_every), int(deposit_groups)))
 
@@ -672,3 +710,138 @@ class Batch:
 
     def set_graph(self, on=True):
         _check(lib().bm_batch_set_graph(self.h, 1 if on else 0))
+
+
+def _host_particles(P):
+    """HostParticles over float32 copies of P's fields; returns (struct, arrays)."""
+    arrs = {k: _f32(P[k], -1) for k in ("x", "v", "F", "C", "m", "V0")}
+    hp = HostParticles(*[arrs[k].ctypes.data_as(C.POINTER(C.c_float)) for k in ("x", "v", "F", "C", "m", "V0")])
+    return hp, arrs
+
+
+def _empty_particles(n):
+    return {"x": np.zeros((n, 3), np.float32), "v": np.zeros((n, 3), np.float32),
+            "F": np.zeros((n, 3, 3), np.float32), "C": np.zeros((n, 3, 3), np.float32),
+            "m": np.zeros(n, np.float32), "V0": np.zeros(n, np.float32)}
+
+
+@dataclass
+class RequestInfo:
+    state: int
+    status: int
+    substeps_done: int
+    arrival: float
+    admitted: float
+    finished: float
+    body: Body | None
+    reaction: tuple | None
+
+    @property
+    def latency(self):
+        """Seconds from arrival to retirement."""
+        return self.finished - self.arrival
+
+
+def batch_footprint(particles, blocks, scenes):
+    """Device bytes of a batch with these capacities (bm_batch_footprint)."""
+    b = C.c_int64()
+    _check(lib().bm_batch_footprint(int(particles), int(blocks), int(scenes), C.byref(b)))
+    return b.value
+
+
+class Server:
+    """Request loop over one batch (bm_server_*): submit rollout requests with
+    arrival times, poll to admit / advance / retire, read results when done.
+    The Python side keeps every request's input and output arrays alive."""
+
+    def __init__(self, max_particles, max_blocks, max_scenes, dt, hbm_budget=0, max_chunk=0, max_bypass=0,
+                 stream=None):
+        d = ServerDescC()
+        d.batch.max_particles, d.batch.max_blocks, d.batch.max_scenes = int(max_particles), int(max_blocks), \
+            int(max_scenes)
+        d.batch.stream = stream
+        d.hbm_budget, d.dt, d.max_chunk, d.max_bypass = int(hbm_budget), float(dt), int(max_chunk), int(max_bypass)
+        h = C.c_void_p()
+        _check(lib().bm_server_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self._keep = {}  # id -> (input structs and arrays, output arrays, has_body)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self):
+        t = C.c_double()
+        _check(lib().bm_server_clock(self.h, C.byref(t)))
+        return t.value
+
+    def batch(self):
+        """The server's batch (for set_tuning / autotune / set_graph); not owned."""
+        h = C.c_void_p()
+        _check(lib().bm_server_batch(self.h, C.byref(h)))
+        b = Batch.__new__(Batch)
+        b.h, b.n, b._borrowed = h, {}, True
+        return b
+
+    def submit(self, desc: SceneDesc, P, substeps, body: Body | None = None, arrival=-1.0, output=True):
+        """Queue a request; returns its id.  BMError(BM_ERR_OOM) when it could
+        never fit.  `arrival` is on the server clock (< 0 = now)."""
+        n = int(np.asarray(P["x"]).reshape(-1, 3).shape[0])
+        hp, arrs = _host_particles(P)
+        out = _empty_particles(n) if output else None
+        op = _host_particles(out)[0] if output else None
+        r = RequestC()
+        r.scene = desc.to_c()
+        r.n = n
+        r.input = C.pointer(hp)
+        bc = body.to_c() if body is not None else None
+        r.body = C.pointer(bc) if bc is not None else None
+        r.output = C.pointer(op) if op is not None else None
+        r.substeps = int(substeps)
+        r.arrival = float(arrival)
+        rid = C.c_int64()
+        _check(lib().bm_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._keep[rid.value] = (hp, arrs, op, out, body is not None)
+        return rid.value
+
+    def poll(self, capacity=64):
+        """One scheduling round; returns the ids finished since the last report."""
+        ids = (C.c_int64 * capacity)()
+        n = C.c_int32()
+        _check(lib().bm_server_poll(self.h, ids, int(capacity), C.byref(n)))
+        return [ids[i] for i in range(n.value)]
+
+    def info(self, rid) -> RequestInfo:
+        i = RequestInfoC()
+        _check(lib().bm_server_info(self.h, int(rid), C.byref(i)))
+        has_body = self._keep.get(int(rid), (None,) * 5)[4]
+        done = i.state == REQ_DONE
+        return RequestInfo(i.state, i.status, i.substeps_done, i.arrival, i.admitted, i.finished,
+                           Body.from_c(i.body) if (has_body and done) else None,
+                           (np.array(i.reaction.force[:]), np.array(i.reaction.torque[:]), int(i.reaction.nodes_inside))
+                           if (has_body and done) else None)
+
+    def result(self, rid):
+        """The output particles of a finished request (insertion order)."""
+        keep = self._keep[int(rid)]
+        return keep[3]
+
+    def release(self, rid):
+        """Forget a finished request's buffers on the Python side."""
+        self._keep.pop(int(rid), None)
+
+    def cancel(self, rid):
+        _check(lib().bm_server_cancel(self.h, int(rid)))
+
+    def load(self):
+        """(queued, running, bytes in use, budget)."""
+        q, r, u, b = C.c_int32(), C.c_int32(), C.c_int64(), C.c_int64()
+        _check(lib().bm_server_load(self.h, C.byref(q), C.byref(r), C.byref(u), C.byref(b)))
+        return q.value, r.value, u.value, b.value

@@ -315,8 +315,14 @@ int bm_batch_remove_scene(bm_batch_handle b, int32_t scene_id);
 /* k coupled substeps of every scene (p2g, grid update, the scene's body
  * coupling, g2p). */
 int bm_batch_step(bm_batch_handle b, float dt, int32_t k);
+/* Errors are kept per scene: a scene whose particles left its box (or whose
+ * body outgrew the coupler's scan bound) fails its own download / get_body
+ * with the error, the other scenes step on.  bm_batch_synchronize reports the
+ * first failed scene. */
 int bm_batch_download_scene(bm_batch_handle b, int32_t scene_id, bm_particles* host);
 int bm_batch_get_body(bm_batch_handle b, int32_t scene_id, bm_body* body, bm_reaction* reaction);
+/* The scene's error status: BM_OK, or the code its download would return. */
+int bm_batch_scene_error(bm_batch_handle b, int32_t scene_id, int32_t* code);
 /* Scenes, particles and grid blocks in use. */
 int bm_batch_usage(bm_batch_handle b, int32_t* scenes, int64_t* particles, int64_t* blocks);
 /* Launch parameters: unit_particles and sort_every as bm_mpm_set_tuning;
@@ -332,6 +338,72 @@ int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* un
                       int32_t* deposit_groups);
 /* Capture the substeps in a HIP graph and replay it (on != 0). */
 int bm_batch_set_graph(bm_batch_handle b, int32_t on);
+/* Device bytes a batch of these capacities allocates (linear in each). */
+int bm_batch_footprint(int64_t particles, int64_t blocks, int32_t scenes, int64_t* bytes);
+
+/* ---------------------------------------------------------------- serving */
+/* A request loop over one batch (no counterpart in the reference, which has
+ * no serving).  A request is a scene, its particles, an optional body and a
+ * substep count, with an arrival time.  Requests whose arrival has come are
+ * admitted while the batch has particle, block and scene room and the HBM
+ * budget allows (a later request may pass an older one that does not fit,
+ * at most max_bypass times), advance together with everything else in the
+ * batch, and are retired one by one when their own substeps are done --
+ * continuous batching: admissions and retirements happen between chunks of
+ * at most max_chunk substeps, never overshooting a request's count.  Every
+ * request runs with the server's dt.  The caller keeps a request's input
+ * buffers until it is admitted and its output buffers until it is done.
+ * A scene that raises a device error (a particle leaving its box) retires as
+ * BM_REQ_FAILED with that status; the others run on. */
+typedef struct bm_server* bm_server_handle;
+
+typedef struct {
+    bm_batch_desc batch;  /* the arenas, allocated once */
+    int64_t hbm_budget;   /* bytes the running requests may hold (bm_batch_footprint of each
+                             request's particles, blocks and scene); 0 = the arenas' */
+    float dt;
+    int32_t max_chunk;    /* most substeps between scheduling points (0 = 32) */
+    int32_t max_bypass;   /* admissions past the oldest waiting request (0 = 8) */
+} bm_server_desc;
+
+typedef struct {
+    bm_scene_desc scene;
+    int64_t n;
+    const bm_particles* input;  /* read at admission */
+    const bm_body* body;        /* NULL = none; copied at submission */
+    bm_particles* output;       /* written at retirement; NULL = not copied back */
+    int32_t substeps;
+    double arrival;             /* seconds on the server clock; < 0 = now */
+} bm_request;
+
+enum bm_request_state { BM_REQ_QUEUED = 0, BM_REQ_RUNNING = 1, BM_REQ_DONE = 2, BM_REQ_FAILED = 3, BM_REQ_CANCELLED = 4 };
+
+typedef struct {
+    int32_t state;              /* bm_request_state */
+    int32_t status;             /* BM_OK, or the error of a failed request */
+    int32_t substeps_done;
+    int32_t pad;
+    double arrival, admitted, finished;  /* server clock, seconds; -1 = not yet */
+    bm_body body;               /* the body at retirement (requests with a body) */
+    bm_reaction reaction;
+} bm_request_info;
+
+int bm_server_create(const bm_server_desc* desc, bm_server_handle* out);
+int bm_server_destroy(bm_server_handle s);
+/* Seconds since the server was created (the clock of arrivals and timings). */
+int bm_server_clock(bm_server_handle s, double* seconds);
+/* The server's batch, for tuning (bm_batch_set_tuning / _autotune / _set_graph). */
+int bm_server_batch(bm_server_handle s, bm_batch_handle* batch);
+/* Queue a request.  BM_ERR_OOM when it could not fit even an empty server. */
+int bm_server_submit(bm_server_handle s, const bm_request* req, int64_t* request_id);
+/* One scheduling round: admit, advance one chunk, retire.  Sleeps until the
+ * next arrival when nothing is running.  Writes up to `capacity` ids of
+ * requests finished (done or failed) since the last report. */
+int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int32_t* n_done);
+int bm_server_info(bm_server_handle s, int64_t request_id, bm_request_info* info);
+/* Drop a queued or running request (its output is not written). */
+int bm_server_cancel(bm_server_handle s, int64_t request_id);
+int bm_server_load(bm_server_handle s, int32_t* queued, int32_t* running, int64_t* bytes_in_use, int64_t* budget);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,9 @@ int main(void) {
   S(bm_body); F(bm_body, inv_mass); F(bm_body, integrate); F(bm_body, applied_torque); F(bm_body, shape);
   S(bm_batch_desc); F(bm_batch_desc, max_scenes); F(bm_batch_desc, stream);
   S(bm_scene_desc); F(bm_scene_desc, young_modulus); F(bm_scene_desc, gravity); F(bm_scene_desc, coupler);
+  S(bm_server_desc); F(bm_server_desc, hbm_budget); F(bm_server_desc, dt); F(bm_server_desc, max_bypass);
+  S(bm_request); F(bm_request, n); F(bm_request, output); F(bm_request, substeps); F(bm_request, arrival);
+  S(bm_request_info); F(bm_request_info, arrival); F(bm_request_info, body); F(bm_request_info, reaction);
   return 0;
 }
 """
@@ -102,9 +105,35 @@ def test_struct_layouts_match_the_python_mirror(tmp_path):
             want[(t, m)] = int(v)
     py = {"bm_mpm_desc": mpm.Desc, "bm_particles": mpm.HostParticles, "bm_particles_device": mpm.DeviceParticles,
           "bm_collider": mpm.ColliderC, "bm_reaction": mpm.ReactionC, "bm_body": mpm.BodyC,
-          "bm_batch_desc": mpm.BatchDescC, "bm_scene_desc": mpm.SceneDescC}
+          "bm_batch_desc": mpm.BatchDescC, "bm_scene_desc": mpm.SceneDescC, "bm_server_desc": mpm.ServerDescC,
+          "bm_request": mpm.RequestC, "bm_request_info": mpm.RequestInfoC}
     for (t, m), v in want.items():
         if m == "sizeof":
             assert C.sizeof(py[t]) == v, (t, C.sizeof(py[t]), v)
         else:
             assert getattr(py[t], m).offset == v, (t, m, getattr(py[t], m).offset, v)
+
+
+def test_batch_footprint_is_linear():
+    from basements_amd import batch_footprint
+
+    _lib_or_skip()
+    z = batch_footprint(0, 0, 0)
+    p, b, s = batch_footprint(1000, 0, 0) - z, batch_footprint(0, 1000, 0) - z, batch_footprint(0, 0, 10) - z
+    assert batch_footprint(1000, 1000, 10) == z + p + b + s
+    # particle state alone is 2 x 26 floats; a block's two grids 2 x 64 x 24 bytes
+    assert p >= 1000 * 208 and b >= 1000 * 3072 and s > 0
+
+
+def test_server_argument_errors_without_device():
+    from basements_amd.mpm import ServerDescC
+
+    L = _lib_or_skip()
+    h = C.c_void_p()
+    assert L.bm_server_create(None, C.byref(h)) == 1
+    d = ServerDescC()
+    d.dt = 0.0
+    assert L.bm_server_create(C.byref(d), C.byref(h)) == 1
+    assert b"dt" in L.bm_last_error()
+    assert L.bm_server_submit(None, None, None) == 1
+    assert L.bm_server_destroy(None) == 0
