@@ -20,6 +20,7 @@
 #include <cstring>
 #include <vector>
 
+#include "host_staging.hpp"
 #include "mpm_internal.hpp"
 
 using namespace bm;
@@ -100,6 +101,7 @@ struct bm_batch {
     float dt_prepared = 0.0f;
     int since_bin = 0;
     int unit_p = 512, sort_every = 16, deposit_groups = 4096;
+    PinnedStage stage;  // host side of uploads and downloads
 
     // Captured substeps.  A replay starts and ends in the state it was
     // captured in (which particle buffer is current, which grid, re-binning
@@ -469,22 +471,12 @@ int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, co
     S.id = b->next_id++;
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (n > 0) {
-        // component-major staging, then one pitched copy into the arena
-        std::vector<float> st((size_t)(PCOMP * n));
-        for (int64_t i = 0; i < n; ++i) {
-            for (int k = 0; k < 3; ++k) {
-                st[(size_t)((PX + k) * n + i)] = p->x[3 * i + k];
-                st[(size_t)((PV + k) * n + i)] = p->v[3 * i + k];
-            }
-            for (int k = 0; k < 9; ++k) {
-                st[(size_t)((PF + k) * n + i)] = p->F[9 * i + k];
-                st[(size_t)((PC + k) * n + i)] = p->C[9 * i + k];
-            }
-            st[(size_t)(PM * n + i)] = p->m[i];
-            st[(size_t)(PV0 * n + i)] = p->V0[i];
-        }
-        HIP_TRY(hipMemcpy2D(b->pbuf + S.p0, sizeof(float) * b->capP, st.data(), sizeof(float) * n, sizeof(float) * n,
-                            PCOMP, hipMemcpyHostToDevice));
+        // component-major pinned staging, then one pitched copy into the arena
+        if (!b->stage.ensure(sizeof(float) * PCOMP * n)) return fail(BM_ERR_OOM, "pinned staging");
+        float* st = b->stage.f();
+        soa_from_particles(p->x, p->v, p->F, p->C, p->m, p->V0, n, st);
+        HIP_TRY(hipMemcpy2D(b->pbuf + S.p0, sizeof(float) * b->capP, st, sizeof(float) * n, sizeof(float) * n, PCOMP,
+                            hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_iota_from, dim3(grid256(n)), dim3(256), 0, b->stream, b->pid + S.p0, n, S.p0);
         HIP_TRY(hipGetLastError());
     }
@@ -595,25 +587,14 @@ int bm_batch_download_scene(bm_batch_handle b, int32_t id, bm_particles* p) {
     const SceneHost& S = b->scenes[s];
     const int64_t n = S.n;
     if (n == 0) return BM_OK;
-    std::vector<float> st((size_t)(PCOMP * n));
-    std::vector<int> order((size_t)n);
-    HIP_TRY(hipMemcpy2D(st.data(), sizeof(float) * n, b->pbuf + S.p0, sizeof(float) * b->capP, sizeof(float) * n,
-                        PCOMP, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(order.data(), b->pid + S.p0, sizeof(int) * n, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; ++i) {
-        const int64_t o = (int64_t)order[(size_t)i] - S.p0;  // insertion index within the scene
-        if (o < 0 || o >= n) return fail(BM_ERR_HIP, "particle order corrupted");
-        for (int k = 0; k < 3; ++k) {
-            if (p->x) p->x[3 * o + k] = st[(size_t)((PX + k) * n + i)];
-            if (p->v) p->v[3 * o + k] = st[(size_t)((PV + k) * n + i)];
-        }
-        for (int k = 0; k < 9; ++k) {
-            if (p->F) p->F[9 * o + k] = st[(size_t)((PF + k) * n + i)];
-            if (p->C) p->C[9 * o + k] = st[(size_t)((PC + k) * n + i)];
-        }
-        if (p->m) p->m[o] = st[(size_t)(PM * n + i)];
-        if (p->V0) p->V0[o] = st[(size_t)(PV0 * n + i)];
-    }
+    if (!b->stage.ensure((sizeof(float) * PCOMP + sizeof(int)) * n)) return fail(BM_ERR_OOM, "pinned staging");
+    float* st = b->stage.f();
+    int* order = reinterpret_cast<int*>(st + PCOMP * n);
+    HIP_TRY(hipMemcpy2D(st, sizeof(float) * n, b->pbuf + S.p0, sizeof(float) * b->capP, sizeof(float) * n, PCOMP,
+                        hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(order, b->pid + S.p0, sizeof(int) * n, hipMemcpyDeviceToHost));
+    // slot i holds the scene's particle order[i] - p0 (insertion index)
+    if (!particles_from_soa(st, n, order, S.p0, p)) return fail(BM_ERR_HIP, "particle order corrupted");
     return BM_OK;
 }
 

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "host_staging.hpp"
 #include "mpm_internal.hpp"
 
 using namespace bm;
@@ -42,6 +43,7 @@ struct bm_mpm_solver {
 
     float* pbuf = nullptr;  // 26 * cap floats
     int64_t cap = 0, n = 0;
+    PinnedStage stage;      // host side of uploads and downloads
 
     GridBuf grid[2] = {};
     int cur = 0;
@@ -385,26 +387,17 @@ int bm_mpm_set_walls(bm_mpm_handle h, int32_t thickness) {
 }
 
 // Copy host SoA (3n/9n interleaved per particle) into component-major device arrays.
+// Particles [off, off + n) from the caller's layout: component-major pinned
+// staging (host threads), then one pitched copy of all components.
 static int upload_range(bm_mpm_handle h, int64_t off, int64_t n, const float* x, const float* v, const float* F,
                         const float* Cm, const float* m, const float* V0) {
-    std::vector<float> tmp((size_t)n);
-    auto put = [&](int comp, const float* src, int stride, int k) -> int {
-        for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = src[i * stride + k];
-        HIP_TRY(hipMemcpyAsync(h->pbuf + comp * h->cap + off, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice,
-                               h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        return BM_OK;
-    };
-    for (int k = 0; k < 3; ++k)
-        if (int rc = put(PX + k, x, 3, k)) return rc;
-    for (int k = 0; k < 3; ++k)
-        if (int rc = put(PV + k, v, 3, k)) return rc;
-    for (int k = 0; k < 9; ++k)
-        if (int rc = put(PF + k, F, 9, k)) return rc;
-    for (int k = 0; k < 9; ++k)
-        if (int rc = put(PC + k, Cm, 9, k)) return rc;
-    if (int rc = put(PM, m, 1, 0)) return rc;
-    if (int rc = put(PV0, V0, 1, 0)) return rc;
+    if (!h->stage.ensure(sizeof(float) * PCOMP * n)) return fail(BM_ERR_OOM, "pinned staging");
+    float* st = h->stage.f();
+    soa_from_particles(x, v, F, Cm, m, V0, n, st);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy2DAsync(h->pbuf + off, sizeof(float) * h->cap, st, sizeof(float) * n, sizeof(float) * n, PCOMP,
+                             hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return BM_OK;
 }
 
@@ -462,33 +455,14 @@ int bm_mpm_download_particles(bm_mpm_handle h, bm_particles* p) {
     if (int rc = sync_and_check(h)) return rc;
     const int64_t n = h->n;
     if (n == 0) return BM_OK;
-    std::vector<float> tmp((size_t)n);
+    if (!h->stage.ensure((sizeof(float) * PCOMP + sizeof(int)) * n)) return fail(BM_ERR_OOM, "pinned staging");
+    float* st = h->stage.f();
+    int* order = reinterpret_cast<int*>(st + PCOMP * n);
+    HIP_TRY(hipMemcpy2D(st, sizeof(float) * n, h->pbuf, sizeof(float) * h->cap, sizeof(float) * n, PCOMP,
+                        hipMemcpyDeviceToHost));
     // fast mode stores particles binned by block: slot i holds particle pid[i]
-    std::vector<int> order;
-    if (h->fast) {
-        order.resize((size_t)n);
-        HIP_TRY(hipMemcpy(order.data(), h->pid, sizeof(int) * n, hipMemcpyDeviceToHost));
-    }
-    auto get = [&](int comp, float* dst, int stride, int k) -> int {
-        if (!dst) return BM_OK;
-        HIP_TRY(hipMemcpy(tmp.data(), h->pbuf + comp * h->cap, sizeof(float) * n, hipMemcpyDeviceToHost));
-        if (h->fast) {
-            for (int64_t i = 0; i < n; ++i) dst[(int64_t)order[(size_t)i] * stride + k] = tmp[(size_t)i];
-        } else {
-            for (int64_t i = 0; i < n; ++i) dst[i * stride + k] = tmp[(size_t)i];
-        }
-        return BM_OK;
-    };
-    for (int k = 0; k < 3; ++k)
-        if (int rc = get(PX + k, p->x, 3, k)) return rc;
-    for (int k = 0; k < 3; ++k)
-        if (int rc = get(PV + k, p->v, 3, k)) return rc;
-    for (int k = 0; k < 9; ++k)
-        if (int rc = get(PF + k, p->F, 9, k)) return rc;
-    for (int k = 0; k < 9; ++k)
-        if (int rc = get(PC + k, p->C, 9, k)) return rc;
-    if (int rc = get(PM, p->m, 1, 0)) return rc;
-    if (int rc = get(PV0, p->V0, 1, 0)) return rc;
+    if (h->fast) HIP_TRY(hipMemcpy(order, h->pid, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (!particles_from_soa(st, n, h->fast ? order : nullptr, 0, p)) return fail(BM_ERR_HIP, "particle order corrupted");
     return BM_OK;
 }
 

@@ -66,6 +66,10 @@ struct bm_server {
     int64_t budget = 0, in_use = 0;
     int bypassed = 0;              // admissions past the current oldest waiting request
     int64_t oldest = 0;
+    // seconds spent per activity: admission, stepping (launch to device
+    // idle), retirement, waiting for arrivals; chunks run; substeps run
+    double t_admit = 0.0, t_step = 0.0, t_retire = 0.0, t_idle = 0.0;
+    int64_t chunks = 0, substeps = 0;
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -226,22 +230,34 @@ int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int3
     if (int rc = check_server(s)) return rc;
     if (capacity < 0 || (capacity > 0 && !done_ids)) return fail(BM_ERR_INVALID_ARG, "bad done-id buffer");
     if (s->done.empty()) {
+        double t = s->now();
         if (int rc = admit(s)) return rc;
         if (s->running.empty() && !s->queue.empty()) {
             // idle: wait for the next arrival, then admit it
             const double wait = s->reqs[s->queue.front()].info.arrival - s->now();
+            const double t1 = s->now();
             if (wait > 0.0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            s->t_idle += s->now() - t1;
             if (int rc = admit(s)) return rc;
         }
+        s->t_admit += s->now() - t;
         if (!s->running.empty()) {
             int32_t k = s->desc.max_chunk;
             for (int64_t id : s->running) k = std::min(k, s->reqs[id].left);
+            t = s->now();
             if (int rc = bm_batch_step(s->batch, s->desc.dt, k)) return rc;
+            int32_t code = 0;  // waits for the device (per-scene errors are read at retirement)
+            if (int rc = bm_batch_scene_error(s->batch, s->reqs[s->running.front()].scene, &code)) return rc;
+            s->t_step += s->now() - t;
+            ++s->chunks;
+            s->substeps += k;
             for (int64_t id : s->running) {
                 s->reqs[id].left -= k;
                 s->reqs[id].info.substeps_done += k;
             }
+            t = s->now();
             if (int rc = retire(s)) return rc;
+            s->t_retire += s->now() - t;
         }
     }
     int32_t n = 0;
@@ -279,6 +295,21 @@ int bm_server_cancel(bm_server_handle s, int64_t id) {
     }
     R.info.state = BM_REQ_CANCELLED;
     R.info.finished = s->now();
+    return BM_OK;
+}
+
+int bm_server_stats(bm_server_handle s, double* seconds, int64_t* counts) {
+    if (int rc = check_server(s)) return rc;
+    if (seconds) {
+        seconds[0] = s->t_admit - s->t_idle;
+        seconds[1] = s->t_step;
+        seconds[2] = s->t_retire;
+        seconds[3] = s->t_idle;
+    }
+    if (counts) {
+        counts[0] = s->chunks;
+        counts[1] = s->substeps;
+    }
     return BM_OK;
 }
 

@@ -241,6 +241,7 @@ _SIGS = {
     "bm_server_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]),
     "bm_server_info": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestInfoC)]),
     "bm_server_cancel": (C.c_int, [C.c_void_p, C.c_int64]),
+    "bm_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "bm_server_load": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64)]),
     "bm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -839,6 +840,12 @@ class Server:
 
     def cancel(self, rid):
         _check(lib().bm_server_cancel(self.h, int(rid)))
+
+    def stats(self):
+        """Seconds admitting / stepping / retiring / idle, chunks and substeps run."""
+        t, c = (C.c_double * 4)(), (C.c_int64 * 2)()
+        _check(lib().bm_server_stats(self.h, t, c))
+        return {"admit_s": t[0], "step_s": t[1], "retire_s": t[2], "idle_s": t[3], "chunks": c[0], "substeps": c[1]}
 
     def load(self):
         """(queued, running, bytes in use, budget)."""

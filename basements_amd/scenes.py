@@ -75,6 +75,7 @@ class Scene:
     dt: float = 1e-4
     substeps: int = 1000
     tool: dict = field(default_factory=dict)
+    walls: int = 0  # slip-wall thickness in nodes on the box faces (0: floor only)
 
     @property
     def n(self):
@@ -125,13 +126,14 @@ def wheel_bed(seed=0, friction_deg=None, cells=(120, 36, 120), scale=1):
                  substeps=2000)
 
 
-def tool_scene(seed, n_target, grid_edge=None, name="tool"):
+def tool_scene(seed, n_target, grid_edge=None, name="tool", depth=None):
     """One soil-tool scene of the batched workload: a soil bed of about
     n_target particles (8 per cell, width = 2 x height) in a cubic box of
     grid_edge nodes (32..128; the smallest that holds the bed when None), a box
     or cylinder tool sunk into the bed top and dragged along x at a speed
     from U(0.05, 0.5) m/s (kinematic), friction angle from U(25, 40) deg,
-    E log-uniform in [1e5, 1e6] Pa.  dt = 1e-4 s."""
+    E log-uniform in [1e5, 1e6] Pa.  dt = 1e-4 s.  `depth`: how far the
+    tool's bottom sits below the bed top (default two cells)."""
     r = _rng(1_000_003 + seed)
     cells = max(8.0, n_target / 8.0)
     H = max(3, int(round((cells / 4.0) ** (1.0 / 3.0))))
@@ -148,16 +150,17 @@ def tool_scene(seed, n_target, grid_edge=None, name="tool"):
     kind = "box" if r.random() < 0.5 else "cylinder"
     top = hi[1] * dx
     size = max(3, W // 6) * dx  # the tool spans a sixth of the bed
+    sink = 2 * dx if depth is None else float(depth)
     cx = (lo[0] + 0.25 * W) * dx
     cz = (lo[2] + 0.5 * W) * dx
     if kind == "box":
         he = (0.5 * size, 0.5 * size, 0.5 * size)
-        tool = {"kind": "box", "half": he, "center": (cx, top + 0.5 * size - 2 * dx, cz), "speed": speed,
+        tool = {"kind": "box", "half": he, "center": (cx, top + 0.5 * size - sink, cz), "speed": speed,
                 "friction": 0.5, "mass": 1.0, "method": 0}
     else:
         R = 0.5 * size
         tool = {"kind": "cylinder", "radius": R, "half_len": 0.5 * size,
-                "center": (cx, top + R - 2 * dx, cz), "speed": speed, "friction": 0.5, "load": 10.0, "method": 1}
+                "center": (cx, top + R - sink, cz), "speed": speed, "friction": 0.5, "load": 10.0, "method": 1}
     return Scene(name, dx, P, origin=(0, 0, 0), dims=(G, G, G), E=E, rho=1600.0, alpha=dp_alpha(phi), tool=tool,
                  substeps=500)
 
@@ -168,6 +171,31 @@ def batch_scenes(seed=0, count=64, n_lo=2e4, n_hi=5e5):
     r = _rng(seed)
     ns = np.exp(r.uniform(math.log(n_lo), math.log(n_hi), size=count))
     return [tool_scene(seed * 1000 + i, float(n), name=f"tool{i}") for i, n in enumerate(ns)]
+
+
+def serve_requests(seed=0, count=48, n_lo=5e4, n_hi=2e6, sub_lo=200, sub_hi=2000):
+    """The served workload's requests: soil-tool rollouts with particle
+    counts log-uniform in [n_lo, n_hi], tool depth U(0.02, 0.1) m, speed
+    U(0.05, 0.5) m/s (drawn by tool_scene) and substeps uniform in
+    {sub_lo..sub_hi}.  The bed sits in a soil bin: slip walls three nodes
+    deep on the box faces keep a slumping bed inside its box over the long
+    rollouts.  Returns [(Scene, substeps)]."""
+    r = _rng(seed + 424_242)
+    ns = np.exp(r.uniform(math.log(n_lo), math.log(n_hi), size=count))
+    depths = r.uniform(0.02, 0.1, size=count)
+    subs = r.integers(sub_lo, sub_hi + 1, size=count)
+    out = []
+    for i, (n, dp, k) in enumerate(zip(ns, depths, subs)):
+        s = tool_scene(seed * 1000 + 500 + i, float(n), name=f"req{i}", depth=float(dp))
+        s.walls = 3
+        out.append((s, int(k)))
+    return out
+
+
+def poisson_arrivals(rate, count, seed=0):
+    """Arrival times (s) of a Poisson stream of `rate` requests per second."""
+    r = _rng(seed + 77_777)
+    return np.cumsum(r.exponential(1.0 / rate, size=count))
 
 
 def tool_body(tool):

@@ -74,12 +74,16 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="wheel_bed", choices=["wheel_bed", "box_drop", "batch"],
+    ap.add_argument("--config", default="wheel_bed", choices=["wheel_bed", "box_drop", "batch", "serve"],
                     help="wheel_bed (the default workload), box_drop, or batch: 64 uneven tool scenes stepped "
                          "together against the same scenes one at a time on the single-scene path")
     ap.add_argument("--scenes", type=int, default=64, help="scenes of --config batch")
     ap.add_argument("--graph", type=int, default=1, help="--config batch: replay captured substeps")
-    ap.add_argument("--serial", type=int, default=1, help="--config batch: also time the scenes one at a time")
+    ap.add_argument("--serial", type=int, default=1, help="--config batch/serve: also time the work one at a time")
+    ap.add_argument("--requests", type=int, default=48, help="--config serve: requests per arrival rate")
+    ap.add_argument("--rates", default="5,200", help="--config serve: arrival rates (requests/s)")
+    ap.add_argument("--serve-particles", type=int, default=16_000_000, help="--config serve: particle room")
+    ap.add_argument("--serve-blocks", type=int, default=400_000, help="--config serve: grid-block room")
     ap.add_argument("--tuning", default="", help="--config batch: unit,sort_every,deposit_groups instead of autotune")
     ap.add_argument("--mode", default="fast", choices=["reference", "fast"])
     ap.add_argument("--seed", type=int, default=0)
@@ -170,6 +174,102 @@ def measured_traffic(workload, mode, kernel):
     return {"bytes": b, "source": TRAFFIC_FILE[len(ROOT) + 1:]}
 
 
+def _scene_desc(s):
+    from basements_amd import M0, M1, SceneDesc
+
+    return SceneDesc(dx=s.dx, origin=s.origin, dims=s.dims, E=s.E, nu=s.nu, rho=s.rho, alpha=s.alpha,
+                     mu_floor=s.mu_floor, gravity=s.gravity, walls=s.walls,
+                     coupler=M1 if s.tool["method"] == 1 else M0)
+
+
+def _percentile(xs, q):
+    return float(np.percentile(np.asarray(xs, np.float64), q)) if len(xs) else None
+
+
+def bench_serve(a):
+    """Served soil-tool rollouts: a Poisson stream of requests (particle
+    counts log-uniform in [5e4, 2e6], tool depth U(0.02, 0.1) m, speed
+    U(0.05, 0.5) m/s, substeps uniform in {200..2000}) through the request
+    loop at each arrival rate; per rate the p50 / p99 latency (arrival to
+    result in host memory) and the aggregate particle-updates/s over the
+    makespan (first arrival to last result).  Against the same requests run
+    one at a time on single-scene handles (upload, coupled substeps,
+    download)."""
+    from basements_amd import MODE_FAST, M0, M1, REQ_DONE, Server, Solver
+    from basements_amd.scenes import poisson_arrivals, serve_requests, tool_body
+
+    reqs = serve_requests(a.seed, a.requests)
+    dt = 1e-4
+    total_pu = float(sum(s.n * k for s, k in reqs))
+    cap_p, cap_b, cap_s = a.serve_particles, a.serve_blocks, 64
+
+    def run(rate):
+        S = Server(max_particles=cap_p, max_blocks=cap_b, max_scenes=cap_s, dt=dt)
+        # warm-up: two small requests through the same loop
+        w = [S.submit(_scene_desc(s), s.particles, 16, body=tool_body(s.tool)) for s, _ in reqs[:2]]
+        while any(S.info(r).state != REQ_DONE for r in w):
+            S.poll()
+        arr = poisson_arrivals(rate, len(reqs), a.seed)
+        t0 = S.clock() + 0.01
+        ids = [S.submit(_scene_desc(s), s.particles, k, body=tool_body(s.tool), arrival=t0 + float(t))
+               for (s, k), t in zip(reqs, arr)]
+        left = set(ids)
+        most = 0
+        while left:
+            for rid in S.poll():
+                left.discard(rid)
+            most = max(most, S.load()[1])
+        infos = [S.info(r) for r in ids]
+        bad = [(r, i.state, i.status, i.substeps_done) for r, i in zip(ids, infos) if i.state != REQ_DONE]
+        if bad:
+            raise RuntimeError(f"requests (id, state, status, substeps done) {bad} did not complete")
+        lat = [i.finished - i.arrival for i in infos]
+        makespan = max(i.finished for i in infos) - min(i.arrival for i in infos)
+        st = S.stats()
+        S.close()
+        return {"rate": rate, "p50_ms": _percentile(lat, 50) * 1e3, "p99_ms": _percentile(lat, 99) * 1e3,
+                "max_ms": max(lat) * 1e3, "makespan_s": makespan, "value": total_pu / makespan,
+                "most_running": most, "server_time": st}
+
+    per_rate = [run(float(r)) for r in a.rates.split(",")]
+    top = max(per_rate, key=lambda r: r["rate"])
+
+    # the same requests one at a time on the plain path
+    t_serial = 0.0
+    if a.serial:
+        for s, k in reqs:
+            S = Solver(s.dx, s.origin, s.dims, s.n, MODE_FAST)
+            S.set_material(s.E, s.nu, s.rho)
+            S.set_plastic_alpha(s.alpha)
+            S.set_floor_friction(s.mu_floor)
+            S.set_gravity(s.gravity)
+            S.set_walls(s.walls)
+            S.synchronize()
+            t0 = time.perf_counter()
+            S.upload(s.particles)
+            S.set_body(tool_body(s.tool))
+            S.step_coupled(M1 if s.tool["method"] == 1 else M0, dt, k)
+            S.download()
+            t_serial += time.perf_counter() - t0
+            S.close()
+    serial = total_pu / t_serial if t_serial > 0 else None
+    out = {"metric": "particle-updates/s (served requests, aggregate)", "value": top["value"],
+           "unit": "particle-updates/s", "n_gpus": 1, "steps": len(reqs), "warmup": 2,
+           "ms_per_step": top["makespan_s"] / len(reqs) * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic (seeded soil-tool requests, Poisson arrivals, no external data)",
+           "config": {"workload": "serve", "requests": len(reqs), "particles": int(sum(s.n for s, _ in reqs)),
+                      "particle_updates": total_pu, "particles_min": min(s.n for s, _ in reqs),
+                      "particles_max": max(s.n for s, _ in reqs), "substeps_min": min(k for _, k in reqs),
+                      "substeps_max": max(k for _, k in reqs), "dt": dt, "server_particles": cap_p,
+                      "server_blocks": cap_b, "server_scenes": cap_s},
+           "rates": per_rate, "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
+           "serial_value": serial, "serial_s": t_serial if serial else None,
+           "speedup_vs_serial": top["value"] / serial if serial else None}
+    print(json.dumps(out))
+    return 0
+
+
 def bench_batch(a):
     """Batched uneven scenes: aggregate particle-updates/s of all scenes in one
     batch (autotuned, graph-replayed) against running the same scenes one at a
@@ -250,6 +350,8 @@ def main():
         return 0
     if a.config == "batch":
         return bench_batch(a)
+    if a.config == "serve":
+        return bench_serve(a)
 
     from basements_amd import M0, M1, MODE_FAST, MODE_REFERENCE, Solver
     from basements_amd.scenes import box_drop, wheel_bed

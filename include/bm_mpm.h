@@ -404,6 +404,10 @@ int bm_server_info(bm_server_handle s, int64_t request_id, bm_request_info* info
 /* Drop a queued or running request (its output is not written). */
 int bm_server_cancel(bm_server_handle s, int64_t request_id);
 int bm_server_load(bm_server_handle s, int32_t* queued, int32_t* running, int64_t* bytes_in_use, int64_t* budget);
+/* Where the time went: seconds[4] = admitting, stepping (launch until the
+ * device is idle), retiring, sleeping for arrivals; counts[2] = chunks and
+ * substeps run.  Either may be NULL. */
+int bm_server_stats(bm_server_handle s, double* seconds, int64_t* counts);
 
 #ifdef __cplusplus
 }
