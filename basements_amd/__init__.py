@@ -5,9 +5,11 @@ basements_amd.build); this package holds its Python host mirror.
 """
 from .mpm import (BODY_DYNAMIC, BODY_KINEMATIC, BODY_STATIC, M0, M1, MODE_FAST, MODE_REFERENCE, REQ_CANCELLED,
                   REQ_DONE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING, SHAPE_BOX, SHAPE_CYLINDER, Batch, BMError, Body,
-                  Collider, MPMSolver, RequestInfo, SceneDesc, Server, Solver, batch_footprint, header_symbols, lib)
+                  Collider, DiffParams, MPMSolver, RequestInfo, Rollout, SceneDesc, Server, Solver, batch_footprint,
+                  header_symbols, lib)
 
 __all__ = ["BODY_DYNAMIC", "BODY_KINEMATIC", "BODY_STATIC", "M0", "M1", "MODE_FAST", "MODE_REFERENCE",
            "REQ_CANCELLED", "REQ_DONE", "REQ_FAILED", "REQ_QUEUED", "REQ_RUNNING", "SHAPE_BOX", "SHAPE_CYLINDER",
-           "Batch", "BMError", "Body", "Collider", "MPMSolver", "RequestInfo", "SceneDesc", "Server", "Solver",
+           "Batch", "BMError", "Body", "Collider", "DiffParams", "MPMSolver", "RequestInfo", "Rollout", "SceneDesc",
+           "Server", "Solver",
            "batch_footprint", "header_symbols", "lib"]

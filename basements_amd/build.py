@@ -40,6 +40,7 @@ UNITS = [
     ("bm_mpm_api.hip", ["-ffp-contract=off"]),
     ("bm_batch.hip", []),
     ("bm_serve.hip", []),
+    ("bm_diff.hip", []),
 ]
 
 

@@ -1,0 +1,1106 @@
+// bm_diff.hip -- differentiable rollout of the soil substep: the reference
+// algorithm's P2G / grid update / M0 tool / G2P with plasticity, in FP32 or
+// FP64, and its hand-written adjoint with checkpoint-and-recompute.
+//
+// The reference has no backward pass (its autodiff tape records nothing,
+// include/basements/physics/mpm/mpm_autodiff_tape.h:63-92); this follows the
+// forward algorithm of mpm_solver.h:114-348 and the M0 clamp of
+// src/physics/coupling/mpm_rigid_coupler.cpp:95-138 and differentiates it by
+// hand.  The loss is the mean squared difference between each particle's
+// final displacement and a target displacement; its gradient is taken with
+// respect to the tool velocity, the Drucker-Prager alpha (friction angle),
+// the Lame parameters (Young's modulus) and the cone factor k.
+//
+// Forward form (per substep, particle state s = x, v, C, F; m, V0 fixed):
+//   P2G   m_i = sum_p w m,  q_i = sum_p w (m (v + C d) + dt K d),
+//         K = -(4/dx^2) V0 tau(F),  tau = U diag(2 mu ln s + lam tr ln s) U^T,
+//         d = x_i - x_p (node i at i*dx)
+//   grid  u = q/m + g dt (m > 1e-6), floor Coulomb rule on the bottom layer,
+//         M0 clamp on nodes inside the tool (m > 1e-8); lighter nodes keep q
+//   G2P   v' = sum w u,  C' = (4/dx^2) sum w u d^T,  x' = x + dt v',
+//         F' = proj((I + dt C') F), floor clamp of x'.y
+// The dense node box replaces the reference's unbounded grid; the per-node
+// mass is a plain float sum (the product path's fixed-point mass is not
+// differentiable); the SVD is the same cyclic Jacobi run to a tighter
+// tolerance so that U and V are accurate enough to differentiate.
+//
+// Adjoint of a function G(F) = U diag(h(sigma)) V^T (both the plastic
+// projection and P(F) = tau F^-T = U diag(tau_k / sigma_k) V^T have this
+// form): with M = U^T Gbar V,
+//   Fbar = U K V^T,  K_kk = sum_i M_ii dh_i/dsigma_k,
+//   K_ij = M_ij (D + S)/2 + M_ji (D - S)/2   (i != j),
+//   D = (h_j - h_i) / (sigma_j - sigma_i),  S = (h_j + h_i) / (sigma_j + sigma_i),
+// with D taken as its limit dh_j/dsigma_j - dh_i/dsigma_j when the two
+// singular values coincide (the first-order SVD perturbation of
+// Papadopoulo & Lourakis 2000, rearranged so that nothing divides by
+// sigma_j^2 - sigma_i^2).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mpm_device.hpp"
+#include "mpm_internal.hpp"
+
+namespace bm {
+namespace dif {
+
+enum { QX = 0, QV = 3, QC = 6, QF = 15, QCOMP = 24 };
+
+template <class R> struct PS {
+    R* p;
+    int64_t cap;
+    __device__ R& at(int c, int64_t i) const { return p[c * cap + i]; }
+};
+
+struct Box {
+    int ox, oy, oz, nx, ny, nz;
+};
+
+// Parameters of one substep (the tool centre moves with the substep).
+template <class R> struct Par {
+    R dx, inv_dx, mu, lam, alpha, koh, mu_floor, g[3];
+    int ref_svd;  // the reference's SVD stopping rule (forward comparisons only)
+    int tool;
+    R tc[3], he[3], Rl[9], Rw[9], vt[3], friction;
+};
+
+// Gradient accumulators (double, device): tool velocity (3), alpha, koh, mu, lam.
+enum { GVT = 0, GALPHA = 3, GKOH = 4, GMU = 5, GLAM = 6, GLOSS = 7, GN = 8 };
+
+template <class R> __device__ __forceinline__ int64_t node_index(const Box& b, int i, int j, int k) {
+    const int a = i - b.ox, c = j - b.oy, e = k - b.oz;
+    if (a < 0 || c < 0 || e < 0 || a >= b.nx || c >= b.ny || e >= b.nz) return -1;
+    return ((int64_t)a * b.ny + c) * b.nz + e;
+}
+
+template <class R> struct St {
+    int b[3];
+    R w[3][3], dw[3][3];  // weights and their derivatives along each axis (per unit x)
+};
+template <class R> __device__ __forceinline__ void stencil(const R x[3], R inv_dx, St<R>& s) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const R f = x[a] * inv_dx - R(0.5);
+        s.b[a] = (int)floor(f);
+        const R fr = f - (R)s.b[a];
+        s.w[a][0] = R(0.5) * (R(1) - fr) * (R(1) - fr);
+        s.w[a][1] = R(0.75) - (fr - R(0.5)) * (fr - R(0.5));
+        s.w[a][2] = R(0.5) * fr * fr;
+        s.dw[a][0] = -(R(1) - fr) * inv_dx;
+        s.dw[a][1] = R(-2) * (fr - R(0.5)) * inv_dx;
+        s.dw[a][2] = fr * inv_dx;
+    }
+}
+
+template <class R> __device__ __forceinline__ m3<R> load_m(const PS<R>& s, int c0, int64_t i) {
+    m3<R> m;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) m.a[c] = s.at(c0 + c, i);
+    return m;
+}
+template <class R> __device__ __forceinline__ void store_m(const PS<R>& s, int c0, int64_t i, const m3<R>& m) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) s.at(c0 + c, i) = m.a[c];
+}
+template <class R> __device__ __forceinline__ m3<R> transpose(const m3<R>& a) {
+    m3<R> r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.a[i * 3 + j] = a.a[j * 3 + i];
+    return r;
+}
+template <class R> __device__ __forceinline__ m3<R> mmt(const m3<R>& p, const m3<R>& q) {  // p q^T
+    return mmul(p, transpose(q));
+}
+
+// SVD accurate enough to differentiate: the reference's cyclic Jacobi with a
+// relative stopping test (and more sweeps in FP64); with ref, the
+// reference's own constants (skip 1e-6, stop 1e-5, 5 sweeps; svd.h:24-107).
+template <class R> __device__ __forceinline__ void svd_d(const m3<R>& F, m3<R>& U, v3<R>& S, m3<R>& V, int ref) {
+    if (ref) {
+        jacobi_svd<R, 5>(F, U, S, V, R(1e-6), R(1e-5));
+        return;
+    }
+    R fro = R(0);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) fro += F.a[c] * F.a[c];
+    if (sizeof(R) == 8)
+        jacobi_svd<R, 12>(F, U, S, V, R(1e-300), R(1e-14) * fro);
+    else
+        jacobi_svd<R, 8>(F, U, S, V, R(1e-30), R(1e-7) * fro);
+}
+
+// h and dh/dsigma of the Drucker-Prager return map (mpm_solver.h:114-166):
+// case 0 tension (h = 1), 1 elastic (h = sigma), 2 projected onto the cone.
+template <class R> struct DPh {
+    v3<R> h;
+    R J[9];             // J[k*3+j] = dh_k / dsigma_j
+    R dalpha[3], dkoh[3];  // dh_k / dalpha, dh_k / dkoh
+};
+template <class R> __device__ __forceinline__ DPh<R> dp_h(v3<R> s, R alpha, R koh) {
+    DPh<R> o;
+    const R sv[3] = {fmax(s.x, R(1e-6)), fmax(s.y, R(1e-6)), fmax(s.z, R(1e-6))};
+    const R e[3] = {log(sv[0]), log(sv[1]), log(sv[2])};
+    const R tr = e[0] + e[1] + e[2];
+    const R eh[3] = {e[0] - tr / R(3), e[1] - tr / R(3), e[2] - tr / R(3)};
+    const R H = sqrt(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) o.J[c] = R(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.dalpha[k] = o.dkoh[k] = R(0);
+    if (tr > R(0)) {
+        o.h = mk(R(1), R(1), R(1));
+        return o;
+    }
+    const R limit = -alpha * koh * tr;
+    if (!(H > limit)) {
+        o.h = s;
+        o.J[0] = o.J[4] = o.J[8] = R(1);
+        return o;
+    }
+    const R sc = limit / H;
+    R hk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) hk[k] = exp(sc * eh[k] + tr / R(3));
+    o.h = mk(hk[0], hk[1], hk[2]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const R dsc = -alpha * koh / H - sc * eh[j] / (H * H);  // d sc / d e_j
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const R de = sc * ((k == j ? R(1) : R(0)) - R(1) / R(3)) + eh[k] * dsc + R(1) / R(3);
+            o.J[k * 3 + j] = hk[k] * de / sv[j];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        o.dalpha[k] = hk[k] * eh[k] * (-koh * tr / H);
+        o.dkoh[k] = hk[k] * eh[k] * (-alpha * tr / H);
+    }
+    return o;
+}
+
+// h = tau_k / sigma_k of the Hencky-Kirchhoff stress (mpm_solver.h:186-217)
+// with its sigma, mu and lambda derivatives.
+template <class R> struct Hh {
+    v3<R> h;
+    R J[9];
+    R dmu[3], dlam[3];
+};
+template <class R> __device__ __forceinline__ Hh<R> hencky_h(v3<R> s, R mu, R lam) {
+    Hh<R> o;
+    const R sv[3] = {fmax(s.x, R(1e-6)), fmax(s.y, R(1e-6)), fmax(s.z, R(1e-6))};
+    const R e[3] = {log(sv[0]), log(sv[1]), log(sv[2])};
+    const R tr = e[0] + e[1] + e[2];
+    R hk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const R tau = R(2) * mu * e[k] + lam * tr;
+        hk[k] = tau / sv[k];
+        o.dmu[k] = R(2) * e[k] / sv[k];
+        o.dlam[k] = tr / sv[k];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const R dtau = (k == j ? R(2) * mu : R(0)) / sv[j] + lam / sv[j];
+            o.J[k * 3 + j] = (dtau - (k == j ? hk[k] : R(0))) / sv[k];
+        }
+    }
+    o.h = mk(hk[0], hk[1], hk[2]);
+    return o;
+}
+
+template <class R> __device__ __forceinline__ m3<R> usv(const m3<R>& U, v3<R> h, const m3<R>& V) {
+    m3<R> D;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) D.a[c] = R(0);
+    D.a[0] = h.x;
+    D.a[4] = h.y;
+    D.a[8] = h.z;
+    return mmt(mmul(U, D), V);
+}
+
+// Fbar of G(F) = U diag(h) V^T given Gbar; returns M = U^T Gbar V too.
+template <class R>
+__device__ __forceinline__ m3<R> usv_adjoint(const m3<R>& U, v3<R> s, const m3<R>& V, v3<R> h, const R J[9],
+                                             const m3<R>& Gbar, m3<R>& M) {
+    M = mmul(mtmul(U, Gbar), V);
+    const R sv[3] = {s.x, s.y, s.z}, hv[3] = {h.x, h.y, h.z};
+    const R thr = sizeof(R) == 8 ? R(1e-7) : R(1e-3);
+    m3<R> K;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        R acc = R(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc += M.a[i * 4] * J[i * 3 + k];
+        K.a[k * 4] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (i == j) continue;
+            const R ds = sv[j] - sv[i];
+            R D;
+            if (fabs(ds) > thr * (sv[i] + sv[j]))
+                D = (hv[j] - hv[i]) / ds;
+            else
+                D = R(0.5) * ((J[j * 3 + j] - J[i * 3 + j]) + (J[i * 3 + i] - J[j * 3 + i]));
+            const R S = (hv[j] + hv[i]) / (sv[j] + sv[i]);
+            K.a[i * 3 + j] = M.a[i * 3 + j] * R(0.5) * (D + S) + M.a[j * 3 + i] * R(0.5) * (D - S);
+        }
+    return mmt(mmul(U, K), V);
+}
+
+// Kirchhoff stress tau = P F^T with P = U diag(h) V^T.
+template <class R> __device__ __forceinline__ m3<R> stress(const m3<R>& F, R mu, R lam, int ref) {
+    m3<R> U, V;
+    v3<R> s;
+    svd_d(F, U, s, V, ref);
+    const Hh<R> H = hencky_h(s, mu, lam);
+    return mmt(usv(U, H.h, V), F);
+}
+
+// ------------------------------------------------------------------ wave sums
+template <class R> __device__ __forceinline__ void wave_add(double* dst, R v) {
+    double x = (double)v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if ((threadIdx.x & 63) == 0 && x != 0.0) atomicAdd(dst, x);
+}
+
+// ------------------------------------------------------------------ forward
+template <class R>
+__global__ void __launch_bounds__(256) k_p2g(PS<R> s, const R* m, const R* V0, int64_t n, Box b, Par<R> P, R dt,
+                                             R* gm, R* gq, int* err) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
+    const v3<R> v = mk(s.at(QV, p), s.at(QV + 1, p), s.at(QV + 2, p));
+    const m3<R> C = load_m(s, QC, p), F = load_m(s, QF, p);
+    const R mp = m[p];
+    const R kf = -R(4) * P.inv_dx * P.inv_dx * V0[p];
+    m3<R> K = stress(F, P.mu, P.lam, P.ref_svd);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) K.a[c] *= kf;
+    St<R> st;
+    stencil(x, P.inv_dx, st);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                const R w = st.w[0][i] * st.w[1][j] * st.w[2][k];
+                const int ni = st.b[0] + i, nj = st.b[1] + j, nk = st.b[2] + k;
+                const int64_t id = node_index<R>(b, ni, nj, nk);
+                if (id < 0) {
+                    atomicOr(err, 1);
+                    continue;
+                }
+                const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
+                const v3<R> a = (v + mvec(C, d)) * mp + mvec(K, d) * dt;
+                atomicAdd(gm + id, w * mp);
+                atomicAdd(gq + 3 * id + 0, w * a.x);
+                atomicAdd(gq + 3 * id + 1, w * a.y);
+                atomicAdd(gq + 3 * id + 2, w * a.z);
+            }
+}
+
+// The tool's probe at a node: inside flag and outward world normal.
+template <class R> __device__ __forceinline__ bool tool_probe(const Par<R>& P, v3<R> xn, v3<R>& n) {
+    const v3<R> r = xn - mk(P.tc[0], P.tc[1], P.tc[2]);
+    m3<R> Rw, Rl;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        Rw.a[c] = P.Rw[c];
+        Rl.a[c] = P.Rl[c];
+    }
+    const v3<R> l = mvec(Rw, r);
+    if (!(fabs(l.x) <= P.he[0] && fabs(l.y) <= P.he[1] && fabs(l.z) <= P.he[2])) return false;
+    const R ddx = P.he[0] - fabs(l.x), ddy = P.he[1] - fabs(l.y), ddz = P.he[2] - fabs(l.z);
+    v3<R> nl;
+    if (ddx <= ddy && ddx <= ddz)
+        nl = mk(l.x >= R(0) ? R(1) : R(-1), R(0), R(0));
+    else if (ddy <= ddz)
+        nl = mk(R(0), l.y >= R(0) ? R(1) : R(-1), R(0));
+    else
+        nl = mk(R(0), R(0), l.z >= R(0) ? R(1) : R(-1));
+    n = mvec(Rl, nl);
+    const R len = sqrt(dot(n, n));
+    n = len < R(1e-6) ? mk(R(0), R(0), R(0)) : n * (R(1) / len);
+    return true;
+}
+
+// Node velocity through update, floor and M0; the intermediate values the
+// adjoint needs are recomputed from (m, q) by the same routine.
+template <class R> struct NodeFwd {
+    bool upd, floor_on, floor_tiny, floor_zero, m0, m0_node_branch;
+    v3<R> u0, u1, u2, n;
+    R sc, tm, vn, fric;
+};
+template <class R>
+__device__ __forceinline__ NodeFwd<R> node_forward(R m, v3<R> q, int j_node, v3<R> xn_c, const Par<R>& P, R dt) {
+    NodeFwd<R> o{};
+    o.upd = m > R(1e-6);
+    if (o.upd) {
+        o.u0 = q * (R(1) / m) + mk(P.g[0] * dt, P.g[1] * dt, P.g[2] * dt);
+        o.u1 = o.u0;
+        const R ny = ((R)j_node + R(0.5)) * P.dx;
+        if (ny < P.dx && o.u0.y < R(0)) {
+            o.floor_on = true;
+            o.vn = -o.u0.y;
+            o.u1.y = R(0);
+            o.tm = sqrt(o.u0.x * o.u0.x + o.u0.z * o.u0.z);
+            const R budget = P.mu_floor * o.vn;
+            if (o.tm > R(1e-12)) {
+                const R raw = (o.tm - budget) / o.tm;
+                o.floor_zero = !(raw > R(0));
+                o.sc = o.floor_zero ? R(0) : raw;
+                o.u1.x = o.u0.x * o.sc;
+                o.u1.z = o.u0.z * o.sc;
+            } else {
+                o.floor_tiny = true;
+            }
+        }
+    } else {
+        o.u0 = o.u1 = q;
+    }
+    o.u2 = o.u1;
+    if (P.tool && m > R(1e-8) && tool_probe(P, xn_c, o.n)) {
+        o.m0 = true;
+        o.fric = fmin(R(1), P.friction);
+        const v3<R> vb = mk(P.vt[0], P.vt[1], P.vt[2]);
+        const R vnn = dot(o.u1, o.n), vnb = dot(vb, o.n);
+        o.m0_node_branch = vnn >= vnb;
+        const R vnc = o.m0_node_branch ? vnn : vnb;
+        const v3<R> vtn = o.u1 - o.n * vnn, vtb = vb - o.n * vnb;
+        o.u2 = o.n * vnc + vtn + (vtb - vtn) * o.fric;
+    }
+    return o;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_update(Box b, Par<R> P, R dt, const R* gm, const R* gq, R* gu) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nn = (int64_t)b.nx * b.ny * b.nz;
+    if (id >= nn) return;
+    const R m = gm[id];
+    if (m == R(0)) {
+        gu[3 * id] = gu[3 * id + 1] = gu[3 * id + 2] = R(0);
+        return;
+    }
+    const int k = (int)(id % b.nz), j = (int)((id / b.nz) % b.ny), i = (int)(id / ((int64_t)b.nz * b.ny));
+    const int gi = b.ox + i, gj = b.oy + j, gk = b.oz + k;
+    const v3<R> xc = mk(((R)gi + R(0.5)) * P.dx, ((R)gj + R(0.5)) * P.dx, ((R)gk + R(0.5)) * P.dx);
+    const NodeFwd<R> f = node_forward(m, mk(gq[3 * id], gq[3 * id + 1], gq[3 * id + 2]), gj, xc, P, dt);
+    gu[3 * id] = f.u2.x;
+    gu[3 * id + 1] = f.u2.y;
+    gu[3 * id + 2] = f.u2.z;
+}
+
+// G2P forward quantities of one particle (shared by the forward kernel and
+// the adjoint's recomputation).
+template <class R> struct G2PFwd {
+    St<R> st;
+    v3<R> vn;
+    m3<R> Cn, Ftr, U, V;
+    v3<R> s;
+    DPh<R> dp;
+    v3<R> xn;
+    bool clamped;
+};
+template <class R>
+__device__ __forceinline__ void g2p_forward(const R x[3], const m3<R>& F, const Box& b, const Par<R>& P, R dt,
+                                            const R* gu, G2PFwd<R>& o, int* err) {
+    stencil(x, P.inv_dx, o.st);
+    const R kC = R(4) * P.inv_dx * P.inv_dx;
+    o.vn = mk(R(0), R(0), R(0));
+#pragma unroll
+    for (int c = 0; c < 9; ++c) o.Cn.a[c] = R(0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                const R w = o.st.w[0][i] * o.st.w[1][j] * o.st.w[2][k];
+                const int ni = o.st.b[0] + i, nj = o.st.b[1] + j, nk = o.st.b[2] + k;
+                const int64_t id = node_index<R>(b, ni, nj, nk);
+                if (id < 0) {
+                    if (err) atomicOr(err, 1);
+                    continue;
+                }
+                const v3<R> u = mk(gu[3 * id], gu[3 * id + 1], gu[3 * id + 2]);
+                const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
+                o.vn = o.vn + u * w;
+                const v3<R> wu = u * (w * kC);
+                o.Cn.a[0] += wu.x * d.x; o.Cn.a[1] += wu.x * d.y; o.Cn.a[2] += wu.x * d.z;
+                o.Cn.a[3] += wu.y * d.x; o.Cn.a[4] += wu.y * d.y; o.Cn.a[5] += wu.y * d.z;
+                o.Cn.a[6] += wu.z * d.x; o.Cn.a[7] += wu.z * d.y; o.Cn.a[8] += wu.z * d.z;
+            }
+    o.xn = mk(x[0] + dt * o.vn.x, x[1] + dt * o.vn.y, x[2] + dt * o.vn.z);
+    m3<R> A = o.Cn;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A.a[c] *= dt;
+    A.a[0] += R(1);
+    A.a[4] += R(1);
+    A.a[8] += R(1);
+    o.Ftr = mmul(A, F);
+    svd_d(o.Ftr, o.U, o.s, o.V, P.ref_svd);
+    o.dp = dp_h(o.s, P.alpha, P.koh);
+    o.clamped = o.xn.y < R(0);
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_g2p(PS<R> s, PS<R> out, int64_t n, Box b, Par<R> P, R dt, const R* gu,
+                                             int* err) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
+    const m3<R> F = load_m(s, QF, p);
+    G2PFwd<R> f;
+    g2p_forward(x, F, b, P, dt, gu, f, err);
+    v3<R> xn = f.xn, vn = f.vn;
+    if (f.clamped) {
+        xn.y = R(0);
+        vn.y = R(0);
+    }
+    out.at(QX, p) = xn.x;
+    out.at(QX + 1, p) = xn.y;
+    out.at(QX + 2, p) = xn.z;
+    out.at(QV, p) = vn.x;
+    out.at(QV + 1, p) = vn.y;
+    out.at(QV + 2, p) = vn.z;
+    store_m(out, QC, p, f.Cn);
+    store_m(out, QF, p, usv(f.U, f.dp.h, f.V));
+}
+
+// ------------------------------------------------------------------ loss
+// L = (1/N) sum_p |x_p(T) - x_p(0) - d_p|^2; seeds xbar of the final state.
+template <class R>
+__global__ void __launch_bounds__(256) k_loss(PS<R> fin, const R* x0, const R* tgt, int64_t n, PS<R> bar,
+                                              double* acc) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    R l = R(0);
+    if (p < n) {
+        const R inv = R(1) / (R)n;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const R r = fin.at(QX + a, p) - x0[3 * p + a] - tgt[3 * p + a];
+            l += r * r * inv;
+            bar.at(QX + a, p) = R(2) * r * inv;
+        }
+#pragma unroll
+        for (int c = QV; c < QCOMP; ++c) bar.at(c, p) = R(0);
+    }
+    wave_add(acc + GLOSS, l);
+}
+
+// ------------------------------------------------------------------ adjoint
+// G2P: from the adjoint of the new state (ain) to the adjoint of x and F of
+// the old state (aout; its v and C parts are zeroed for the P2G adjoint) and
+// the adjoint of the node velocities (gub, accumulated).
+template <class R>
+__global__ void __launch_bounds__(256) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, int64_t n, Box b, Par<R> P, R dt,
+                                                 const R* gu, R* gub, double* grad) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    R galpha = R(0), gkoh = R(0);
+    if (p < n) {
+        const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
+        const m3<R> F = load_m(s, QF, p);
+        G2PFwd<R> f;
+        g2p_forward(x, F, b, P, dt, gu, f, (int*)nullptr);
+        v3<R> xb = mk(ain.at(QX, p), ain.at(QX + 1, p), ain.at(QX + 2, p));
+        v3<R> vb = mk(ain.at(QV, p), ain.at(QV + 1, p), ain.at(QV + 2, p));
+        m3<R> Cb = load_m(ain, QC, p);
+        const m3<R> Fb_new = load_m(ain, QF, p);
+        if (f.clamped) {  // x'.y and v'.y were set to constants
+            xb.y = R(0);
+            vb.y = R(0);
+        }
+        // F' = proj(Ftr)
+        m3<R> M;
+        const m3<R> Ftrb = usv_adjoint(f.U, f.s, f.V, f.dp.h, f.dp.J, Fb_new, M);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            galpha += M.a[k * 4] * f.dp.dalpha[k];
+            gkoh += M.a[k * 4] * f.dp.dkoh[k];
+        }
+        // Ftr = (I + dt C') F
+        m3<R> A = f.Cn;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) A.a[c] *= dt;
+        A.a[0] += R(1);
+        A.a[4] += R(1);
+        A.a[8] += R(1);
+        const m3<R> Fb = mtmul(A, Ftrb);
+        const m3<R> dC = mmt(Ftrb, F);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Cb.a[c] += dt * dC.a[c];
+        // x' = x + dt v'
+        vb = vb + xb * dt;
+        v3<R> xo = xb;
+        const R kC = R(4) * P.inv_dx * P.inv_dx;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    const R w = f.st.w[0][i] * f.st.w[1][j] * f.st.w[2][k];
+                    const v3<R> gw = mk(f.st.dw[0][i] * f.st.w[1][j] * f.st.w[2][k],
+                                        f.st.w[0][i] * f.st.dw[1][j] * f.st.w[2][k],
+                                        f.st.w[0][i] * f.st.w[1][j] * f.st.dw[2][k]);
+                    const int ni = f.st.b[0] + i, nj = f.st.b[1] + j, nk = f.st.b[2] + k;
+                    const int64_t id = node_index<R>(b, ni, nj, nk);
+                    if (id < 0) continue;
+                    const v3<R> u = mk(gu[3 * id], gu[3 * id + 1], gu[3 * id + 2]);
+                    const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
+                    const v3<R> Cd = mvec(Cb, d);
+                    // v' = sum w u;  C' = kC sum w u d^T
+                    const v3<R> ub = (vb + Cd * kC) * w;
+                    atomicAdd(gub + 3 * id + 0, ub.x);
+                    atomicAdd(gub + 3 * id + 1, ub.y);
+                    atomicAdd(gub + 3 * id + 2, ub.z);
+                    const R su = dot(u, vb) + kC * dot(u, Cd);
+                    const v3<R> CTu = mk(Cb.a[0] * u.x + Cb.a[3] * u.y + Cb.a[6] * u.z,
+                                         Cb.a[1] * u.x + Cb.a[4] * u.y + Cb.a[7] * u.z,
+                                         Cb.a[2] * u.x + Cb.a[5] * u.y + Cb.a[8] * u.z);
+                    xo = xo + gw * su - CTu * (kC * w);
+                }
+        aout.at(QX, p) = xo.x;
+        aout.at(QX + 1, p) = xo.y;
+        aout.at(QX + 2, p) = xo.z;
+        aout.at(QV, p) = aout.at(QV + 1, p) = aout.at(QV + 2, p) = R(0);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) aout.at(QC + c, p) = R(0);
+        store_m(aout, QF, p, Fb);
+    }
+    wave_add(grad + GALPHA, galpha);
+    wave_add(grad + GKOH, gkoh);
+}
+
+// Grid: from the node-velocity adjoint to the adjoint of (m, q); the tool
+// velocity's adjoint from the M0 nodes.
+template <class R>
+__global__ void __launch_bounds__(256) k_update_adj(Box b, Par<R> P, R dt, const R* gm, const R* gq, const R* gub,
+                                                    R* gqb, R* gmb, double* grad) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nn = (int64_t)b.nx * b.ny * b.nz;
+    v3<R> vtb = mk(R(0), R(0), R(0));
+    if (id < nn) {
+        const R m = gm[id];
+        gqb[3 * id] = gqb[3 * id + 1] = gqb[3 * id + 2] = R(0);
+        gmb[id] = R(0);
+        if (m != R(0)) {
+            const int k = (int)(id % b.nz), j = (int)((id / b.nz) % b.ny), i = (int)(id / ((int64_t)b.nz * b.ny));
+            const int gj = b.oy + j;
+            const v3<R> xc = mk(((R)(b.ox + i) + R(0.5)) * P.dx, ((R)gj + R(0.5)) * P.dx,
+                                ((R)(b.oz + k) + R(0.5)) * P.dx);
+            const v3<R> q = mk(gq[3 * id], gq[3 * id + 1], gq[3 * id + 2]);
+            const NodeFwd<R> f = node_forward(m, q, gj, xc, P, dt);
+            v3<R> u2b = mk(gub[3 * id], gub[3 * id + 1], gub[3 * id + 2]);
+            v3<R> u1b = u2b;
+            if (f.m0) {
+                const R nb = dot(u2b, f.n);
+                const v3<R> tan = u2b - f.n * nb;
+                u1b = tan * (R(1) - f.fric) + (f.m0_node_branch ? f.n * nb : mk(R(0), R(0), R(0)));
+                vtb = tan * f.fric + (f.m0_node_branch ? mk(R(0), R(0), R(0)) : f.n * nb);
+            }
+            v3<R> u0b = u1b;
+            if (f.floor_on) {
+                if (f.floor_tiny) {
+                    u0b = mk(u1b.x, R(0), u1b.z);
+                } else if (f.floor_zero) {
+                    u0b = mk(R(0), R(0), R(0));
+                } else {
+                    const R gsc = u1b.x * f.u0.x + u1b.z * f.u0.z;  // dL / d sc
+                    const R t3 = f.tm * f.tm * f.tm;
+                    u0b.x = u1b.x * f.sc + gsc * P.mu_floor * f.vn * f.u0.x / t3;
+                    u0b.z = u1b.z * f.sc + gsc * P.mu_floor * f.vn * f.u0.z / t3;
+                    u0b.y = gsc * P.mu_floor / f.tm;
+                }
+            }
+            if (f.upd) {
+                const R im = R(1) / m;
+                gqb[3 * id] = u0b.x * im;
+                gqb[3 * id + 1] = u0b.y * im;
+                gqb[3 * id + 2] = u0b.z * im;
+                gmb[id] = -dot(u0b, q) * im * im;
+            } else {
+                gqb[3 * id] = u0b.x;
+                gqb[3 * id + 1] = u0b.y;
+                gqb[3 * id + 2] = u0b.z;
+            }
+        }
+    }
+    wave_add(grad + GVT + 0, vtb.x);
+    wave_add(grad + GVT + 1, vtb.y);
+    wave_add(grad + GVT + 2, vtb.z);
+}
+
+// P2G: from the adjoint of (m, q) to the adjoint of x, v, C, F (added to
+// aout) and of mu and lambda.
+template <class R>
+__global__ void __launch_bounds__(256) k_p2g_adj(PS<R> s, const R* m, const R* V0, int64_t n, Box b, Par<R> P, R dt,
+                                                 const R* gqb, const R* gmb, PS<R> aout, double* grad) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    R gmu = R(0), glam = R(0);
+    if (p < n) {
+        const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
+        const v3<R> v = mk(s.at(QV, p), s.at(QV + 1, p), s.at(QV + 2, p));
+        const m3<R> C = load_m(s, QC, p), F = load_m(s, QF, p);
+        const R mp = m[p];
+        const R kf = -R(4) * P.inv_dx * P.inv_dx * V0[p];
+        m3<R> U, V;
+        v3<R> sg;
+        svd_d(F, U, sg, V, P.ref_svd);
+        const Hh<R> H = hencky_h(sg, P.mu, P.lam);
+        const m3<R> Pm = usv(U, H.h, V);
+        m3<R> K = mmt(Pm, F);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) K.a[c] *= kf;
+        St<R> st;
+        stencil(x, P.inv_dx, st);
+        v3<R> xb = mk(R(0), R(0), R(0)), vb = xb;
+        m3<R> Cb, Kb;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Cb.a[c] = Kb.a[c] = R(0);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    const R w = st.w[0][i] * st.w[1][j] * st.w[2][k];
+                    const v3<R> gw = mk(st.dw[0][i] * st.w[1][j] * st.w[2][k],
+                                        st.w[0][i] * st.dw[1][j] * st.w[2][k],
+                                        st.w[0][i] * st.w[1][j] * st.dw[2][k]);
+                    const int ni = st.b[0] + i, nj = st.b[1] + j, nk = st.b[2] + k;
+                    const int64_t id = node_index<R>(b, ni, nj, nk);
+                    if (id < 0) continue;
+                    const v3<R> Qb = mk(gqb[3 * id], gqb[3 * id + 1], gqb[3 * id + 2]);
+                    const R Mb = gmb[id];
+                    const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
+                    const v3<R> a = (v + mvec(C, d)) * mp + mvec(K, d) * dt;
+                    vb = vb + Qb * (w * mp);
+                    const v3<R> wq = Qb * w;
+                    // C += w m Qb d^T ; K += dt w Qb d^T
+                    Cb.a[0] += wq.x * d.x * mp; Cb.a[1] += wq.x * d.y * mp; Cb.a[2] += wq.x * d.z * mp;
+                    Cb.a[3] += wq.y * d.x * mp; Cb.a[4] += wq.y * d.y * mp; Cb.a[5] += wq.y * d.z * mp;
+                    Cb.a[6] += wq.z * d.x * mp; Cb.a[7] += wq.z * d.y * mp; Cb.a[8] += wq.z * d.z * mp;
+                    Kb.a[0] += wq.x * d.x * dt; Kb.a[1] += wq.x * d.y * dt; Kb.a[2] += wq.x * d.z * dt;
+                    Kb.a[3] += wq.y * d.x * dt; Kb.a[4] += wq.y * d.y * dt; Kb.a[5] += wq.y * d.z * dt;
+                    Kb.a[6] += wq.z * d.x * dt; Kb.a[7] += wq.z * d.y * dt; Kb.a[8] += wq.z * d.z * dt;
+                    // weights and d depend on x
+                    const v3<R> CTq = mk(C.a[0] * Qb.x + C.a[3] * Qb.y + C.a[6] * Qb.z,
+                                         C.a[1] * Qb.x + C.a[4] * Qb.y + C.a[7] * Qb.z,
+                                         C.a[2] * Qb.x + C.a[5] * Qb.y + C.a[8] * Qb.z);
+                    const v3<R> KTq = mk(K.a[0] * Qb.x + K.a[3] * Qb.y + K.a[6] * Qb.z,
+                                         K.a[1] * Qb.x + K.a[4] * Qb.y + K.a[7] * Qb.z,
+                                         K.a[2] * Qb.x + K.a[5] * Qb.y + K.a[8] * Qb.z);
+                    xb = xb + gw * (dot(Qb, a) + Mb * mp) - (CTq * mp + KTq * dt) * w;
+                }
+        // K = kf tau,  tau = P F^T
+        m3<R> taub = Kb;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) taub.a[c] *= kf;
+        m3<R> Fb = mtmul(taub, Pm);  // tau^T-bar P  (d tau = P dF^T)
+        const m3<R> Pb = mmul(taub, F);
+        m3<R> M;
+        const m3<R> Fb2 = usv_adjoint(U, sg, V, H.h, H.J, Pb, M);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            gmu += M.a[k * 4] * H.dmu[k];
+            glam += M.a[k * 4] * H.dlam[k];
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Fb.a[c] += Fb2.a[c];
+        aout.at(QX, p) += xb.x;
+        aout.at(QX + 1, p) += xb.y;
+        aout.at(QX + 2, p) += xb.z;
+        aout.at(QV, p) += vb.x;
+        aout.at(QV + 1, p) += vb.y;
+        aout.at(QV + 2, p) += vb.z;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            aout.at(QC + c, p) += Cb.a[c];
+            aout.at(QF + c, p) += Fb.a[c];
+        }
+    }
+    wave_add(grad + GMU, gmu);
+    wave_add(grad + GLAM, glam);
+}
+
+template <class R> __global__ void k_zero(R* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = R(0);
+}
+
+}  // namespace dif
+}  // namespace bm
+
+using namespace bm;
+using namespace bm::dif;
+
+namespace {
+
+inline int blocks_for(int64_t n) { return (int)((n + 255) / 256); }
+
+}  // namespace
+
+struct bm_diff {
+    bm_diff_desc desc{};
+    bm_diff_params prm{};
+    bool has_prm = false;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool fp64 = false;
+    int K = 16;
+    Box box{};
+    int64_t cap = 0, n = 0, nn = 0;
+    size_t rs = 4;  // bytes per scalar
+    // device buffers (scalar type per precision)
+    void *m = nullptr, *V0 = nullptr, *x0 = nullptr, *tgt = nullptr, *init = nullptr;
+    void *gm = nullptr, *gq = nullptr, *gu = nullptr, *gub = nullptr, *gqb = nullptr, *gmb = nullptr;
+    void *ck = nullptr, *seg = nullptr, *abar = nullptr, *abar2 = nullptr;
+    int64_t ck_slots = 0, seg_slots = 0;
+    double* grad = nullptr;
+    int* err = nullptr;
+    // last forward
+    float dt = 0.0f;
+    int substeps = 0;
+    bool have_forward = false;
+};
+
+static int check_diff(bm_diff_handle h) { return h ? BM_OK : fail(BM_ERR_INVALID_ARG, "null diff handle"); }
+
+template <class R> static Par<R> params_at(const bm_diff_handle h, int step) {
+    const bm_diff_params& q = h->prm;
+    Par<R> P{};
+    P.dx = (R)h->desc.dx;
+    P.inv_dx = R(1) / P.dx;
+    const double E = q.young_modulus, nu = q.poisson_ratio;
+    const double mu = E / (2.0 * (1.0 + nu)), lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    P.mu = (R)mu;
+    P.lam = (R)lam;
+    const double sphi = std::sin(q.friction_deg * M_PI / 180.0);
+    P.alpha = (R)(std::sqrt(2.0 / 3.0) * 2.0 * sphi / (3.0 - sphi));
+    P.koh = (R)((3.0 * lam + 2.0 * mu) / (2.0 * mu));
+    P.mu_floor = (R)q.floor_friction;
+    P.ref_svd = h->desc.reference_svd ? 1 : 0;
+    for (int a = 0; a < 3; ++a) P.g[a] = (R)q.gravity[a];
+    P.tool = q.has_tool ? 1 : 0;
+    if (P.tool) {
+        const double t = (double)step * (double)h->dt;
+        for (int a = 0; a < 3; ++a) {
+            P.tc[a] = (R)(q.tool_center[a] + q.tool_velocity[a] * t);
+            P.he[a] = (R)q.tool_half_extents[a];
+            P.vt[a] = (R)q.tool_velocity[a];
+        }
+        // rotation from the unit quaternion (w, x, y, z)
+        double w = q.tool_quat[0], x = q.tool_quat[1], y = q.tool_quat[2], z = q.tool_quat[3];
+        const double nq = std::sqrt(w * w + x * x + y * y + z * z);
+        if (nq > 0) {
+            w /= nq;
+            x /= nq;
+            y /= nq;
+            z /= nq;
+        } else {
+            w = 1;
+        }
+        const double Rm[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - w * z),     2 * (x * z + w * y),
+                              2 * (x * y + w * z),     1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                              2 * (x * z - w * y),     2 * (y * z + w * x),     1 - 2 * (x * x + y * y)};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                P.Rl[i * 3 + j] = (R)Rm[i * 3 + j];
+                P.Rw[i * 3 + j] = (R)Rm[j * 3 + i];
+            }
+        P.friction = (R)q.tool_friction;
+    }
+    return P;
+}
+
+template <class R> static PS<R> slot(void* base, int64_t cap, int64_t k) {
+    return PS<R>{static_cast<R*>(base) + k * QCOMP * cap, cap};
+}
+
+// One forward substep from state `in` to state `out` (grid left in gm/gq/gu).
+template <class R> static void substep(bm_diff_handle h, PS<R> in, PS<R> out, int step) {
+    const Par<R> P = params_at<R>(h, step);
+    const R dt = (R)h->dt;
+    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
+    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
+    hipLaunchKernelGGL(k_p2g<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, in, (const R*)h->m,
+                       (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err);
+    hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
+                       (const R*)h->gm, (const R*)h->gq, (R*)h->gu);
+    hipLaunchKernelGGL(k_g2p<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, in, out, h->n, h->box, P, dt,
+                       (const R*)h->gu, h->err);
+}
+
+static int ensure(void** p, size_t bytes) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return fail(BM_ERR_OOM, "diff allocation");
+    return BM_OK;
+}
+
+static int read_err(bm_diff_handle h) {
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int e = 0;
+    HIP_TRY(hipMemcpy(&e, h->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) {
+        HIP_TRY(hipMemset(h->err, 0, sizeof(int)));
+        return fail(BM_ERR_OUT_OF_DOMAIN, "a particle stencil left the node box");
+    }
+    return BM_OK;
+}
+
+template <class R> static int forward_t(bm_diff_handle h, int substeps, double* loss) {
+    const int K = h->K;
+    // checkpoint slot c holds the state after min(c K, substeps) substeps
+    const int64_t nck = (substeps + K - 1) / K + 1;
+    if (nck > h->ck_slots) {
+        if (int rc = ensure(&h->ck, h->rs * QCOMP * h->cap * nck)) return rc;
+        h->ck_slots = nck;
+    }
+    if (h->seg_slots < K) {
+        if (int rc = ensure(&h->seg, h->rs * QCOMP * h->cap * K)) return rc;
+        h->seg_slots = K;
+    }
+    HIP_TRY(hipMemcpyAsync(h->ck, h->init, h->rs * QCOMP * h->cap, hipMemcpyDeviceToDevice, h->stream));
+    PS<R> cur = slot<R>(h->ck, h->cap, 0);
+    int pp = 0;  // states between checkpoints alternate between two scratch slots
+    for (int s = 0; s < substeps; ++s) {
+        const int ns = s + 1;
+        PS<R> nxt;
+        if (ns % K == 0 || ns == substeps) {
+            nxt = slot<R>(h->ck, h->cap, (ns + K - 1) / K);
+        } else {
+            nxt = slot<R>(h->seg, h->cap, pp);
+            pp ^= 1;
+        }
+        substep<R>(h, cur, nxt, s);
+        cur = nxt;
+    }
+    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
+    hipLaunchKernelGGL(k_loss<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, cur, (const R*)h->x0,
+                       (const R*)h->tgt, h->n, slot<R>(h->abar, h->cap, 0), h->grad);
+    HIP_TRY(hipGetLastError());
+    if (int rc = read_err(h)) return rc;
+    double g[GN];
+    HIP_TRY(hipMemcpy(g, h->grad, sizeof(g), hipMemcpyDeviceToHost));
+    if (loss) *loss = g[GLOSS];
+    h->substeps = substeps;
+    h->have_forward = true;
+    return BM_OK;
+}
+
+template <class R> static int backward_t(bm_diff_handle h, double* g) {
+    const int K = h->K;
+    const int T = h->substeps;
+    const int nseg = (T + K - 1) / K;
+    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
+    PS<R> a = slot<R>(h->abar, h->cap, 0), a2 = slot<R>(h->abar2, h->cap, 0);
+    const R dt = (R)h->dt;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * K, len = std::min(K, T - s0);
+        // recompute the segment's states from its checkpoint
+        hipMemcpyAsync(slot<R>(h->seg, h->cap, 0).p, slot<R>(h->ck, h->cap, sg).p, h->rs * QCOMP * h->cap,
+                       hipMemcpyDeviceToDevice, h->stream);
+        for (int j = 0; j < len - 1; ++j)
+            substep<R>(h, slot<R>(h->seg, h->cap, j), slot<R>(h->seg, h->cap, j + 1), s0 + j);
+        for (int j = len - 1; j >= 0; --j) {
+            const PS<R> sj = slot<R>(h->seg, h->cap, j);
+            const Par<R> P = params_at<R>(h, s0 + j);
+            // the grid of substep s0 + j
+            hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
+            hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
+            hipLaunchKernelGGL(k_p2g<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, sj, (const R*)h->m,
+                               (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err);
+            hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
+                               (const R*)h->gm, (const R*)h->gq, (R*)h->gu);
+            hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gub,
+                               3 * h->nn);
+            hipLaunchKernelGGL(k_g2p_adj<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, sj, a, a2, h->n,
+                               h->box, P, dt, (const R*)h->gu, (R*)h->gub, h->grad);
+            hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
+                               (const R*)h->gm, (const R*)h->gq, (const R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad);
+            hipLaunchKernelGGL(k_p2g_adj<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, sj, (const R*)h->m,
+                               (const R*)h->V0, h->n, h->box, P, dt, (const R*)h->gqb, (const R*)h->gmb, a2, h->grad);
+            std::swap(a, a2);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    if (int rc = read_err(h)) return rc;
+    HIP_TRY(hipMemcpy(g, h->grad, sizeof(double) * GN, hipMemcpyDeviceToHost));
+    return BM_OK;
+}
+
+template <class R> static int download_t(bm_diff_handle h, bm_particles* out) {
+    const int64_t n = h->n;
+    const int64_t last = (h->substeps + h->K - 1) / h->K;
+    std::vector<R> st((size_t)(QCOMP * n));
+    HIP_TRY(hipMemcpy2D(st.data(), sizeof(R) * n, slot<R>(h->ck, h->cap, last).p, sizeof(R) * h->cap, sizeof(R) * n,
+                        QCOMP, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            if (out->x) out->x[3 * i + k] = (float)st[(size_t)((QX + k) * n + i)];
+            if (out->v) out->v[3 * i + k] = (float)st[(size_t)((QV + k) * n + i)];
+        }
+        for (int k = 0; k < 9; ++k) {
+            if (out->F) out->F[9 * i + k] = (float)st[(size_t)((QF + k) * n + i)];
+            if (out->C) out->C[9 * i + k] = (float)st[(size_t)((QC + k) * n + i)];
+        }
+    }
+    return BM_OK;
+}
+
+template <class R> static int upload_t(bm_diff_handle h, int64_t n, const bm_particles* p, const float* tgt) {
+    std::vector<R> st((size_t)(QCOMP * n)), mm((size_t)n), vv((size_t)n), x0((size_t)(3 * n)), tg((size_t)(3 * n));
+    for (int64_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            st[(size_t)((QX + k) * n + i)] = (R)p->x[3 * i + k];
+            st[(size_t)((QV + k) * n + i)] = (R)p->v[3 * i + k];
+            x0[(size_t)(3 * i + k)] = (R)p->x[3 * i + k];
+            tg[(size_t)(3 * i + k)] = tgt ? (R)tgt[3 * i + k] : R(0);
+        }
+        for (int k = 0; k < 9; ++k) {
+            st[(size_t)((QC + k) * n + i)] = (R)p->C[9 * i + k];
+            st[(size_t)((QF + k) * n + i)] = (R)p->F[9 * i + k];
+        }
+        mm[(size_t)i] = (R)p->m[i];
+        vv[(size_t)i] = (R)p->V0[i];
+    }
+    HIP_TRY(hipMemcpy2D(h->init, sizeof(R) * h->cap, st.data(), sizeof(R) * n, sizeof(R) * n, QCOMP,
+                        hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->m, mm.data(), sizeof(R) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->V0, vv.data(), sizeof(R) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->x0, x0.data(), sizeof(R) * 3 * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->tgt, tg.data(), sizeof(R) * 3 * n, hipMemcpyHostToDevice));
+    return BM_OK;
+}
+
+extern "C" {
+
+int bm_diff_create(const bm_diff_desc* desc, bm_diff_handle* out) {
+    if (!desc || !out) return fail(BM_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (!(desc->dx > 0.0f)) return fail(BM_ERR_INVALID_ARG, "dx must be > 0");
+    for (int a = 0; a < 3; ++a)
+        if (desc->grid_dims[a] <= 0) return fail(BM_ERR_INVALID_ARG, "grid_dims must be > 0");
+    if (desc->max_particles <= 0) return fail(BM_ERR_INVALID_ARG, "max_particles must be > 0");
+    if (desc->precision != 32 && desc->precision != 64) return fail(BM_ERR_INVALID_ARG, "precision must be 32 or 64");
+    if (desc->checkpoint_every < 0) return fail(BM_ERR_INVALID_ARG, "checkpoint_every must be >= 0");
+    auto* h = new bm_diff();
+    h->desc = *desc;
+    h->fp64 = desc->precision == 64;
+    h->rs = h->fp64 ? 8 : 4;
+    h->K = desc->checkpoint_every > 0 ? desc->checkpoint_every : 16;
+    h->box = Box{desc->grid_origin[0], desc->grid_origin[1], desc->grid_origin[2],
+                 desc->grid_dims[0],   desc->grid_dims[1],   desc->grid_dims[2]};
+    h->nn = (int64_t)desc->grid_dims[0] * desc->grid_dims[1] * desc->grid_dims[2];
+    h->cap = desc->max_particles;
+    auto bail = [&](int rc) {
+        bm_diff_destroy(h);
+        return rc;
+    };
+    if (desc->stream) {
+        h->stream = (hipStream_t)desc->stream;
+    } else {
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+            return bail(fail(BM_ERR_HIP, "stream creation failed"));
+        h->own_stream = true;
+    }
+    const size_t r = h->rs;
+    int rc = BM_OK;
+    if (!rc) rc = ensure(&h->m, r * h->cap);
+    if (!rc) rc = ensure(&h->V0, r * h->cap);
+    if (!rc) rc = ensure(&h->x0, r * 3 * h->cap);
+    if (!rc) rc = ensure(&h->tgt, r * 3 * h->cap);
+    if (!rc) rc = ensure(&h->init, r * QCOMP * h->cap);
+    if (!rc) rc = ensure(&h->gm, r * h->nn);
+    if (!rc) rc = ensure(&h->gq, r * 3 * h->nn);
+    if (!rc) rc = ensure(&h->gu, r * 3 * h->nn);
+    if (!rc) rc = ensure(&h->gub, r * 3 * h->nn);
+    if (!rc) rc = ensure(&h->gqb, r * 3 * h->nn);
+    if (!rc) rc = ensure(&h->gmb, r * h->nn);
+    if (!rc) rc = ensure(&h->abar, r * QCOMP * h->cap);
+    if (!rc) rc = ensure(&h->abar2, r * QCOMP * h->cap);
+    if (!rc) rc = ensure((void**)&h->grad, sizeof(double) * GN);
+    if (!rc) rc = ensure((void**)&h->err, sizeof(int));
+    if (rc) return bail(rc);
+    if (hipMemset(h->err, 0, sizeof(int)) != hipSuccess) return bail(fail(BM_ERR_HIP, "diff initialisation"));
+    *out = h;
+    return BM_OK;
+}
+
+int bm_diff_destroy(bm_diff_handle h) {
+    if (!h) return BM_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void* p : {h->m, h->V0, h->x0, h->tgt, h->init, h->gm, h->gq, h->gu, h->gub, h->gqb, h->gmb, h->ck, h->seg, h->abar,
+                    h->abar2, (void*)h->grad, (void*)h->err})
+        if (p) hipFree(p);
+    if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return BM_OK;
+}
+
+int bm_diff_set_params(bm_diff_handle h, const bm_diff_params* p) {
+    if (int rc = check_diff(h)) return rc;
+    if (!p) return fail(BM_ERR_INVALID_ARG, "null params");
+    if (!(p->young_modulus > 0.0) || !(p->poisson_ratio > -1.0 && p->poisson_ratio < 0.5))
+        return fail(BM_ERR_INVALID_ARG, "bad elastic constants");
+    if (p->has_tool && !(p->tool_half_extents[0] > 0 && p->tool_half_extents[1] > 0 && p->tool_half_extents[2] > 0))
+        return fail(BM_ERR_INVALID_ARG, "tool half extents must be > 0");
+    h->prm = *p;
+    h->has_prm = true;
+    h->have_forward = false;
+    return BM_OK;
+}
+
+int bm_diff_upload(bm_diff_handle h, int64_t n, const bm_particles* p, const float* target_displacement) {
+    if (int rc = check_diff(h)) return rc;
+    if (n <= 0 || !p || !p->x || !p->v || !p->F || !p->C || !p->m || !p->V0)
+        return fail(BM_ERR_INVALID_ARG, "bad particle arrays");
+    if (n > h->cap) return fail(BM_ERR_OOM, "capacity %lld exceeded", (long long)h->cap);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->n = n;
+    h->have_forward = false;
+    return h->fp64 ? upload_t<double>(h, n, p, target_displacement) : upload_t<float>(h, n, p, target_displacement);
+}
+
+int bm_diff_forward(bm_diff_handle h, float dt, int32_t substeps, double* loss) {
+    if (int rc = check_diff(h)) return rc;
+    if (!h->has_prm) return fail(BM_ERR_STATE, "parameters not set");
+    if (h->n <= 0) return fail(BM_ERR_STATE, "no particles");
+    if (!(dt > 0.0f) || substeps <= 0) return fail(BM_ERR_INVALID_ARG, "dt and substeps must be > 0");
+    h->dt = dt;
+    return h->fp64 ? forward_t<double>(h, substeps, loss) : forward_t<float>(h, substeps, loss);
+}
+
+int bm_diff_backward(bm_diff_handle h, bm_diff_grad* out) {
+    if (int rc = check_diff(h)) return rc;
+    if (!out) return fail(BM_ERR_INVALID_ARG, "null output");
+    if (!h->have_forward) return fail(BM_ERR_STATE, "backward needs a forward pass with the current parameters");
+    double g[GN];
+    if (int rc = h->fp64 ? backward_t<double>(h, g) : backward_t<float>(h, g)) return rc;
+    const bm_diff_params& q = h->prm;
+    const double E = q.young_modulus, nu = q.poisson_ratio;
+    const double mu = E / (2.0 * (1.0 + nu)), lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    // koh = (3 lam + 2 mu) / (2 mu)
+    const double gmu = g[GMU] + g[GKOH] * (-3.0 * lam / (2.0 * mu * mu));
+    const double glam = g[GLAM] + g[GKOH] * (3.0 / (2.0 * mu));
+    const double phi = q.friction_deg * M_PI / 180.0, s = std::sin(phi);
+    const double dalpha_dphi_deg = std::sqrt(2.0 / 3.0) * 6.0 * std::cos(phi) / ((3.0 - s) * (3.0 - s)) * M_PI / 180.0;
+    for (int a = 0; a < 3; ++a) out->tool_velocity[a] = g[GVT + a];
+    out->friction_deg = g[GALPHA] * dalpha_dphi_deg;
+    out->young_modulus = gmu / (2.0 * (1.0 + nu)) + glam * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    out->alpha = g[GALPHA];
+    out->mu = gmu;
+    out->lambda = glam;
+    return BM_OK;
+}
+
+int bm_diff_download(bm_diff_handle h, bm_particles* out) {
+    if (int rc = check_diff(h)) return rc;
+    if (!out) return fail(BM_ERR_INVALID_ARG, "null output");
+    if (!h->have_forward) return fail(BM_ERR_STATE, "no forward pass");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return h->fp64 ? download_t<double>(h, out) : download_t<float>(h, out);
+}
+
+}  // extern "C"

@@ -188,6 +188,24 @@ class RequestInfoC(C.Structure):
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
 
+
+class DiffDescC(C.Structure):
+    _fields_ = [("dx", C.c_float), ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_int32 * 3),
+                ("max_particles", C.c_int64), ("precision", C.c_int32), ("checkpoint_every", C.c_int32),
+                ("reference_svd", C.c_int32), ("pad", C.c_int32), ("stream", C.c_void_p)]
+
+
+class DiffParamsC(C.Structure):
+    _fields_ = [("young_modulus", C.c_double), ("poisson_ratio", C.c_double), ("friction_deg", C.c_double),
+                ("floor_friction", C.c_double), ("gravity", C.c_double * 3), ("has_tool", C.c_int32),
+                ("pad", C.c_int32), ("tool_center", C.c_double * 3), ("tool_half_extents", C.c_double * 3),
+                ("tool_quat", C.c_double * 4), ("tool_velocity", C.c_double * 3), ("tool_friction", C.c_double)]
+
+
+class DiffGradC(C.Structure):
+    _fields_ = [("tool_velocity", C.c_double * 3), ("friction_deg", C.c_double), ("young_modulus", C.c_double),
+                ("alpha", C.c_double), ("mu", C.c_double), ("lambda_", C.c_double)]
+
 # ------------------------------------------------------------------ library
 _lib = None
 
@@ -244,6 +262,13 @@ _SIGS = {
     "bm_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "bm_server_load": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64)]),
+    "bm_diff_create": (C.c_int, [C.POINTER(DiffDescC), C.POINTER(C.c_void_p)]),
+    "bm_diff_destroy": (C.c_int, [C.c_void_p]),
+    "bm_diff_set_params": (C.c_int, [C.c_void_p, C.POINTER(DiffParamsC)]),
+    "bm_diff_upload": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(HostParticles), C.POINTER(C.c_float)]),
+    "bm_diff_forward": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_double)]),
+    "bm_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(DiffGradC)]),
+    "bm_diff_download": (C.c_int, [C.c_void_p, C.POINTER(HostParticles)]),
     "bm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "bm_mpm_download_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -852,3 +877,93 @@ class Server:
         q, r, u, b = C.c_int32(), C.c_int32(), C.c_int64(), C.c_int64()
         _check(lib().bm_server_load(self.h, C.byref(q), C.byref(r), C.byref(u), C.byref(b)))
         return q.value, r.value, u.value, b.value
+
+
+# ------------------------------------------------------------------ adjoint
+@dataclass
+class DiffParams:
+    """Parameters of a differentiable rollout (bm_diff_params)."""
+
+    young_modulus: float = 3.5e5
+    poisson_ratio: float = 0.3
+    friction_deg: float = 35.0
+    floor_friction: float = 0.4
+    gravity: tuple = (0.0, -9.81, 0.0)
+    tool: bool = False
+    tool_center: tuple = (0.0, 0.0, 0.0)
+    tool_half_extents: tuple = (0.01, 0.05, 0.075)
+    tool_quat: tuple = (1.0, 0.0, 0.0, 0.0)
+    tool_velocity: tuple = (0.0, 0.0, 0.0)
+    tool_friction: float = 0.5
+
+    def to_c(self) -> DiffParamsC:
+        p = DiffParamsC()
+        p.young_modulus, p.poisson_ratio = float(self.young_modulus), float(self.poisson_ratio)
+        p.friction_deg, p.floor_friction = float(self.friction_deg), float(self.floor_friction)
+        p.gravity[:] = [float(v) for v in self.gravity]
+        p.has_tool = 1 if self.tool else 0
+        p.tool_center[:] = [float(v) for v in self.tool_center]
+        p.tool_half_extents[:] = [float(v) for v in self.tool_half_extents]
+        p.tool_quat[:] = [float(v) for v in self.tool_quat]
+        p.tool_velocity[:] = [float(v) for v in self.tool_velocity]
+        p.tool_friction = float(self.tool_friction)
+        return p
+
+
+class Rollout:
+    """Differentiable rollout with a hand-written adjoint (bm_diff_*)."""
+
+    def __init__(self, dx, origin, dims, max_particles, precision=32, checkpoint_every=16, reference_svd=False,
+                 stream=None):
+        d = DiffDescC()
+        d.dx = float(dx)
+        d.grid_origin[:] = [int(v) for v in origin]
+        d.grid_dims[:] = [int(v) for v in dims]
+        d.max_particles, d.precision, d.checkpoint_every = int(max_particles), int(precision), int(checkpoint_every)
+        d.reference_svd = 1 if reference_svd else 0
+        d.stream = stream
+        h = C.c_void_p()
+        _check(lib().bm_diff_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bm_diff_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p: DiffParams):
+        pc = p.to_c()
+        _check(lib().bm_diff_set_params(self.h, C.byref(pc)))
+
+    def upload(self, P, target=None):
+        n = int(np.asarray(P["x"]).reshape(-1, 3).shape[0])
+        hp, arrs = _host_particles(P)
+        t = None if target is None else _f32(target, -1)
+        _check(lib().bm_diff_upload(self.h, n, C.byref(hp),
+                                    t.ctypes.data_as(C.POINTER(C.c_float)) if t is not None else None))
+        self.n = n
+
+    def forward(self, dt, substeps):
+        loss = C.c_double()
+        _check(lib().bm_diff_forward(self.h, float(dt), int(substeps), C.byref(loss)))
+        return loss.value
+
+    def backward(self):
+        """dict of d loss / d parameter."""
+        g = DiffGradC()
+        _check(lib().bm_diff_backward(self.h, C.byref(g)))
+        return {"tool_velocity": np.array(g.tool_velocity[:]), "friction_deg": g.friction_deg,
+                "young_modulus": g.young_modulus, "alpha": g.alpha, "mu": g.mu, "lambda": g.lambda_}
+
+    def download(self):
+        out = _empty_particles(self.n)
+        hp = _host_particles(out)[0]
+        _check(lib().bm_diff_download(self.h, C.byref(hp)))
+        return out

@@ -212,3 +212,45 @@ def tool_body(tool):
     return Body(position=tool["center"], linear_velocity=(tool["speed"], 0.0, 0.0), inv_mass=1.0 / tool["mass"],
                 linear_damping=0.0, angular_damping=0.0, half_extents=tool["half"], friction=tool["friction"],
                 mass=tool["mass"], shape=SHAPE_BOX, integrate=BODY_KINEMATIC)
+
+
+def tine_scene(seed=0, cells=(32, 4, 32), velocity=None):
+    """The differentiable rollout: a soil bed of cells[0] x cells[1] x
+    cells[2] cells (x, height, z) at dx = 1/32 m, 8 particles per cell
+    (32 x 4 x 32 -> 32,768 particles), in a node box with 4 nodes of margin
+    below and on the sides and room above; a 0.02 x 0.15 x 0.1 m tine
+    (thickness along x, height, width) whose bottom sits 0.05 m below the
+    bed top, dragged along x (velocity from U(0.05, 0.5) m/s by seed unless
+    given) through M0; 400 substeps of 1e-4 s.  The tine is thinner than a
+    cell, so it starts with the node plane x = (i + 1/2) dx just behind its
+    front face; the plane stays inside while the tine moves less than
+    1.95 cm (0.49 m/s over the 400 substeps).  The tine sits 1.1 mm off the
+    bed's mid-plane in z and 0.7 mm deeper than 5 cm, so that no node inside
+    it is equally far from two faces: at such a tie the M0 normal, and with
+    it the loss, jumps under a small change of the tool's path, and central
+    differences there measure the jump.  Material E = 3.5e5 Pa, nu 0.3,
+    friction 35 deg.  The loss target is a seeded affine displacement field
+    of about a millimetre: d(x0) = a + B (x0 - c).
+    Returns a dict with particles, params (DiffParams), target, dx, origin,
+    dims, dt, substeps."""
+    from basements_amd import DiffParams
+
+    dx = 1.0 / 32.0
+    cx, cy, cz = (int(c) for c in cells)
+    P = soil_block(dx, (0, 0, 0), (cx, cy, cz), seed=seed, rho=1600.0)
+    r = _rng(seed + 31_337)
+    v = float(r.uniform(0.05, 0.5)) if velocity is None else float(velocity)
+    top = cy * dx
+    i_plane = max(1, cx // 4)
+    xc = (i_plane + 0.5) * dx - 0.0095
+    prm = DiffParams(young_modulus=3.5e5, poisson_ratio=0.3, friction_deg=35.0, floor_friction=0.4,
+                     tool=True, tool_center=(xc, top - 0.05 + 0.075 - 0.0007, 0.5 * cz * dx + 0.0011),
+                     tool_half_extents=(0.01, 0.075, 0.05), tool_velocity=(v, 0.0, 0.0), tool_friction=0.5)
+    x0 = P["x"].astype(np.float64)
+    c = x0.mean(axis=0)
+    a = r.normal(0.0, 1e-3, size=3)
+    B = r.normal(0.0, 1e-2, size=(3, 3))
+    target = (a[None, :] + (x0 - c) @ B.T).astype(np.float32)
+    dims = (cx + 8, cy + 16, cz + 8)
+    return {"particles": P, "params": prm, "target": target, "dx": dx, "origin": (-4, -4, -4), "dims": dims,
+            "dt": 1e-4, "substeps": 400}

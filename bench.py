@@ -74,13 +74,14 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="wheel_bed", choices=["wheel_bed", "box_drop", "batch", "serve"],
+    ap.add_argument("--config", default="wheel_bed", choices=["wheel_bed", "box_drop", "batch", "serve", "adjoint"],
                     help="wheel_bed (the default workload), box_drop, or batch: 64 uneven tool scenes stepped "
                          "together against the same scenes one at a time on the single-scene path")
     ap.add_argument("--scenes", type=int, default=64, help="scenes of --config batch")
     ap.add_argument("--graph", type=int, default=1, help="--config batch: replay captured substeps")
     ap.add_argument("--serial", type=int, default=1, help="--config batch/serve: also time the work one at a time")
     ap.add_argument("--requests", type=int, default=48, help="--config serve: requests per arrival rate")
+    ap.add_argument("--fd-check", type=int, default=1, help="--config adjoint: FP64 central-difference check")
     ap.add_argument("--rates", default="5,200", help="--config serve: arrival rates (requests/s)")
     ap.add_argument("--serve-particles", type=int, default=16_000_000, help="--config serve: particle room")
     ap.add_argument("--serve-blocks", type=int, default=400_000, help="--config serve: grid-block room")
@@ -270,6 +271,90 @@ def bench_serve(a):
     return 0
 
 
+def bench_adjoint(a):
+    """Differentiable tine rollout: 32 x 4 x 32 cells at dx = 1/32 (32,768
+    particles), 400 substeps, checkpoint every 16; loss = mean squared final
+    displacement against a seeded target field; gradient w.r.t. tool velocity
+    (3), friction angle and Young's modulus.  value = particle-substeps per
+    second through forward + adjoint in FP32.  The check: the FP64 adjoint
+    against FP64 central differences (step 1e-4 relative) and the FP32
+    adjoint against the FP64 one."""
+    import dataclasses
+
+    from basements_amd import Rollout
+    from basements_amd.scenes import tine_scene
+
+    sc = tine_scene(seed=a.seed)
+    n, T, dt = sc["particles"]["x"].shape[0], sc["substeps"], sc["dt"]
+
+    def rollout(prec, prm=None):
+        R = Rollout(sc["dx"], sc["origin"], sc["dims"], n, precision=prec, checkpoint_every=16)
+        R.set_params(prm or sc["params"])
+        R.upload(sc["particles"], sc["target"])
+        return R
+
+    R32 = rollout(32)
+    R32.forward(dt, T)
+    R32.backward()  # warm-up
+    t_f = t_b = 0.0
+    reps = max(1, a.steps // 25)
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        loss32 = R32.forward(dt, T)
+        t1 = time.perf_counter()
+        g32 = R32.backward()
+        t2 = time.perf_counter()
+        t_f += t1 - t0
+        t_b += t2 - t1
+    t_f /= reps
+    t_b /= reps
+    check = None
+    if a.fd_check:
+        R64 = rollout(64)
+        loss64 = R64.forward(dt, T)
+        g64 = R64.backward()
+        p0 = sc["params"]
+
+        def fd(name, idx=None, rel=1e-4):
+            if name == "tool_velocity":
+                h = rel * float(np.linalg.norm(p0.tool_velocity))
+                vp, vm = list(p0.tool_velocity), list(p0.tool_velocity)
+                vp[idx] += h
+                vm[idx] -= h
+                pp = dataclasses.replace(p0, tool_velocity=tuple(vp))
+                pm = dataclasses.replace(p0, tool_velocity=tuple(vm))
+            else:
+                v = getattr(p0, name)
+                h = rel * abs(v)
+                pp, pm = dataclasses.replace(p0, **{name: v + h}), dataclasses.replace(p0, **{name: v - h})
+            return (rollout(64, pp).forward(dt, T) - rollout(64, pm).forward(dt, T)) / (2 * h)
+
+        fdv = [fd("tool_velocity", k) for k in range(3)]
+        fde, fdp = fd("young_modulus"), fd("friction_deg")
+        adj = [*g64["tool_velocity"], g64["young_modulus"], g64["friction_deg"]]
+        fds = [*fdv, fde, fdp]
+        vscale = max(abs(v) for v in fdv)
+        rel = [abs(x - y) / vscale for x, y in zip(adj[:3], fds[:3])] + \
+              [abs(x - y) / abs(y) for x, y in zip(adj[3:], fds[3:])]
+        g32v = [*g32["tool_velocity"], g32["young_modulus"], g32["friction_deg"]]
+        rel32 = [abs(x - y) / vscale for x, y in zip(g32v[:3], adj[:3])] + \
+                [abs(x - y) / abs(y) for x, y in zip(g32v[3:], adj[3:])]
+        check = {"params": ["tool_vx", "tool_vy", "tool_vz", "young_modulus", "friction_deg"],
+                 "adjoint_fp64": adj, "central_fd_fp64": fds, "rel_err_fp64_vs_fd": rel,
+                 "adjoint_fp32": g32v, "rel_err_fp32_vs_fp64": rel32, "loss_fp64": loss64, "loss_fp32": loss32,
+                 "fd_step": "1e-4 relative (tool speed for velocity components)"}
+    value = n * T / (t_f + t_b)
+    out = {"metric": "particle-substeps/s (forward + adjoint, FP32, checkpoint every 16)", "value": value,
+           "unit": "particle-substeps/s", "n_gpus": 1, "steps": reps, "warmup": 1,
+           "ms_per_step": (t_f + t_b) / T * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32", "data": "synthetic (seeded soil bed and target field, no external data)",
+           "config": {"workload": "adjoint", "particles": n, "substeps": T, "dt": dt, "checkpoint_every": 16,
+                      "grid_dims": list(sc["dims"]), "tool_velocity": list(sc["params"].tool_velocity)},
+           "forward_ms": t_f * 1e3, "backward_ms": t_b * 1e3, "check": check}
+    print(json.dumps(out))
+    return 0
+
+
 def bench_batch(a):
     """Batched uneven scenes: aggregate particle-updates/s of all scenes in one
     batch (autotuned, graph-replayed) against running the same scenes one at a
@@ -352,6 +437,8 @@ def main():
         return bench_batch(a)
     if a.config == "serve":
         return bench_serve(a)
+    if a.config == "adjoint":
+        return bench_adjoint(a)
 
     from basements_amd import M0, M1, MODE_FAST, MODE_REFERENCE, Solver
     from basements_amd.scenes import box_drop, wheel_bed

@@ -409,6 +409,63 @@ int bm_server_load(bm_server_handle s, int32_t* queued, int32_t* running, int64_
  * substeps run.  Either may be NULL. */
 int bm_server_stats(bm_server_handle s, double* seconds, int64_t* counts);
 
+/* ---------------------------------------------------------------- adjoint */
+/* Differentiable rollout (no counterpart in the reference, whose autodiff
+ * tape records nothing: mpm_autodiff_tape.h:63-92).  The forward substep is
+ * the reference algorithm (P2G with Hencky-Kirchhoff stress, grid update
+ * with gravity and the floor rule, the M0 clamp of a kinematic box tool,
+ * G2P with the Drucker-Prager return map) in FP32 or FP64 on a dense node
+ * box; the backward pass is its hand-written adjoint, recomputing each run
+ * of checkpoint_every substeps from the state saved at its start.  The loss
+ * is the mean over particles of |x(T) - x(0) - target|^2. */
+typedef struct bm_diff* bm_diff_handle;
+
+typedef struct {
+    float dx;
+    int32_t grid_origin[3];
+    int32_t grid_dims[3];
+    int64_t max_particles;
+    int32_t precision;         /* 32 or 64 */
+    int32_t checkpoint_every;  /* substeps between saved states (0 = 16) */
+    int32_t reference_svd;     /* 1: the reference's SVD stopping rule (for comparing forwards; the
+                                  adjoint needs the default, tighter rule) */
+    int32_t pad;
+    void* stream;              /* hipStream_t; NULL = the handle creates its own */
+} bm_diff_desc;
+
+typedef struct {
+    double young_modulus, poisson_ratio;
+    double friction_deg;       /* Drucker-Prager alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)) */
+    double floor_friction;
+    double gravity[3];
+    int32_t has_tool;          /* a kinematic box tool coupled by M0 */
+    int32_t pad;
+    double tool_center[3];     /* at substep 0; it moves with tool_velocity */
+    double tool_half_extents[3];
+    double tool_quat[4];       /* (w, x, y, z) */
+    double tool_velocity[3];
+    double tool_friction;
+} bm_diff_params;
+
+typedef struct {
+    double tool_velocity[3];   /* d loss / d parameter */
+    double friction_deg;
+    double young_modulus;
+    double alpha, mu, lambda;  /* the same through the solver's own constants */
+} bm_diff_grad;
+
+int bm_diff_create(const bm_diff_desc* desc, bm_diff_handle* out);
+int bm_diff_destroy(bm_diff_handle h);
+int bm_diff_set_params(bm_diff_handle h, const bm_diff_params* params);
+/* Initial particle state and the target displacement (3n floats; NULL = 0). */
+int bm_diff_upload(bm_diff_handle h, int64_t n, const bm_particles* p, const float* target_displacement);
+/* Run `substeps` substeps, saving a state every checkpoint_every; the loss. */
+int bm_diff_forward(bm_diff_handle h, float dt, int32_t substeps, double* loss);
+/* Gradient of the last forward's loss. */
+int bm_diff_backward(bm_diff_handle h, bm_diff_grad* grad);
+/* Final state of the last forward (x, v, F, C; m and V0 are not written). */
+int bm_diff_download(bm_diff_handle h, bm_particles* p);
+
 #ifdef __cplusplus
 }
 #endif

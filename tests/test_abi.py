@@ -80,6 +80,9 @@ int main(void) {
   S(bm_server_desc); F(bm_server_desc, hbm_budget); F(bm_server_desc, dt); F(bm_server_desc, max_bypass);
   S(bm_request); F(bm_request, n); F(bm_request, output); F(bm_request, substeps); F(bm_request, arrival);
   S(bm_request_info); F(bm_request_info, arrival); F(bm_request_info, body); F(bm_request_info, reaction);
+  S(bm_diff_desc); F(bm_diff_desc, max_particles); F(bm_diff_desc, reference_svd); F(bm_diff_desc, stream);
+  S(bm_diff_params); F(bm_diff_params, has_tool); F(bm_diff_params, tool_center); F(bm_diff_params, tool_friction);
+  S(bm_diff_grad); F(bm_diff_grad, young_modulus); F(bm_diff_grad, lambda);
   return 0;
 }
 """
@@ -106,8 +109,10 @@ def test_struct_layouts_match_the_python_mirror(tmp_path):
     py = {"bm_mpm_desc": mpm.Desc, "bm_particles": mpm.HostParticles, "bm_particles_device": mpm.DeviceParticles,
           "bm_collider": mpm.ColliderC, "bm_reaction": mpm.ReactionC, "bm_body": mpm.BodyC,
           "bm_batch_desc": mpm.BatchDescC, "bm_scene_desc": mpm.SceneDescC, "bm_server_desc": mpm.ServerDescC,
-          "bm_request": mpm.RequestC, "bm_request_info": mpm.RequestInfoC}
+          "bm_request": mpm.RequestC, "bm_request_info": mpm.RequestInfoC, "bm_diff_desc": mpm.DiffDescC,
+          "bm_diff_params": mpm.DiffParamsC, "bm_diff_grad": mpm.DiffGradC}
     for (t, m), v in want.items():
+        m = "lambda_" if m == "lambda" else m
         if m == "sizeof":
             assert C.sizeof(py[t]) == v, (t, C.sizeof(py[t]), v)
         else:
