@@ -737,6 +737,11 @@ using namespace bm::dif;
 namespace {
 
 inline int blocks_for(int64_t n) { return (int)((n + 255) / 256); }
+// Particle kernels run 64 lanes per workgroup: a rollout of a few 10^4
+// particles then still reaches every CU (each lane's work -- SVD, stress,
+// 27-node scatter -- is long, so the launch is latency bound).
+constexpr int PWG = 64;
+inline int pblocks(int64_t n) { return (int)((n + PWG - 1) / PWG); }
 
 }  // namespace
 
@@ -823,11 +828,11 @@ template <class R> static void substep(bm_diff_handle h, PS<R> in, PS<R> out, in
     const R dt = (R)h->dt;
     hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
     hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
-    hipLaunchKernelGGL(k_p2g<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, in, (const R*)h->m,
+    hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, (const R*)h->m,
                        (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err);
     hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
                        (const R*)h->gm, (const R*)h->gq, (R*)h->gu);
-    hipLaunchKernelGGL(k_g2p<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, in, out, h->n, h->box, P, dt,
+    hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, out, h->n, h->box, P, dt,
                        (const R*)h->gu, h->err);
 }
 
@@ -877,7 +882,7 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
         cur = nxt;
     }
     HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
-    hipLaunchKernelGGL(k_loss<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, cur, (const R*)h->x0,
+    hipLaunchKernelGGL(k_loss<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, cur, (const R*)h->x0,
                        (const R*)h->tgt, h->n, slot<R>(h->abar, h->cap, 0), h->grad);
     HIP_TRY(hipGetLastError());
     if (int rc = read_err(h)) return rc;
@@ -909,17 +914,17 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
             // the grid of substep s0 + j
             hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
             hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
-            hipLaunchKernelGGL(k_p2g<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, sj, (const R*)h->m,
+            hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
                                (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err);
             hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
                                (const R*)h->gm, (const R*)h->gq, (R*)h->gu);
             hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gub,
                                3 * h->nn);
-            hipLaunchKernelGGL(k_g2p_adj<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, sj, a, a2, h->n,
+            hipLaunchKernelGGL(k_g2p_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, a, a2, h->n,
                                h->box, P, dt, (const R*)h->gu, (R*)h->gub, h->grad);
             hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
                                (const R*)h->gm, (const R*)h->gq, (const R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad);
-            hipLaunchKernelGGL(k_p2g_adj<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, sj, (const R*)h->m,
+            hipLaunchKernelGGL(k_p2g_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
                                (const R*)h->V0, h->n, h->box, P, dt, (const R*)h->gqb, (const R*)h->gmb, a2, h->grad);
             std::swap(a, a2);
         }
