@@ -178,11 +178,12 @@ static int upload_tables(bm_batch_handle b) {
     b->n_chunks = (int)ch.size();
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (ns > 0) {
-        HIP_TRY(hipMemcpy(b->d_sc, sc.data(), sizeof(SceneDev) * ns, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(b->d_has_body, has.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(b->d_methods, meth.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(b->d_sc, sc.data(), sizeof(SceneDev) * ns, hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->d_has_body, has.data(), sizeof(int) * ns, hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->d_methods, meth.data(), sizeof(int) * ns, hipMemcpyHostToDevice, b->stream));
     }
-    if (!ch.empty()) HIP_TRY(hipMemcpy(b->chunks, ch.data(), sizeof(int4) * ch.size(), hipMemcpyHostToDevice));
+    if (!ch.empty())
+        HIP_TRY(hipMemcpyAsync(b->chunks, ch.data(), sizeof(int4) * ch.size(), hipMemcpyHostToDevice, b->stream));
     for (int s = 0; s < ns; ++s) {
         const SceneHost& S = b->scenes[s];
         if (S.nb > 0)
@@ -206,7 +207,7 @@ static int batch_harvest_err(bm_batch_handle b) {
         std::vector<int> w((size_t)ns + 1);
         HIP_TRY(hipMemcpy(w.data(), b->d_err, sizeof(int) * (ns + 1), hipMemcpyDeviceToHost));
         for (int s = 0; s < ns; ++s) b->scenes[s].err |= w[(size_t)s + 1];
-        HIP_TRY(hipMemset(b->d_err, 0, sizeof(int) * (b->capS + 1)));
+        HIP_TRY(hipMemsetAsync(b->d_err, 0, sizeof(int) * (b->capS + 1), b->stream));
     }
     return BM_OK;
 }
@@ -475,14 +476,15 @@ int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, co
         if (!b->stage.ensure(sizeof(float) * PCOMP * n)) return fail(BM_ERR_OOM, "pinned staging");
         float* st = b->stage.f();
         soa_from_particles(p->x, p->v, p->F, p->C, p->m, p->V0, n, st);
-        HIP_TRY(hipMemcpy2D(b->pbuf + S.p0, sizeof(float) * b->capP, st, sizeof(float) * n, sizeof(float) * n, PCOMP,
-                            hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy2DAsync(b->pbuf + S.p0, sizeof(float) * b->capP, st, sizeof(float) * n, sizeof(float) * n,
+                                 PCOMP, hipMemcpyHostToDevice, b->stream));
         hipLaunchKernelGGL(k_iota_from, dim3(grid256(n)), dim3(256), 0, b->stream, b->pid + S.p0, n, S.p0);
         HIP_TRY(hipGetLastError());
     }
     const int s = (int)b->scenes.size();
-    if (body) HIP_TRY(hipMemcpy(b->d_bodies + s, body, sizeof(bm_body), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(b->d_out7 + (int64_t)s * 8, 0, sizeof(double) * 8));
+    if (body) HIP_TRY(hipMemcpyAsync(b->d_bodies + s, body, sizeof(bm_body), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemsetAsync(b->d_out7 + (int64_t)s * 8, 0, sizeof(double) * 8, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));  // the caller's buffers and the staging are free again
     b->scenes.push_back(S);
     b->P += n;
     b->B += S.nb;
@@ -539,8 +541,10 @@ int bm_batch_remove_scene(bm_batch_handle b, int32_t id) {
         HIP_TRY(hipMemcpyAsync(out.data(), b->d_out7 + (int64_t)(s + 1) * 8, sizeof(double) * out.size(),
                                hipMemcpyDeviceToHost, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
-        HIP_TRY(hipMemcpy(b->d_bodies + s, bodies.data(), sizeof(bm_body) * bodies.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(b->d_out7 + (int64_t)s * 8, out.data(), sizeof(double) * out.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(b->d_bodies + s, bodies.data(), sizeof(bm_body) * bodies.size(), hipMemcpyHostToDevice,
+                               b->stream));
+        HIP_TRY(hipMemcpyAsync(b->d_out7 + (int64_t)s * 8, out.data(), sizeof(double) * out.size(),
+                               hipMemcpyHostToDevice, b->stream));
     }
     HIP_TRY(hipStreamSynchronize(b->stream));
     for (int t = s + 1; t < ns; ++t) {
@@ -679,18 +683,22 @@ int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* un
     }
     const size_t pitch = sizeof(float) * b->capP, spitch = sizeof(float) * std::max<int64_t>(b->P, 1);
     auto save = [&]() -> int {
-        HIP_TRY(hipMemcpy2D(sp, spitch, b->pbuf, pitch, sizeof(float) * b->P, PCOMP, hipMemcpyDeviceToDevice));
-        HIP_TRY(hipMemcpy(sid, b->pid, sizeof(int) * b->P, hipMemcpyDeviceToDevice));
-        HIP_TRY(hipMemcpy(sbody, b->d_bodies, sizeof(bm_body) * ns, hipMemcpyDeviceToDevice));
-        HIP_TRY(hipMemcpy(sever, b->bever, b->B, hipMemcpyDeviceToDevice));
+        // stream-ordered: copies on the null stream would not be ordered
+        // with the batch's (non-blocking) stream
+        HIP_TRY(hipMemcpy2DAsync(sp, spitch, b->pbuf, pitch, sizeof(float) * b->P, PCOMP, hipMemcpyDeviceToDevice,
+                                 b->stream));
+        HIP_TRY(hipMemcpyAsync(sid, b->pid, sizeof(int) * b->P, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(sbody, b->d_bodies, sizeof(bm_body) * ns, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(sever, b->bever, b->B, hipMemcpyDeviceToDevice, b->stream));
         return BM_OK;
     };
     auto restore = [&]() -> int {
         HIP_TRY(hipStreamSynchronize(b->stream));
-        HIP_TRY(hipMemcpy2D(b->pbuf, pitch, sp, spitch, sizeof(float) * b->P, PCOMP, hipMemcpyDeviceToDevice));
-        HIP_TRY(hipMemcpy(b->pid, sid, sizeof(int) * b->P, hipMemcpyDeviceToDevice));
-        HIP_TRY(hipMemcpy(b->d_bodies, sbody, sizeof(bm_body) * ns, hipMemcpyDeviceToDevice));
-        HIP_TRY(hipMemcpy(b->bever, sever, b->B, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy2DAsync(b->pbuf, pitch, sp, spitch, sizeof(float) * b->P, PCOMP, hipMemcpyDeviceToDevice,
+                                 b->stream));
+        HIP_TRY(hipMemcpyAsync(b->pid, sid, sizeof(int) * b->P, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->d_bodies, sbody, sizeof(bm_body) * ns, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->bever, sever, b->B, hipMemcpyDeviceToDevice, b->stream));
         b->invalidate();
         return BM_OK;
     };

@@ -848,7 +848,7 @@ static int read_err(bm_diff_handle h) {
     int e = 0;
     HIP_TRY(hipMemcpy(&e, h->err, sizeof(int), hipMemcpyDeviceToHost));
     if (e) {
-        HIP_TRY(hipMemset(h->err, 0, sizeof(int)));
+        HIP_TRY(hipMemsetAsync(h->err, 0, sizeof(int), h->stream));
         return fail(BM_ERR_OUT_OF_DOMAIN, "a particle stencil left the node box");
     }
     return BM_OK;
@@ -970,12 +970,15 @@ template <class R> static int upload_t(bm_diff_handle h, int64_t n, const bm_par
         mm[(size_t)i] = (R)p->m[i];
         vv[(size_t)i] = (R)p->V0[i];
     }
-    HIP_TRY(hipMemcpy2D(h->init, sizeof(R) * h->cap, st.data(), sizeof(R) * n, sizeof(R) * n, QCOMP,
-                        hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->m, mm.data(), sizeof(R) * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->V0, vv.data(), sizeof(R) * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->x0, x0.data(), sizeof(R) * 3 * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->tgt, tg.data(), sizeof(R) * 3 * n, hipMemcpyHostToDevice));
+    // stream-ordered with the handle's (non-blocking) stream; the host
+    // vectors live until the synchronisation
+    HIP_TRY(hipMemcpy2DAsync(h->init, sizeof(R) * h->cap, st.data(), sizeof(R) * n, sizeof(R) * n, QCOMP,
+                             hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->m, mm.data(), sizeof(R) * n, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->V0, vv.data(), sizeof(R) * n, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->x0, x0.data(), sizeof(R) * 3 * n, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->tgt, tg.data(), sizeof(R) * 3 * n, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return BM_OK;
 }
 

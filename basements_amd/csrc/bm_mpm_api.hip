@@ -206,8 +206,7 @@ static int sync_and_check(bm_mpm_handle h) {
     int err = 0;
     HIP_TRY(hipMemcpy(&err, h->d_err, sizeof(int), hipMemcpyDeviceToHost));
     if (err) {
-        int zero = 0;
-        HIP_TRY(hipMemcpy(h->d_err, &zero, sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(h->d_err, 0, sizeof(int), h->stream));  // ordered with the handle's stream
         if (err & 1) return fail(BM_ERR_OUT_OF_DOMAIN, "a particle stencil left the grid domain");
         return fail(BM_ERR_HIP, "device body collider exceeded its scan-range bound");
     }
@@ -824,9 +823,9 @@ int bm_mpm_upload_grid(bm_mpm_handle h, const int32_t org[3], const int32_t dims
             hipFree(d_act);
             return fail(BM_ERR_OOM, "grid upload staging");
         }
-        hipError_t e = hipMemcpy(d_mass, mass, 4 * tot, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(d_vel, vel, 12 * tot, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(d_act, active, tot, hipMemcpyHostToDevice);
+        hipError_t e = hipMemcpyAsync(d_mass, mass, 4 * tot, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_vel, vel, 12 * tot, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_act, active, tot, hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) {
             launch_grid_scatter(g, h->bever, h->dom, org, dims, h->qscale_inv(), d_mass, d_vel, d_act, h->stream);
             e = hipStreamSynchronize(h->stream);

@@ -196,6 +196,9 @@ void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phy
 // the FP64 partial sums -- and the reaction -- do not depend on how the scan
 // range was bounded (host-built or device-built collider).
 constexpr int COUPLE_WGS = 128;
+// Coupler workgroups per scene of a batch: a tool's scan range is a few
+// blocks, so most of COUPLE_WGS would find nothing to do in every scene.
+constexpr int COUPLE_BATCH_WGS = 16;
 void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* bever, const Domain& d, float dt,
                    double* partials, double* out7, hipStream_t s);
 // Coupled substep of the device body: applied loads, collider from the body

@@ -690,19 +690,19 @@ __global__ void __launch_bounds__(256) k_couple_batch(const int* methods, const 
     const int s = blockIdx.y;
     if (!has_body[s]) return;
     const ColliderDev b = cols[s];
-    couple_wg(methods[s], b, g, bever, sv.sc[s].d, dt, partials + (int64_t)s * COUPLE_WGS * 7);
+    couple_wg(methods[s], b, g, bever, sv.sc[s].d, dt, partials + (int64_t)s * COUPLE_BATCH_WGS * 7);
 }
 __global__ void __launch_bounds__(64) k_body_end_batch(bm_body* bodies, const int* has_body, const double* partials,
                                                        double* out7, float dt) {
     const int s = blockIdx.x;
     if (!has_body[s]) return;
-    body_end(bodies + s, partials + (int64_t)s * COUPLE_WGS * 7, COUPLE_WGS, out7 + (int64_t)s * 8, dt);
+    body_end(bodies + s, partials + (int64_t)s * COUPLE_BATCH_WGS * 7, COUPLE_BATCH_WGS, out7 + (int64_t)s * 8, dt);
 }
 void launch_couple_batch(const int* methods, const int* has_body, bm_body* bodies, const ColliderDev* cols,
                          const GridBuf& g, uint8_t* bever, SceneView sv, float dt, double* partials, double* out7,
                          hipStream_t s) {
     if (sv.n_scenes <= 0 || !(dt > 0.0f)) return;
-    hipLaunchKernelGGL(k_couple_batch, dim3(COUPLE_WGS, sv.n_scenes), dim3(256), 0, s, methods, has_body, cols, g,
+    hipLaunchKernelGGL(k_couple_batch, dim3(COUPLE_BATCH_WGS, sv.n_scenes), dim3(256), 0, s, methods, has_body, cols, g,
                        bever, sv, dt, partials);
     hipLaunchKernelGGL(k_body_end_batch, dim3(sv.n_scenes), dim3(64), 0, s, bodies, has_body,
                        (const double*)partials, out7, dt);

@@ -377,7 +377,7 @@ def bench_batch(a):
         unit, sort_every, groups = (int(t) for t in a.tuning.split(","))
         B.set_tuning(unit, sort_every, groups)
     else:
-        unit, sort_every, groups = B.autotune(dt, substeps=32)
+        unit, sort_every, groups = B.autotune(dt, substeps=64)
     B.set_graph(bool(a.graph))
     # warm-up long enough to capture the replayed substeps (two re-binning cycles)
     warmup = max(a.warmup, 3 * sort_every)
