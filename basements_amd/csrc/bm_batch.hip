@@ -214,11 +214,13 @@ static int batch_harvest_err(bm_batch_handle b) {
 
 static int scene_err_code(int bits) {
     if (bits & 1) return BM_ERR_OUT_OF_DOMAIN;
+    if (bits & 8) return BM_ERR_NONFINITE;
     return bits ? BM_ERR_HIP : BM_OK;
 }
 
 static int scene_fail(const SceneHost& S) {
     if (S.err & 1) return fail(BM_ERR_OUT_OF_DOMAIN, "scene %d: a particle stencil left the scene's grid box", S.id);
+    if (S.err & 8) return fail(BM_ERR_NONFINITE, "scene %d: a particle's position or velocity became non-finite", S.id);
     return fail(BM_ERR_HIP, "scene %d: device error flag %d", S.id, S.err);
 }
 
@@ -475,7 +477,8 @@ int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, co
         // component-major pinned staging, then one pitched copy into the arena
         if (!b->stage.ensure(sizeof(float) * PCOMP * n)) return fail(BM_ERR_OOM, "pinned staging");
         float* st = b->stage.f();
-        soa_from_particles(p->x, p->v, p->F, p->C, p->m, p->V0, n, st);
+        if (!soa_from_particles(p->x, p->v, p->F, p->C, p->m, p->V0, n, st))
+            return fail(BM_ERR_NONFINITE, "particle state holds a non-finite value");
         HIP_TRY(hipMemcpy2DAsync(b->pbuf + S.p0, sizeof(float) * b->capP, st, sizeof(float) * n, sizeof(float) * n,
                                  PCOMP, hipMemcpyHostToDevice, b->stream));
         hipLaunchKernelGGL(k_iota_from, dim3(grid256(n)), dim3(256), 0, b->stream, b->pid + S.p0, n, S.p0);

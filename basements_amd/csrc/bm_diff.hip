@@ -1065,6 +1065,13 @@ int bm_diff_upload(bm_diff_handle h, int64_t n, const bm_particles* p, const flo
     if (n <= 0 || !p || !p->x || !p->v || !p->F || !p->C || !p->m || !p->V0)
         return fail(BM_ERR_INVALID_ARG, "bad particle arrays");
     if (n > h->cap) return fail(BM_ERR_OOM, "capacity %lld exceeded", (long long)h->cap);
+    for (int64_t i = 0; i < n; ++i) {
+        const float sc = 0x1p-100f;  // NaN and Inf survive the sum; finite values cannot overflow it
+        float acc = p->m[i] * sc + p->V0[i] * sc;
+        for (int k = 0; k < 3; ++k) acc += p->x[3 * i + k] * sc + p->v[3 * i + k] * sc;
+        for (int k = 0; k < 9; ++k) acc += p->F[9 * i + k] * sc + p->C[9 * i + k] * sc;
+        if (!std::isfinite(acc)) return fail(BM_ERR_NONFINITE, "particle %lld holds a non-finite value", (long long)i);
+    }
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->n = n;
     h->have_forward = false;

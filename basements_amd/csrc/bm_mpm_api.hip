@@ -208,6 +208,7 @@ static int sync_and_check(bm_mpm_handle h) {
     if (err) {
         HIP_TRY(hipMemsetAsync(h->d_err, 0, sizeof(int), h->stream));  // ordered with the handle's stream
         if (err & 1) return fail(BM_ERR_OUT_OF_DOMAIN, "a particle stencil left the grid domain");
+        if (err & 8) return fail(BM_ERR_NONFINITE, "a particle's position or velocity became non-finite");
         return fail(BM_ERR_HIP, "device body collider exceeded its scan-range bound");
     }
     return BM_OK;
@@ -392,7 +393,8 @@ static int upload_range(bm_mpm_handle h, int64_t off, int64_t n, const float* x,
                         const float* Cm, const float* m, const float* V0) {
     if (!h->stage.ensure(sizeof(float) * PCOMP * n)) return fail(BM_ERR_OOM, "pinned staging");
     float* st = h->stage.f();
-    soa_from_particles(x, v, F, Cm, m, V0, n, st);
+    if (!soa_from_particles(x, v, F, Cm, m, V0, n, st))
+        return fail(BM_ERR_NONFINITE, "particle state holds a non-finite value");
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMemcpy2DAsync(h->pbuf + off, sizeof(float) * h->cap, st, sizeof(float) * n, sizeof(float) * n, PCOMP,
                              hipMemcpyHostToDevice, h->stream));

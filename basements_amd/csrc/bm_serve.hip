@@ -105,10 +105,19 @@ static int admit(bm_server_handle s) {
         if (s->in_use + R.bytes <= s->budget) {
             rc = bm_batch_add_scene(s->batch, &R.req.scene, R.req.n, R.req.input, R.has_body ? &R.body : nullptr,
                                     &R.scene);
-            if (rc != BM_OK && rc != BM_ERR_OOM) return rc;
+            if (rc == BM_ERR_HIP) return rc;
         }
         if (rc == BM_ERR_OOM) {  // no room now: wait (the request fits an empty server)
             ++qi;
+            continue;
+        }
+        if (rc != BM_OK) {  // the request itself is bad (non-finite state, bad scene): it fails alone
+            R.info.state = BM_REQ_FAILED;
+            R.info.status = rc;
+            R.info.finished = s->now();
+            R.scene = -1;
+            s->done.push_back(R.id);
+            s->queue.erase(s->queue.begin() + (std::ptrdiff_t)qi);
             continue;
         }
         R.info.state = BM_REQ_RUNNING;

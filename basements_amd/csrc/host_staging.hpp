@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdint>
 #include <thread>
 #include <vector>
@@ -66,9 +67,12 @@ struct PinnedStage {
 };
 
 // Caller layout -> component-major st[c * n + i].  Null fields are skipped.
-inline void soa_from_particles(const float* x, const float* v, const float* F, const float* Cm, const float* m,
+// Returns false when any value is not finite.
+inline bool soa_from_particles(const float* x, const float* v, const float* F, const float* Cm, const float* m,
                                const float* V0, int64_t n, float* st) {
+    std::atomic<bool> bad{false};
     parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+        float acc = 0.0f;  // NaN and Inf survive the sum (scaled so finite values cannot overflow it)
         for (int64_t i = lo; i < hi; ++i) {
             for (int k = 0; k < 3; ++k) {
                 if (x) st[(PX + k) * n + i] = x[3 * i + k];
@@ -81,7 +85,14 @@ inline void soa_from_particles(const float* x, const float* v, const float* F, c
             if (m) st[PM * n + i] = m[i];
             if (V0) st[PV0 * n + i] = V0[i];
         }
+        for (int64_t i = lo; i < hi; ++i)
+            for (int c = 0; c < PCOMP; ++c) {
+                const bool present = (c < PV ? x : c < PF ? v : c < PC ? F : c < PM ? Cm : c == PM ? m : V0) != nullptr;
+                if (present) acc += st[c * n + i] * 0x1p-100f;
+            }
+        if (!std::isfinite(acc)) bad.store(true, std::memory_order_relaxed);
     });
+    return !bad.load();
 }
 
 // Component-major st -> caller layout; slot i holds particle order[i] - base

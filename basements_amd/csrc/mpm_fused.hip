@@ -126,6 +126,7 @@ struct Fused {
                 y = 0.0f;
                 vel.y = 0.0f;
             }
+            check_finite(err, -1, x, y, z, vel.x, vel.y, vel.z);
             P.p[(PX + 0) * P.cap + i] = x;
             P.p[(PX + 1) * P.cap + i] = y;
             P.p[(PX + 2) * P.cap + i] = z;

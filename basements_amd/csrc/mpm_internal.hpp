@@ -175,6 +175,12 @@ __device__ __forceinline__ void flag_error(int* err, int scene, int code) {
     atomicOr(err, code);
     if (scene >= 0) atomicOr(err + 1 + scene, code);
 }
+// Error codes: 1 a stencil left the box, 4 a collider outgrew its scan
+// bound, 8 a particle's new position or velocity is not finite.
+__device__ __forceinline__ void check_finite(int* err, int scene, float x, float y, float z, float vx, float vy,
+                                             float vz) {
+    if (!isfinite(((x + y) + (z + vx)) + (vy + vz))) flag_error(err, scene, 8);
+}
 
 // ------------------------------------------------------------- launchers
 // Zero the blocks on g's list and empty the list; also zeroes *zero_word

@@ -274,6 +274,7 @@ __global__ void __launch_bounds__(256) k_g2p_ref(PView P, GridBuf g, Domain d, P
         y = 0.0f;
         v.y = 0.0f;
     }
+    check_finite(err, -1, x, y, z, v.x, v.y, v.z);
     P.p[(PX + 0) * P.cap + i] = x;
     P.p[(PX + 1) * P.cap + i] = y;
     P.p[(PX + 2) * P.cap + i] = z;

@@ -126,6 +126,7 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
             y = 0.0f;
             vel.y = 0.0f;
         }
+        check_finite(err, scene, x, y, z, vel.x, vel.y, vel.z);
         P.p[(PX + 0) * P.cap + i] = x;
         P.p[(PX + 1) * P.cap + i] = y;
         P.p[(PX + 2) * P.cap + i] = z;
@@ -159,12 +160,13 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
 // The FP64 pass for the particles k_advance listed: same outputs.
 template <bool DO_G2P>
 __global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domain d0, Phys ph0, FastArgs fa, float4* rec,
-                                                     unsigned long long* counters, SceneView sv) {
+                                                     int* err, unsigned long long* counters, SceneView sv) {
     const int cnt = *fa.n_redo;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
         const int64_t i = fa.redo[r];
         Domain d = d0;
         Phys ph = ph0;
+        int scene = -1;
         if (sv.sc) {  // the scene whose particle range holds i
             int lo = 0, hi = sv.n_scenes - 1;
             while (lo < hi) {
@@ -175,6 +177,7 @@ __global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domai
             d = sv.sc[lo].d;
             ph = sv.sc[lo].ph;
             fa.qscale_inv = sv.sc[lo].qscale_inv;
+            scene = lo;
         }
         const float dx = d.dx, inv_dx = d.inv_dx;
         float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
@@ -207,6 +210,7 @@ __global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domai
                 y = 0.0f;
                 vel.y = 0.0f;
             }
+            check_finite(err, sv.sc ? scene : -1, x, y, z, vel.x, vel.y, vel.z);
             P.p[(PX + 0) * P.cap + i] = x;
             P.p[(PX + 1) * P.cap + i] = y;
             P.p[(PX + 2) * P.cap + i] = z;
@@ -538,9 +542,10 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
     }
     if (mark) mark(ctx, SPLIT_REDO, nullptr, nullptr);
     if (do_g2p)
-        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters, sv);
+        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, err, counters, sv);
     else
-        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, counters, sv);
+        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, err, counters,
+                           sv);
     e0 = e1 = nullptr;
     if (mark) mark(ctx, SPLIT_DEPOSIT, &e0, &e1);
     // a batch walks the device-counted units with a fixed grid

@@ -141,3 +141,18 @@ def test_failing_request_leaves_the_others_running():
     inf = S.info(failing)
     assert inf.state == REQ_FAILED and inf.status == 6
     _check_alone(S, good, scenes[0], 12, "request beside a failing one")
+
+
+def test_request_with_non_finite_state_fails_alone():
+    from basements_amd import REQ_FAILED
+
+    scenes = _scenes()
+    S = _server(max_chunk=4)
+    bad = {k: np.array(v, copy=True) for k, v in scenes[2][0].items()}
+    bad["v"][5] = (np.nan, 0.0, 0.0)
+    good = S.submit(scenes[0][2], scenes[0][0], 8, body=scenes[0][3])
+    failing = S.submit(scenes[2][2], bad, 8)
+    _drain(S, [good, failing])
+    inf = S.info(failing)
+    assert inf.state == REQ_FAILED and inf.status == 4  # BM_ERR_NONFINITE
+    _check_alone(S, good, scenes[0], 8, "request beside a non-finite one")
