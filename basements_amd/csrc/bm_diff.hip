@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -271,39 +272,127 @@ template <class R> __device__ __forceinline__ void wave_add(double* dst, R v) {
     if ((threadIdx.x & 63) == 0 && x != 0.0) atomicAdd(dst, x);
 }
 
+// ------------------------------------------------------------------ scatter tile
+// The scatter kernels run one wave per workgroup over consecutive particles,
+// which sit close together: the wave's deposits are summed in an LDS tile
+// over the nodes its stencils cover and flushed with one global atomic per
+// node and value, instead of 27 contended global atomics per particle.  A
+// wave whose nodes do not fit the tile deposits directly.
+constexpr int STILE = 1024;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+struct TileBox {
+    int lo[3], ex[3];
+    bool on;
+    __device__ __forceinline__ int at(int i, int j, int k) const {
+        return ((i - lo[0]) * ex[1] + (j - lo[1])) * ex[2] + (k - lo[2]);
+    }
+};
+// The box of nodes the live lanes' stencils (base .. base + 2) cover.
+__device__ __forceinline__ TileBox tile_box(bool live, const int base[3]) {
+    TileBox t;
+    int vol = 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        t.lo[a] = wave_min_i(live ? base[a] : INT_MAX);
+        const int hi = wave_max_i(live ? base[a] + 2 : INT_MIN);
+        t.ex[a] = hi - t.lo[a] + 1;
+        vol = (t.ex[a] > 0 && vol <= STILE) ? vol * t.ex[a] : STILE + 1;
+    }
+    t.on = vol <= STILE;
+    return t;
+}
+
+// NV values per node from the tile to global memory (dst[v] + NV * id + ...):
+// out-of-box nodes raise the domain error.
+template <class R, int NV>
+__device__ __forceinline__ void tile_flush(const TileBox& t, R (*tl)[STILE], const Box& b, R* g0, R* g1, int* err) {
+    const int vol = t.ex[0] * t.ex[1] * t.ex[2];
+    for (int e = threadIdx.x; e < vol; e += 64) {
+        bool any = false;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) any = any || tl[v][e] != R(0);
+        if (!any) continue;
+        const int k = e % t.ex[2], j = (e / t.ex[2]) % t.ex[1], i = e / (t.ex[2] * t.ex[1]);
+        const int64_t id = node_index<R>(b, t.lo[0] + i, t.lo[1] + j, t.lo[2] + k);
+        if (id < 0) {
+            if (err) atomicOr(err, 1);
+            continue;
+        }
+        if (g0) atomicAdd(g0 + id, tl[0][e]);
+#pragma unroll
+        for (int v = (g0 ? 1 : 0); v < NV; ++v) atomicAdd(g1 + 3 * id + (v - (g0 ? 1 : 0)), tl[v][e]);
+    }
+}
+
 // ------------------------------------------------------------------ forward
 template <class R>
-__global__ void __launch_bounds__(256) k_p2g(PS<R> s, const R* m, const R* V0, int64_t n, Box b, Par<R> P, R dt,
-                                             R* gm, R* gq, int* err) {
+__global__ void __launch_bounds__(64) k_p2g(PS<R> s, const R* m, const R* V0, int64_t n, Box b, Par<R> P, R dt,
+                                            R* gm, R* gq, int* err) {
+    __shared__ R tl[4][STILE];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
-    const v3<R> v = mk(s.at(QV, p), s.at(QV + 1, p), s.at(QV + 2, p));
-    const m3<R> C = load_m(s, QC, p), F = load_m(s, QF, p);
-    const R mp = m[p];
-    const R kf = -R(4) * P.inv_dx * P.inv_dx * V0[p];
-    m3<R> K = stress(F, P.mu, P.lam, P.ref_svd);
+    const bool live = p < n;
+    R x[3] = {R(0), R(0), R(0)};
+    St<R> st{};
+    if (live) {
+        x[0] = s.at(QX, p);
+        x[1] = s.at(QX + 1, p);
+        x[2] = s.at(QX + 2, p);
+        stencil(x, P.inv_dx, st);
+    }
+    const TileBox tb = tile_box(live, st.b);
+    if (tb.on) {
+        for (int e = threadIdx.x; e < STILE; e += 64) tl[0][e] = tl[1][e] = tl[2][e] = tl[3][e] = R(0);
+        __syncthreads();
+    }
+    if (live) {
+        const v3<R> v = mk(s.at(QV, p), s.at(QV + 1, p), s.at(QV + 2, p));
+        const m3<R> C = load_m(s, QC, p), F = load_m(s, QF, p);
+        const R mp = m[p];
+        const R kf = -R(4) * P.inv_dx * P.inv_dx * V0[p];
+        m3<R> K = stress(F, P.mu, P.lam, P.ref_svd);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) K.a[c] *= kf;
-    St<R> st;
-    stencil(x, P.inv_dx, st);
-    for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 9; ++c) K.a[c] *= kf;
+        for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             for (int k = 0; k < 3; ++k) {
                 const R w = st.w[0][i] * st.w[1][j] * st.w[2][k];
                 const int ni = st.b[0] + i, nj = st.b[1] + j, nk = st.b[2] + k;
+                const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
+                const v3<R> a = (v + mvec(C, d)) * mp + mvec(K, d) * dt;
+                if (tb.on) {
+                    const int e = tb.at(ni, nj, nk);
+                    atomicAdd(&tl[0][e], w * mp);
+                    atomicAdd(&tl[1][e], w * a.x);
+                    atomicAdd(&tl[2][e], w * a.y);
+                    atomicAdd(&tl[3][e], w * a.z);
+                    continue;
+                }
                 const int64_t id = node_index<R>(b, ni, nj, nk);
                 if (id < 0) {
                     atomicOr(err, 1);
                     continue;
                 }
-                const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
-                const v3<R> a = (v + mvec(C, d)) * mp + mvec(K, d) * dt;
                 atomicAdd(gm + id, w * mp);
                 atomicAdd(gq + 3 * id + 0, w * a.x);
                 atomicAdd(gq + 3 * id + 1, w * a.y);
                 atomicAdd(gq + 3 * id + 2, w * a.z);
             }
+    }
+    if (tb.on) {
+        __syncthreads();
+        tile_flush<R, 4>(tb, tl, b, gm, gq, err);
+    }
 }
 
 // The tool's probe at a node: inside flag and outward world normal.
@@ -498,10 +587,25 @@ __global__ void __launch_bounds__(256) k_loss(PS<R> fin, const R* x0, const R* t
 // the old state (aout; its v and C parts are zeroed for the P2G adjoint) and
 // the adjoint of the node velocities (gub, accumulated).
 template <class R>
-__global__ void __launch_bounds__(256) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, int64_t n, Box b, Par<R> P, R dt,
-                                                 const R* gu, R* gub, double* grad) {
+__global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, int64_t n, Box b, Par<R> P, R dt,
+                                                const R* gu, R* gub, double* grad) {
+    __shared__ R tl[3][STILE];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     R galpha = R(0), gkoh = R(0);
+    int base[3] = {0, 0, 0};
+    if (p < n) {
+        const R xs[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
+        St<R> st;
+        stencil(xs, P.inv_dx, st);
+        base[0] = st.b[0];
+        base[1] = st.b[1];
+        base[2] = st.b[2];
+    }
+    const TileBox tb = tile_box(p < n, base);
+    if (tb.on) {
+        for (int e = threadIdx.x; e < STILE; e += 64) tl[0][e] = tl[1][e] = tl[2][e] = R(0);
+        __syncthreads();
+    }
     if (p < n) {
         const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
         const m3<R> F = load_m(s, QF, p);
@@ -553,9 +657,16 @@ __global__ void __launch_bounds__(256) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout,
                     const v3<R> Cd = mvec(Cb, d);
                     // v' = sum w u;  C' = kC sum w u d^T
                     const v3<R> ub = (vb + Cd * kC) * w;
-                    atomicAdd(gub + 3 * id + 0, ub.x);
-                    atomicAdd(gub + 3 * id + 1, ub.y);
-                    atomicAdd(gub + 3 * id + 2, ub.z);
+                    if (tb.on) {
+                        const int e = tb.at(ni, nj, nk);
+                        atomicAdd(&tl[0][e], ub.x);
+                        atomicAdd(&tl[1][e], ub.y);
+                        atomicAdd(&tl[2][e], ub.z);
+                    } else {
+                        atomicAdd(gub + 3 * id + 0, ub.x);
+                        atomicAdd(gub + 3 * id + 1, ub.y);
+                        atomicAdd(gub + 3 * id + 2, ub.z);
+                    }
                     const R su = dot(u, vb) + kC * dot(u, Cd);
                     const v3<R> CTu = mk(Cb.a[0] * u.x + Cb.a[3] * u.y + Cb.a[6] * u.z,
                                          Cb.a[1] * u.x + Cb.a[4] * u.y + Cb.a[7] * u.z,
@@ -569,6 +680,10 @@ __global__ void __launch_bounds__(256) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout,
 #pragma unroll
         for (int c = 0; c < 9; ++c) aout.at(QC + c, p) = R(0);
         store_m(aout, QF, p, Fb);
+    }
+    if (tb.on) {
+        __syncthreads();
+        tile_flush<R, 3>(tb, tl, b, (R*)nullptr, gub, (int*)nullptr);
     }
     wave_add(grad + GALPHA, galpha);
     wave_add(grad + GKOH, gkoh);
