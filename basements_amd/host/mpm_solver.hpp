@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -136,6 +137,18 @@ inline void check(int rc) {
     if (rc != BM_OK) throw Error(rc, std::string("libbm_mpm: ") + bm_last_error());
 }
 
+// The reference's tape (mpm_autodiff_tape.h:65-92) records nothing yet, so a
+// caller's tape stays empty here too; gradients come from bm_diff_* instead.
+class AutodiffTape {
+public:
+    AutodiffTape() = default;
+    explicit AutodiffTape(std::size_t) {}
+    void clear() {}
+    std::size_t size() const { return 0; }
+    bool empty() const { return true; }
+    std::size_t bytes() const { return 0; }
+};
+
 class MPMSolver {
    public:
     explicit MPMSolver(int mode = BM_MODE_REFERENCE, int margin_cells = 16) : mode_(mode), margin_(margin_cells) {
@@ -184,6 +197,12 @@ class MPMSolver {
         check(bm_mpm_step(h_, dt));
         host_valid_ = false;
         if (view_out_) materialize();
+    }
+    // step(dt, tape) (mpm_solver.h:76-85): the tape does not change the
+    // forward step and records nothing.
+    void step(float dt, AutodiffTape* tape) {
+        (void)tape;
+        step(dt);
     }
     // Split phases and the couplers (the reference keeps the phases private).
     void p2g(float dt) {

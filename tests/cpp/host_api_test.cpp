@@ -1,12 +1,14 @@
 // host_api_test.cpp -- the reference's MPM unit tests, restated against the
 // C++ host mirror (basements_amd/host/mpm_solver.hpp) so they read as the
 // reference's own: tests/test_mpm_constitutive.cpp:58-159,
-// tests/test_mpm_stress_isolation.cpp:24-118, tests/test_mpm_cpu.cpp:8-64.
+// tests/test_mpm_stress_isolation.cpp:24-118, tests/test_mpm_cpu.cpp:8-64,
+// tests/test_mpm_autodiff_tape_neutral.cpp:78-130.
 // Needs a GPU.  Exit status 0 when every check passes; one line per test.
 #include <mpm_solver.hpp>
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <functional>
 #include <string>
 #include <vector>
@@ -219,6 +221,50 @@ void MirrorBookkeeping() {
     CHECK(solver.get_particles().size() == 25);
 }
 
+// tests/test_mpm_autodiff_tape_neutral.cpp:78-130: step(dt), step(dt, nullptr)
+// and step(dt, &tape) give the same particles, and the tape stays empty.
+void TapeNeutral() {
+    auto build = [](MPMSolver& s) {
+        s.initialize(0.1f);
+        s.set_material(1.0e5f, 0.3f, 1500.0f);
+        s.set_plastic_alpha(0.4f);
+        s.set_gravity(Vec3(0, -9.81f, 0));
+        s.add_particle(Vec3(0.20f, 0.50f, 0.20f), Vec3(0, 0, 0), 0.01f);
+        s.add_particle(Vec3(0.30f, 0.55f, 0.20f), Vec3(0, 0, 0), 0.01f);
+        s.add_particle(Vec3(0.20f, 0.50f, 0.30f), Vec3(0, 0, 0), 0.01f);
+        s.add_particle(Vec3(0.30f, 0.55f, 0.30f), Vec3(0.01f, 0, 0), 0.01f);
+    };
+    auto same = [](const std::vector<Particle>& a, const std::vector<Particle>& b) {
+        if (a.size() != b.size()) return false;
+        auto eq = [](float x, float y) { return std::memcmp(&x, &y, sizeof(float)) == 0; };
+        for (size_t i = 0; i < a.size(); ++i) {
+            if (!eq(a[i].position.x, b[i].position.x) || !eq(a[i].position.y, b[i].position.y) ||
+                !eq(a[i].position.z, b[i].position.z) || !eq(a[i].velocity.x, b[i].velocity.x) ||
+                !eq(a[i].velocity.y, b[i].velocity.y) || !eq(a[i].velocity.z, b[i].velocity.z))
+                return false;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c)
+                    if (!eq(a[i].F.m[r][c], b[i].F.m[r][c]) || !eq(a[i].C.m[r][c], b[i].C.m[r][c])) return false;
+        }
+        return true;
+    };
+    MPMSolver a, b, c;
+    build(a);
+    build(b);
+    build(c);
+    AutodiffTape tape;
+    const float dt = 1.0f / 480.0f;
+    bool ok = true;
+    for (int i = 0; i < 50 && ok; ++i) {
+        a.step(dt);
+        b.step(dt, nullptr);
+        c.step(dt, &tape);
+        ok = same(a.get_particles(), b.get_particles()) && same(a.get_particles(), c.get_particles());
+    }
+    CHECK(ok);
+    CHECK(tape.size() == 0);
+}
+
 }  // namespace
 
 int main() {
@@ -231,6 +277,7 @@ int main() {
         {"SandCollapseInit", SandCollapseInit},
         {"BoundaryCollision", BoundaryCollision},
         {"MirrorBookkeeping", MirrorBookkeeping},
+        {"TapeNeutral", TapeNeutral},
     };
     for (const auto& t : tests) {
         g_test = t.first;

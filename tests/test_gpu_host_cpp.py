@@ -20,5 +20,5 @@ def test_cpp_host_mirror_runs_the_reference_unit_tests():
     r = subprocess.run([BIN], capture_output=True, text=True, timeout=110)
     print(r.stdout)
     lines = [ln for ln in r.stdout.splitlines() if ln.startswith(("PASS", "FAIL"))]
-    assert len(lines) == 8, r.stdout + r.stderr
+    assert len(lines) == 9, r.stdout + r.stderr
     assert r.returncode == 0, r.stdout + r.stderr
