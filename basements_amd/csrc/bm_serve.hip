@@ -238,36 +238,34 @@ int bm_server_submit(bm_server_handle s, const bm_request* req, int64_t* id) {
 int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int32_t* n_done) {
     if (int rc = check_server(s)) return rc;
     if (capacity < 0 || (capacity > 0 && !done_ids)) return fail(BM_ERR_INVALID_ARG, "bad done-id buffer");
-    if (s->done.empty()) {
-        double t = s->now();
+    double t = s->now();
+    if (int rc = admit(s)) return rc;
+    if (s->running.empty() && !s->queue.empty() && s->done.empty()) {
+        // idle: wait for the next arrival, then admit it
+        const double wait = s->reqs[s->queue.front()].info.arrival - s->now();
+        const double t1 = s->now();
+        if (wait > 0.0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+        s->t_idle += s->now() - t1;
         if (int rc = admit(s)) return rc;
-        if (s->running.empty() && !s->queue.empty()) {
-            // idle: wait for the next arrival, then admit it
-            const double wait = s->reqs[s->queue.front()].info.arrival - s->now();
-            const double t1 = s->now();
-            if (wait > 0.0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-            s->t_idle += s->now() - t1;
-            if (int rc = admit(s)) return rc;
+    }
+    s->t_admit += s->now() - t;
+    if (!s->running.empty()) {
+        int32_t k = s->desc.max_chunk;
+        for (int64_t id : s->running) k = std::min(k, s->reqs[id].left);
+        t = s->now();
+        if (int rc = bm_batch_step(s->batch, s->desc.dt, k)) return rc;
+        int32_t code = 0;  // waits for the device (per-scene errors are read at retirement)
+        if (int rc = bm_batch_scene_error(s->batch, s->reqs[s->running.front()].scene, &code)) return rc;
+        s->t_step += s->now() - t;
+        ++s->chunks;
+        s->substeps += k;
+        for (int64_t id : s->running) {
+            s->reqs[id].left -= k;
+            s->reqs[id].info.substeps_done += k;
         }
-        s->t_admit += s->now() - t;
-        if (!s->running.empty()) {
-            int32_t k = s->desc.max_chunk;
-            for (int64_t id : s->running) k = std::min(k, s->reqs[id].left);
-            t = s->now();
-            if (int rc = bm_batch_step(s->batch, s->desc.dt, k)) return rc;
-            int32_t code = 0;  // waits for the device (per-scene errors are read at retirement)
-            if (int rc = bm_batch_scene_error(s->batch, s->reqs[s->running.front()].scene, &code)) return rc;
-            s->t_step += s->now() - t;
-            ++s->chunks;
-            s->substeps += k;
-            for (int64_t id : s->running) {
-                s->reqs[id].left -= k;
-                s->reqs[id].info.substeps_done += k;
-            }
-            t = s->now();
-            if (int rc = retire(s)) return rc;
-            s->t_retire += s->now() - t;
-        }
+        t = s->now();
+        if (int rc = retire(s)) return rc;
+        s->t_retire += s->now() - t;
     }
     int32_t n = 0;
     while (n < capacity && !s->done.empty()) {

@@ -582,7 +582,9 @@ class MPMSolver:
             return None
         return self._dev.download_grid()
 
-    def step(self, dt):
+    def step(self, dt, tape=None):
+        """MPMSolver::step(dt[, tape]) (mpm_solver.h:62-87); the reference's
+        tape records nothing, so `tape` is left as it is."""
         self._sync_device()
         self._dev.step(dt)
         self._P = None

@@ -397,8 +397,9 @@ int bm_server_batch(bm_server_handle s, bm_batch_handle* batch);
 /* Queue a request.  BM_ERR_OOM when it could not fit even an empty server. */
 int bm_server_submit(bm_server_handle s, const bm_request* req, int64_t* request_id);
 /* One scheduling round: admit, advance one chunk, retire.  Sleeps until the
- * next arrival when nothing is running.  Writes up to `capacity` ids of
- * requests finished (done or failed) since the last report. */
+ * next arrival when nothing is running and nothing is left to report.
+ * Writes up to `capacity` ids of requests finished (done or failed) since
+ * the last report; the rest are reported by later calls. */
 int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int32_t* n_done);
 int bm_server_info(bm_server_handle s, int64_t request_id, bm_request_info* info);
 /* Drop a queued or running request (its output is not written). */
