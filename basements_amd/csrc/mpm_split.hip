@@ -65,6 +65,15 @@ __device__ __forceinline__ float quanta_of(float m, double qscale_inv) { return 
 
 __global__ void k_set_zero(int* p) { *p = 0; }
 
+// Workgroups are dealt to the 8 XCDs round-robin (w % 8).  This bijection of
+// [0, G) gives XCD x a contiguous run of logical indices, so the particles of
+// neighbouring blocks -- which gather the same grid nodes -- are served from
+// one XCD's L2 instead of being spread over all eight.
+__device__ __forceinline__ int xcd_slot(int w, int G) {
+    const int x = w & 7, q = w >> 3, base = G >> 3, rem = G & 7;
+    return x * base + min(x, rem) + q;
+}
+
 // G2P + plasticity + write-back + deposit coefficients; one lane per particle.
 template <bool DO_G2P>
 __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float4* rec,
@@ -72,10 +81,11 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
     // the clear before this pass left the output grid's block list to be
     // emptied here, before the deposit kernel appends to it
     if (blockIdx.x == 0 && threadIdx.x == 0 && fa.list_reset) *fa.list_reset = 0;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int wg = xcd_slot((int)blockIdx.x, (int)gridDim.x);
+    int64_t i = (int64_t)wg * blockDim.x + threadIdx.x;
     int scene = -1;
     if (sv.chunks) {  // a batch: one scene per workgroup
-        const int4 ch = sv.chunks[blockIdx.x];
+        const int4 ch = sv.chunks[wg];
         if ((int)threadIdx.x >= ch.z) return;
         i = ch.y + threadIdx.x;
         scene = ch.x;
