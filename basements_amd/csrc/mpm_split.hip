@@ -285,8 +285,12 @@ __device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, cons
         accm[k] = 0u;
     }
     constexpr bool CENTRE = A0 <= 1 && 1 < A1;
+    // the next particle's record is fetched while this one is deposited
+    DepositRec nxt;
+    if (mine > 0) nxt = load_rec(rec, (int64_t)U.y + ord[s0]);
     for (int k = 0; k < mine; ++k) {
-        const DepositRec r = load_rec(rec, (int64_t)U.y + ord[s0 + k]);
+        const DepositRec r = nxt;
+        if (k + 1 < mine) nxt = load_rec(rec, (int64_t)U.y + ord[s0 + k + 1]);
         Stencil s;
         make_stencil(r.x, r.y, r.z, inv_dx, s);
         const v3<float>& mvel = r.mvel;
@@ -386,9 +390,19 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
     if (lane == 0) S.nleft = 0;
     __syncthreads();
 
-    // ---- count the particles per cell of the block
-    for (int pi = lane; pi < U.z; pi += 64) {
-        const float4 r0 = rec[rec_at((int64_t)U.y + pi, 0)];
+    // ---- count the particles per cell of the block (a unit holds at most
+    // SUNIT_MAX = 8 x 64 particles: all of a lane's positions are in flight at once)
+    float4 pos[SUNIT_MAX / 64];
+#pragma unroll
+    for (int j = 0; j < SUNIT_MAX / 64; ++j) {
+        const int pi = lane + 64 * j;
+        if (pi < U.z) pos[j] = rec[rec_at((int64_t)U.y + pi, 0)];
+    }
+#pragma unroll
+    for (int j = 0; j < SUNIT_MAX / 64; ++j) {
+        const int pi = lane + 64 * j;
+        if (pi >= U.z) break;
+        const float4 r0 = pos[j];
         Stencil s;
         make_stencil(r0.x, r0.y, r0.z, inv_dx, s);
         const int cx = s.b[0] - d.ox - tx0 - 1, cy = s.b[1] - d.oy - ty0 - 1, cz = s.b[2] - d.oz - tz0 - 1;
