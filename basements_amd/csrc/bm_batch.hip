@@ -105,13 +105,15 @@ struct bm_batch {
 
     // Captured substeps.  A replay starts and ends in the state it was
     // captured in (which particle buffer is current, which grid, re-binning
-    // due), so one graph is kept per (particle buffer, grid) combination.
+    // due), so one graph is kept per (particle buffer, grid) combination, and
+    // per whether its last substep ends a call.
     bool use_graph = false;
     struct GraphSlot {
         hipGraphExec_t exec = nullptr;
         const float* pbuf = nullptr;
         int cur = 0;
-    } graphs[4];
+        bool last = false;
+    } graphs[8];
     bool graph_valid = false;  // the slots match the current layout, dt and tuning
     float graph_dt = 0.0f;
     void drop_graphs() {
@@ -233,16 +235,18 @@ static int batch_check_err(bm_batch_handle b, int s = -1) {
 }
 
 // ------------------------------------------------------------------ substep
-static void batch_rebin(bm_batch_handle b) {
+// with_vc = false when a G2P follows, which rewrites velocity and C.
+static void batch_rebin(bm_batch_handle b, bool with_vc) {
     launch_rebin(b->pview(), b->pbuf2, b->pid, b->pid2, Domain{}, (int)b->B, b->unit_p, b->bs, b->stream, nullptr,
-                 b->view(), b->n_chunks);
+                 b->view(), b->n_chunks, with_vc);
     std::swap(b->pbuf, b->pbuf2);
     std::swap(b->pid, b->pid2);
     b->binned = true;
     b->since_bin = 0;
 }
 
-static void batch_pass(bm_batch_handle b, bool do_g2p, const GridBuf& gin, const GridBuf& gout, float dt) {
+static void batch_pass(bm_batch_handle b, bool do_g2p, const GridBuf& gin, const GridBuf& gout, float dt,
+                       bool store_vc) {
     SplitBatch sb;
     sb.sv = b->view();
     sb.n_chunks = b->n_chunks;
@@ -253,15 +257,17 @@ static void batch_pass(bm_batch_handle b, bool do_g2p, const GridBuf& gin, const
     // empties the output grid's block list
     launch_clear(gout, (int)std::min<int64_t>(b->B, 1 << 30), b->stream, b->n_redo, true);
     launch_split(do_g2p, b->pview(), b->bs.units, 0, gin, gout, b->bever, Domain{}, Phys{}, dt, dt, 0.0, b->d_err,
-                 b->d_counters, b->redo, b->n_redo, b->coef, b->stream, nullptr, nullptr, false, sb);
+                 b->d_counters, b->redo, b->n_redo, b->coef, b->stream, nullptr, nullptr, false, sb, store_vc);
 }
 
 // One coupled substep of every scene.  Host state only changes the way the
-// device state does, so the same calls record a graph.
-static void batch_substep(bm_batch_handle b, float dt) {
-    if (!b->binned) batch_rebin(b);
+// device state does, so the same calls record a graph.  last: the last
+// substep of a bm_batch_step call, whose G2P writes the particle velocity and
+// C (the earlier ones leave them to it; only downloads read them).
+static void batch_substep(bm_batch_handle b, float dt, bool last) {
+    if (!b->binned) batch_rebin(b, true);
     if (!b->prepared || dt != b->dt_prepared) {
-        batch_pass(b, false, b->grid[b->cur], b->grid[b->cur], dt);
+        batch_pass(b, false, b->grid[b->cur], b->grid[b->cur], dt, true);
         b->prepared = true;
         b->dt_prepared = dt;
     }
@@ -269,24 +275,25 @@ static void batch_substep(bm_batch_handle b, float dt) {
                   b->d_err, b->view(), b->d_has_body);
     launch_couple_batch(b->d_methods, b->d_has_body, b->d_bodies, b->d_cols, b->grid[b->cur], b->bever, b->view(), dt,
                         b->d_partials, b->d_out7, b->stream);
-    if (b->since_bin >= b->sort_every) batch_rebin(b);
+    if (b->since_bin >= b->sort_every) batch_rebin(b, false);
     const int nxt = 1 - b->cur;
-    batch_pass(b, true, b->grid[b->cur], b->grid[nxt], dt);
+    batch_pass(b, true, b->grid[b->cur], b->grid[nxt], dt, last);
     b->cur = nxt;
     b->since_bin += 1;
 }
 
 // The graph of 2 x sort_every substeps from the current state (re-binning
 // due): two re-binnings and an even number of grid swaps bring buffers and
-// grids back to where they started, so the graph can be replayed.
-static int graph_for_state(bm_batch_handle b, float dt, hipGraphExec_t* out) {
+// grids back to where they started, so the graph can be replayed.  last: its
+// final substep ends a bm_batch_step call (writes velocity and C).
+static int graph_for_state(bm_batch_handle b, float dt, bool last, hipGraphExec_t* out) {
     if (!b->graph_valid || b->graph_dt != dt) {
         b->drop_graphs();
         b->graph_valid = true;
         b->graph_dt = dt;
     }
     for (auto& g : b->graphs)
-        if (g.exec && g.pbuf == b->pbuf && g.cur == b->cur) {
+        if (g.exec && g.pbuf == b->pbuf && g.cur == b->cur && g.last == last) {
             *out = g.exec;
             return BM_OK;
         }
@@ -298,8 +305,9 @@ static int graph_for_state(bm_batch_handle b, float dt, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     slot->pbuf = b->pbuf;
     slot->cur = b->cur;
+    slot->last = last;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < G; ++i) batch_substep(b, dt);
+    for (int i = 0; i < G; ++i) batch_substep(b, dt, last && i == G - 1);
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess) return fail(BM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     if (b->pbuf != slot->pbuf || b->cur != slot->cur) {
@@ -571,16 +579,16 @@ int bm_batch_step(bm_batch_handle b, float dt, int32_t k) {
         const int G = 2 * b->sort_every;
         // bring the batch to the replay's starting point: deposited, re-binning due
         while (done < k && (!b->prepared || dt != b->dt_prepared || !b->binned || b->since_bin != b->sort_every)) {
-            batch_substep(b, dt);
+            batch_substep(b, dt, done == k - 1);
             ++done;
         }
-        if (k - done >= G) {
+        for (; k - done >= G; done += G) {
             hipGraphExec_t ge = nullptr;
-            if (int rc = graph_for_state(b, dt, &ge)) return rc;
-            for (; k - done >= G; done += G) HIP_TRY(hipGraphLaunch(ge, b->stream));
+            if (int rc = graph_for_state(b, dt, k - done == G, &ge)) return rc;
+            HIP_TRY(hipGraphLaunch(ge, b->stream));
         }
     }
-    for (; done < k; ++done) batch_substep(b, dt);
+    for (; done < k; ++done) batch_substep(b, dt, done == k - 1);
     HIP_TRY(hipGetLastError());
     return BM_OK;
 }
@@ -727,9 +735,10 @@ int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* un
         b->sort_every = c.sort;
         b->deposit_groups = c.groups;
         const int window = (std::max(substeps, c.sort) + c.sort - 1) / c.sort * c.sort;
-        for (int w = 0; w < 2; ++w) batch_substep(b, dt);
+        // the saved state is restored afterwards: no substep needs velocity and C
+        for (int w = 0; w < 2; ++w) batch_substep(b, dt, false);
         hipEventRecord(e0, b->stream);
-        for (int i = 0; i < window; ++i) batch_substep(b, dt);
+        for (int i = 0; i < window; ++i) batch_substep(b, dt, false);
         hipEventRecord(e1, b->stream);
         if (hipEventSynchronize(e1) != hipSuccess) return fail(BM_ERR_HIP, "autotune timing");
         float ms = 0.0f;

@@ -504,10 +504,11 @@ int bm_mpm_commit_device_particles(bm_mpm_handle h, int64_t n) {
 
 // ------------------------------------------------------------------ substep
 // Fast mode: re-bin the particles by grid block (swaps the particle buffers).
-static int rebin(bm_mpm_handle h) {
+// with_vc = false when a G2P follows, which rewrites velocity and C.
+static int rebin(bm_mpm_handle h, bool with_vc = true) {
     h->begin(BM_PHASE_SORT);
     const int rc = launch_rebin(h->pview(), h->pbuf2, h->pid, h->pid2, h->dom, (int)h->nblocks, h->unit_p, h->bs,
-                                h->stream, &h->n_units);
+                                h->stream, &h->n_units, SceneView{}, 0, with_vc);
     h->end();
     if (rc != 0) return fail(BM_ERR_HIP, "particle re-binning failed: %s", hipGetErrorString(hipGetLastError()));
     std::swap(h->pbuf, h->pbuf2);
@@ -518,9 +519,10 @@ static int rebin(bm_mpm_handle h) {
 }
 
 // The fast substep's particle kernels: fused G2P+P2G, or the two-kernel form
-// (tuning variant bit 2).  do_g2p = false deposits the current state only.
+// (tuning variant bit 2).  do_g2p = false deposits the current state only;
+// store_vc = false skips the velocity and C stores of the two-kernel G2P.
 static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const GridBuf& gout, float dt_g2p,
-                         float dt_fold) {
+                         float dt_fold, bool store_vc = true) {
     if (h->variant & 4) {
         // Time each kernel on its own: the particle pass under the caller's
         // phase (G2P, or P2G for a deposit of the current state) and the cell
@@ -549,7 +551,7 @@ static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const
         if (ctx.adv_phase >= 0) h->open_phase = BM_PHASE_OTHER;  // the opening event now starts OTHER
         launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
                      dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream, mark,
-                     &ctx, (h->variant & 8) != 0);
+                     &ctx, (h->variant & 8) != 0, SplitBatch{}, store_vc);
     } else
         launch_fused(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
                      dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant & 3,
@@ -578,17 +580,19 @@ static int p2g_fast(bm_mpm_handle h, float dt) {
     return BM_OK;
 }
 
-static int g2p_fast(bm_mpm_handle h, float dt) {
+// store_vc = false inside a multi-substep call: a later G2P of the same call
+// writes velocity and C (the re-binning before a G2P leaves them behind).
+static int g2p_fast(bm_mpm_handle h, float dt, bool store_vc) {
     // the current grid survives the re-binning: only the particle order changes
     if (!h->binned || h->since_bin >= h->sort_every)
-        if (int rc = rebin(h)) return rc;
+        if (int rc = rebin(h, false)) return rc;
     const int nxt = 1 - h->cur;
     h->begin(BM_PHASE_CLEAR);
     launch_clear(h->grid[nxt], (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream, h->n_redo,
                  (h->variant & 4) != 0);
     h->end();
     h->begin((h->variant & 4) ? BM_PHASE_G2P : BM_PHASE_G2P_P2G);
-    fast_kernels(h, true, h->grid[h->cur], h->grid[nxt], dt, dt);
+    fast_kernels(h, true, h->grid[h->cur], h->grid[nxt], dt, dt, store_vc);
     h->end();
     HIP_TRY(hipGetLastError());
     h->prepared = true;
@@ -636,10 +640,11 @@ int bm_mpm_grid_update(bm_mpm_handle h, float dt) {
     return grid_update_impl(h, dt, false);
 }
 
-int bm_mpm_g2p(bm_mpm_handle h, float dt) {
+// last = false: not the last G2P of the caller's call (see g2p_fast).
+static int g2p_impl(bm_mpm_handle h, float dt, bool last) {
     if (int rc = check_handle(h)) return rc;
     if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "g2p needs an updated grid");
-    if (h->fast) return g2p_fast(h, dt);
+    if (h->fast) return g2p_fast(h, dt, last);
     h->begin(BM_PHASE_G2P);
     launch_g2p_ref(h->pview(), h->grid[h->cur], h->dom, h->phys(), dt, h->d_err, h->stream);
     h->end();
@@ -647,16 +652,20 @@ int bm_mpm_g2p(bm_mpm_handle h, float dt) {
     return BM_OK;
 }
 
-int bm_mpm_step(bm_mpm_handle h, float dt) {
+int bm_mpm_g2p(bm_mpm_handle h, float dt) { return g2p_impl(h, dt, true); }
+
+static int step_impl(bm_mpm_handle h, float dt, bool last) {
     if (int rc = bm_mpm_p2g(h, dt)) return rc;
     if (int rc = bm_mpm_grid_update(h, dt)) return rc;
-    return bm_mpm_g2p(h, dt);
+    return g2p_impl(h, dt, last);
 }
+
+int bm_mpm_step(bm_mpm_handle h, float dt) { return step_impl(h, dt, true); }
 
 int bm_mpm_step_n(bm_mpm_handle h, float dt, int32_t k) {
     if (k < 0) return fail(BM_ERR_INVALID_ARG, "k < 0");
     for (int32_t i = 0; i < k; ++i)
-        if (int rc = bm_mpm_step(h, dt)) return rc;
+        if (int rc = step_impl(h, dt, i == k - 1)) return rc;
     return BM_OK;
 }
 
@@ -759,7 +768,7 @@ int bm_mpm_step_coupled(bm_mpm_handle h, int32_t method, float dt, int32_t k) {
         if (int rc = bm_mpm_p2g(h, dt)) return rc;
         if (int rc = grid_update_impl(h, dt, true)) return rc;
         if (int rc = bm_mpm_couple_body(h, method, dt, nullptr)) return rc;
-        if (int rc = bm_mpm_g2p(h, dt)) return rc;
+        if (int rc = g2p_impl(h, dt, i == k - 1)) return rc;
     }
     return BM_OK;
 }

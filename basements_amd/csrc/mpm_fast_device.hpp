@@ -124,6 +124,7 @@ struct FastArgs {
     int* n_redo;
     int* list_reset;  // block-list length of the output grid, zeroed by the particle pass (two-kernel form)
     const int* n_units_dev;  // unit count on the device (batches: no host read-back), else n_units
+    int store_vc = 1;        // the G2P writes the particle velocity and C (see launch_split)
 };
 
 __device__ __forceinline__ bool stencil_ok(const Domain& d, int lx, int ly, int lz) {

@@ -66,13 +66,19 @@ __global__ void __launch_bounds__(256) k_bin(PView P, Domain d, int* bcnt, int* 
     }
 }
 
+// VC = false leaves the velocity and C components behind (their values are
+// dead: the next particle pass is a G2P that rewrites them).
+template <bool VC>
 __global__ void k_reorder(PView src, float* dst, const int* key, const int* slot, const int* bstart,
                           const int* pid_src, int* pid_dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= src.n) return;
     const int64_t j = (int64_t)bstart[key[i]] + slot[i];
-#pragma unroll 2
-    for (int c = 0; c < PCOMP; ++c) dst[c * src.cap + j] = src.p[c * src.cap + i];
+#pragma unroll
+    for (int c = 0; c < PCOMP; ++c) {
+        if (!VC && ((c >= PV && c < PV + 3) || (c >= PC && c < PC + 9))) continue;
+        dst[c * src.cap + j] = src.p[c * src.cap + i];
+    }
     pid_dst[j] = pid_src[i];
 }
 
@@ -192,7 +198,8 @@ size_t scan_temp_bytes(int n) {
 // Re-bin the particles by grid block and rebuild the work units.  Returns the
 // unit count on the host (one stream synchronisation per re-binning).
 int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, const Domain& d, int nblocks,
-                 int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host, SceneView sv, int n_chunks) {
+                 int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host, SceneView sv, int n_chunks,
+                 bool with_vc) {
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nblocks, 256)), dim3(256), 0, s, bs.bcnt, nblocks, 0);
     if (sv.chunks) {
         if (n_chunks > 0)
@@ -203,8 +210,8 @@ int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, 
     size_t tb = bs.temp_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(bs.temp, tb, bs.bcnt, bs.bstart, nblocks, s);
     if (P.n > 0)
-        hipLaunchKernelGGL(k_reorder, dim3(grid_for(P.n, 256)), dim3(256), 0, s, P, pdst, bs.key, bs.slot, bs.bstart,
-                           pid_src, pid_dst);
+        hipLaunchKernelGGL(with_vc ? k_reorder<true> : k_reorder<false>, dim3(grid_for(P.n, 256)), dim3(256), 0, s,
+                           P, pdst, bs.key, bs.slot, bs.bstart, pid_src, pid_dst);
     hipLaunchKernelGGL(k_units_count, dim3(grid_for(nblocks, 256)), dim3(256), 0, s, bs.bcnt, nblocks, unit_p,
                        bs.ucnt);
     tb = bs.temp_bytes;

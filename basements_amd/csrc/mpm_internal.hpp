@@ -248,9 +248,11 @@ struct BinScratch {
 size_t scan_temp_bytes(int n);
 // Over a batch (sv.chunks set) k_bin runs one workgroup per chunk in its
 // scene's box.  n_units_host = nullptr skips the read-back (no host sync).
+// with_vc = false does not carry the velocity and C components over: only
+// for a re-binning that a G2P follows (it rewrites them before they are read).
 int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, const Domain& d, int nblocks,
                  int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host, SceneView sv = SceneView{},
-                 int n_chunks = 0);
+                 int n_chunks = 0, bool with_vc = true);
 void launch_iota(int* p, int64_t n, int64_t first, hipStream_t s);
 void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
@@ -274,10 +276,15 @@ struct SplitBatch {
     int n_units_bound = 0;
     int deposit_groups = 4096;  // fixed grid of the deposit when the count is on the device
 };
+// store_vc = false: the G2P does not write the particle velocity and C.  Only
+// a download and a deposit of the current state (do_g2p = false) read them,
+// and the deposit record carries what the next P2G needs, so a multi-substep
+// call stores them on its last substep only.
 void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*) = nullptr,
-                  void* ctx = nullptr, bool two_pass = false, const SplitBatch& batch = SplitBatch{});
+                  void* ctx = nullptr, bool two_pass = false, const SplitBatch& batch = SplitBatch{},
+                  bool store_vc = true);
 
 }  // namespace bm

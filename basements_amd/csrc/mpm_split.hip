@@ -140,13 +140,14 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
         P.p[(PX + 0) * P.cap + i] = x;
         P.p[(PX + 1) * P.cap + i] = y;
         P.p[(PX + 2) * P.cap + i] = z;
-        P.p[(PV + 0) * P.cap + i] = vel.x;
-        P.p[(PV + 1) * P.cap + i] = vel.y;
-        P.p[(PV + 2) * P.cap + i] = vel.z;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            P.p[(PF + c) * P.cap + i] = F.a[c];
-            P.p[(PC + c) * P.cap + i] = Cm.a[c];
+        for (int c = 0; c < 9; ++c) P.p[(PF + c) * P.cap + i] = F.a[c];
+        if (fa.store_vc) {
+            P.p[(PV + 0) * P.cap + i] = vel.x;
+            P.p[(PV + 1) * P.cap + i] = vel.y;
+            P.p[(PV + 2) * P.cap + i] = vel.z;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) P.p[(PC + c) * P.cap + i] = Cm.a[c];
         }
     } else {
         vel = mk(P.p[(PV + 0) * P.cap + i], P.p[(PV + 1) * P.cap + i], P.p[(PV + 2) * P.cap + i]);
@@ -538,14 +539,14 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx, bool two_pass,
-                  const SplitBatch& batch) {
+                  const SplitBatch& batch, bool store_vc) {
     const bool batched = batch.sv.chunks != nullptr;
     if (batched ? batch.n_chunks <= 0 : P.n <= 0) {  // no particle pass to empty the output grid's block list
         hipLaunchKernelGGL(k_set_zero, dim3(1), dim3(1), 0, s, gout.bcount);
         return;
     }
     float4* rec = (float4*)coef;  // 16 floats per particle (whole groups of 64: see rec_at)
-    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, gout.bcount, batch.n_units_dev};
+    FastArgs fa{dt_g2p, dt_fold, n_units, qscale_inv, redo, n_redo, gout.bcount, batch.n_units_dev, store_vc ? 1 : 0};
     // n_redo was zeroed by the grid clear that precedes every fast pass
     const int g = batched ? batch.n_chunks : (int)((P.n + 255) / 256);
     const SceneView sv = batch.sv;
