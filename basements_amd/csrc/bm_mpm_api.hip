@@ -93,7 +93,7 @@ struct bm_mpm_solver {
         int phase;
         hipEvent_t a, b;
     };
-    bool timing = false;
+    int timing = 0;  // 1: every phase; 2: dispatch-recorded kernel events only
     std::vector<hipEvent_t> ev_pool;
     std::vector<Timed> pending;
     double t_ms[BM_NUM_PHASES] = {};
@@ -114,15 +114,19 @@ struct bm_mpm_solver {
     void begin(int ph) {
         if (!timing) return;
         open_phase = ph;
+        open_ev = nullptr;
+        if (timing != 1) return;  // the phase is known, no event between launches
         open_ev = take_event();
         if (open_ev) hipEventRecord(open_ev, stream);
     }
     void end() {
-        if (!timing || open_phase < 0 || !open_ev) return;
-        hipEvent_t e = take_event();
-        if (e) {
-            hipEventRecord(e, stream);
-            pending.push_back(Timed{open_phase, open_ev, e});
+        if (!timing || open_phase < 0) return;
+        if (open_ev) {
+            hipEvent_t e = take_event();
+            if (e) {
+                hipEventRecord(e, stream);
+                pending.push_back(Timed{open_phase, open_ev, e});
+            }
         }
         open_phase = -1;
         open_ev = nullptr;
@@ -902,7 +906,8 @@ int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_ever
 
 int bm_mpm_set_timing(bm_mpm_handle h, int32_t on) {
     if (int rc = check_handle(h)) return rc;
-    h->timing = on != 0;
+    if (on < 0 || on > 2) return fail(BM_ERR_INVALID_ARG, "timing level %d not in 0..2", on);
+    h->timing = on;
     return BM_OK;
 }
 

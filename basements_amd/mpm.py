@@ -496,8 +496,11 @@ class Solver:
         """Fast-mode launch parameters (bm_mpm_set_tuning); 0 / -1 keep the current value."""
         _check(lib().bm_mpm_set_tuning(self.h, int(unit_particles), int(sort_every), int(variant)))
 
-    def set_timing(self, on=True):
-        _check(lib().bm_mpm_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, on=True, kernels_only=False):
+        """on: per-phase HIP events.  kernels_only: only the fast form's G2P
+        and P2G kernels, timed by events their own dispatches record (no
+        events between launches to perturb the stream)."""
+        _check(lib().bm_mpm_set_timing(self.h, (2 if kernels_only else 1) if on else 0))
 
     def read_timing(self, reset=True):
         """{phase: (total_ms, launches)} from HIP events on the handle's stream."""

@@ -266,6 +266,10 @@ enum bm_phase {
     BM_PHASE_OTHER = 7,
     BM_NUM_PHASES = 8
 };
+/* on = 1: every phase (event pairs recorded between the launches, which
+ * adds their own gaps to the stream); on = 2: only the two-kernel fast
+ * form's G2P and P2G kernels, timed by events that their own dispatches
+ * record (nothing between launches); 0: off. */
 int bm_mpm_set_timing(bm_mpm_handle h, int32_t on);
 /* Synchronises, then returns the accumulated milliseconds and launch counts
  * per phase (arrays of BM_NUM_PHASES); reset != 0 clears the accumulators. */
