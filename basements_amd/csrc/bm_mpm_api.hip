@@ -81,7 +81,7 @@ struct bm_mpm_solver {
     int n_units = 0;
     bool binned = false;      // units describe the current particle order
     int since_bin = 0;        // substeps since the last re-binning
-    int sort_every = 16;
+    int sort_every = 32;
     int unit_p = 512;
     int variant = 4;          // fast-kernel variant (see bm_mpm_set_tuning): two-kernel form
     bool prepared = false;    // grid[prep] already holds the next substep's P2G

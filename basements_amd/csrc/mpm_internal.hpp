@@ -60,11 +60,14 @@ __host__ __device__ inline int64_t block_of(const Domain& d, int lx, int ly, int
     return d.b0 + ((int64_t)(lx >> 2) * d.nby + (ly >> 2)) * d.nbz + (lz >> 2);
 }
 // Block coordinates of a block of the grid arrays inside its box.
+// 32-bit division: a box holds fewer than 2^31 blocks (64-bit division is a
+// long software sequence on the GPU).
 __host__ __device__ inline void block_coords(const Domain& d, int64_t blk, int& bx, int& by, int& bz) {
-    const int64_t l = blk - d.b0;
-    bz = (int)(l % d.nbz);
-    by = (int)((l / d.nbz) % d.nby);
-    bx = (int)(l / ((int64_t)d.nbz * d.nby));
+    const int l = (int)(blk - d.b0);
+    const int q = l / d.nbz;
+    bz = l - q * d.nbz;
+    bx = q / d.nby;
+    by = q - bx * d.nby;
 }
 __host__ __device__ inline int64_t node_of(const Domain& d, int lx, int ly, int lz) {
     return block_of(d, lx, ly, lz) * 64 + ((lx & 3) * 4 + (ly & 3)) * 4 + (lz & 3);

@@ -85,7 +85,9 @@ def _args():
     ap.add_argument("--rates", default="5,200", help="--config serve: arrival rates (requests/s)")
     ap.add_argument("--serve-particles", type=int, default=16_000_000, help="--config serve: particle room")
     ap.add_argument("--serve-blocks", type=int, default=400_000, help="--config serve: grid-block room")
-    ap.add_argument("--tuning", default="", help="--config batch: unit,sort_every,deposit_groups instead of autotune")
+    ap.add_argument("--tuning", default="",
+                    help="--config batch: unit,sort_every,deposit_groups instead of autotune; "
+                         "wheel_bed / box_drop fast mode: unit,sort_every")
     ap.add_argument("--mode", default="fast", choices=["reference", "fast"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", type=int, default=1)
@@ -452,6 +454,9 @@ def main():
     S.set_gravity(scene.gravity)
     S.upload(scene.particles)
     S.synchronize()
+    if a.tuning and mode == MODE_FAST:
+        unit, sort_every = (int(t) for t in a.tuning.split(",")[:2])
+        S.set_tuning(unit, sort_every, -1)
 
     dt = scene.dt
     tool = scene.tool

@@ -67,7 +67,7 @@ typedef struct {
     int32_t grid_dims[3];    /* nodes per axis; rounded up to a multiple of 4 */
     int64_t max_particles;   /* capacity of the particle arrays */
     int32_t mode;            /* enum bm_mode */
-    int32_t sort_every;      /* BM_MODE_FAST: re-bin particles every k substeps (0 = default 16) */
+    int32_t sort_every;      /* BM_MODE_FAST: re-bin particles every k substeps (0 = default 32) */
     void* stream;            /* hipStream_t to run on; NULL = the handle creates its own */
 } bm_mpm_desc;
 
@@ -239,7 +239,7 @@ int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallback
 /* ---------------------------------------------------------------- tuning */
 /* BM_MODE_FAST launch parameters (what the autotuner searches):
  * unit_particles: particles per workgroup unit (16..512, default 512);
- * sort_every: substeps between particle re-binnings (>= 1, default 16);
+ * sort_every: substeps between particle re-binnings (>= 1, default 32);
  * variant: bit 2 = two-kernel form (per-particle G2P writing deposit
  * coefficients, then a per-block P2G accumulating each cell's particles in
  * registers; the default, variant 4); without bit 2 the fused G2P+P2G

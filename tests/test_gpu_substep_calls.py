@@ -35,7 +35,7 @@ def _one_by_one(S, body, coupler, n):
 
 def test_single_handle_long_calls_match_single_substep_calls():
     P, prm, sd, body = _scenes()[1]  # the rolling wheel on M1
-    calls = [5, 16, 16, 1, 9]        # crosses re-binnings at every 16th substep
+    calls = [5, 16, 16, 1, 9]        # crosses a re-binning (every 32nd substep by default)
     L = gpu_solver(P, sd.dx, sd.origin, sd.dims, prm, mode=1)
     O = gpu_solver(P, sd.dx, sd.origin, sd.dims, prm, mode=1)
     L.set_body(body)
