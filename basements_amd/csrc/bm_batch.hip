@@ -245,8 +245,9 @@ static void batch_rebin(bm_batch_handle b, bool with_vc) {
     b->since_bin = 0;
 }
 
+// clear = false: the grid update already cleared gout (and the redo count).
 static void batch_pass(bm_batch_handle b, bool do_g2p, const GridBuf& gin, const GridBuf& gout, float dt,
-                       bool store_vc) {
+                       bool store_vc, bool clear = true) {
     SplitBatch sb;
     sb.sv = b->view();
     sb.n_chunks = b->n_chunks;
@@ -255,7 +256,7 @@ static void batch_pass(bm_batch_handle b, bool do_g2p, const GridBuf& gin, const
     sb.deposit_groups = b->deposit_groups;
     // the clear before every pass also zeroes the redo counter; the pass
     // empties the output grid's block list
-    launch_clear(gout, (int)std::min<int64_t>(b->B, 1 << 30), b->stream, b->n_redo, true);
+    if (clear) launch_clear(gout, (int)std::min<int64_t>(b->B, 1 << 30), b->stream, b->n_redo, true);
     launch_split(do_g2p, b->pview(), b->bs.units, 0, gin, gout, b->bever, Domain{}, Phys{}, dt, dt, 0.0, b->d_err,
                  b->d_counters, b->redo, b->n_redo, b->coef, b->stream, nullptr, nullptr, false, sb, store_vc);
 }
@@ -271,13 +272,14 @@ static void batch_substep(bm_batch_handle b, float dt, bool last) {
         b->prepared = true;
         b->dt_prepared = dt;
     }
+    // the update also clears the grid the G2P below deposits into
     launch_update(b->grid[b->cur], (int)b->B, Domain{}, Phys{}, 0.0, dt, b->stream, b->d_bodies, b->d_cols, LLONG_MAX,
-                  b->d_err, b->view(), b->d_has_body);
+                  b->d_err, b->view(), b->d_has_body, &b->grid[1 - b->cur], b->n_redo);
     launch_couple_batch(b->d_methods, b->d_has_body, b->d_bodies, b->d_cols, b->grid[b->cur], b->bever, b->view(), dt,
                         b->d_partials, b->d_out7, b->stream);
     if (b->since_bin >= b->sort_every) batch_rebin(b, false);
     const int nxt = 1 - b->cur;
-    batch_pass(b, true, b->grid[b->cur], b->grid[nxt], dt, last);
+    batch_pass(b, true, b->grid[b->cur], b->grid[nxt], dt, last, false);
     b->cur = nxt;
     b->since_bin += 1;
 }

@@ -81,6 +81,7 @@ struct bm_mpm_solver {
     int n_units = 0;
     bool binned = false;      // units describe the current particle order
     int since_bin = 0;        // substeps since the last re-binning
+    bool next_cleared = false;  // the last grid update cleared the other grid for the next G2P
     int sort_every = 32;
     int unit_p = 512;
     int variant = 4;          // fast-kernel variant (see bm_mpm_set_tuning): two-kernel form
@@ -591,10 +592,13 @@ static int g2p_fast(bm_mpm_handle h, float dt, bool store_vc) {
     if (!h->binned || h->since_bin >= h->sort_every)
         if (int rc = rebin(h, false)) return rc;
     const int nxt = 1 - h->cur;
-    h->begin(BM_PHASE_CLEAR);
-    launch_clear(h->grid[nxt], (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream, h->n_redo,
-                 (h->variant & 4) != 0);
-    h->end();
+    if (!h->next_cleared) {
+        h->begin(BM_PHASE_CLEAR);
+        launch_clear(h->grid[nxt], (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream, h->n_redo,
+                     (h->variant & 4) != 0);
+        h->end();
+    }
+    h->next_cleared = false;
     h->begin((h->variant & 4) ? BM_PHASE_G2P : BM_PHASE_G2P_P2G);
     fast_kernels(h, true, h->grid[h->cur], h->grid[nxt], dt, dt, store_vc);
     h->end();
@@ -626,14 +630,21 @@ int bm_mpm_p2g(bm_mpm_handle h, float dt) {
 // (the caller couples the body next).
 static int grid_update_impl(bm_mpm_handle h, float dt, bool with_body) {
     if (h->phase != PH_P2G) return fail(BM_ERR_STATE, "grid_update needs a preceding p2g");
+    // two-kernel fast form: the update also clears the grid the next G2P
+    // deposits into (and the FP64 redo count), saving that launch
+    const bool clear_next = h->fast && (h->variant & 4);
+    const GridBuf* nxt = clear_next ? &h->grid[1 - h->cur] : nullptr;
+    int* zw = clear_next ? h->n_redo : nullptr;
     h->begin(BM_PHASE_UPDATE);
     if (with_body)
         launch_update(h->grid[h->cur], (int)h->nblocks, h->dom, h->phys(), h->qscale(), dt, h->stream, h->d_body,
-                      h->d_col, h->body_blocks, h->d_err);
+                      h->d_col, h->body_blocks, h->d_err, SceneView{}, nullptr, nxt, zw);
     else
-        launch_update(h->grid[h->cur], (int)h->nblocks, h->dom, h->phys(), h->qscale(), dt, h->stream);
+        launch_update(h->grid[h->cur], (int)h->nblocks, h->dom, h->phys(), h->qscale(), dt, h->stream, nullptr,
+                      nullptr, 0, nullptr, SceneView{}, nullptr, nxt, zw);
     h->end();
     HIP_TRY(hipGetLastError());
+    h->next_cleared = clear_next;
     h->phase = PH_UPDATED;
     h->body_begun = with_body;
     return BM_OK;

@@ -196,9 +196,12 @@ void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Doma
 // (applied loads, collider), as k_body_begin.
 // Over a batch (sv.sc set) each block takes its scene's box and physics, and
 // body[s] / col[s] are the bodies of the scenes with has_body[s] != 0.
+// clear_next: also clear that grid's listed blocks (as launch_clear with
+// defer_reset, for a grid without a force array) and zero *zero_word.
 void launch_update(const GridBuf& g, int nblocks, const Domain& d, const Phys& ph, double qscale, float dt,
                    hipStream_t s, bm_body* body = nullptr, ColliderDev* col = nullptr, int64_t max_blocks = 0,
-                   int* err = nullptr, SceneView sv = SceneView{}, const int* has_body = nullptr);
+                   int* err = nullptr, SceneView sv = SceneView{}, const int* has_body = nullptr,
+                   const GridBuf* clear_next = nullptr, int* zero_word = nullptr);
 void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phys& ph, float dt, int* err,
                     hipStream_t s);
 // The couplers run a fixed number of workgroups over the scanned blocks, so

@@ -149,7 +149,20 @@ __device__ void body_begin(bm_body* body, ColliderDev* col, const Domain& d, lon
 
 __global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d0, Phys ph0, double qscale0, float dt, bm_body* body,
                                                ColliderDev* col, long long max_blocks, int* err, SceneView sv,
-                                               const int* has_body) {
+                                               const int* has_body, GridBuf clr, int* zero_word) {
+    // the other grid's clear rides along (what k_clear_blocks does, block
+    // list length left to the next particle pass)
+    if (clr.mq) {
+        const int cc = *clr.bcount;
+        for (int b = blockIdx.x; b < cc; b += gridDim.x) {
+            const int blk = clr.blist[b];
+            const int64_t nid = (int64_t)blk * 64 + threadIdx.x;
+            clr.mq[nid] = 0;
+            clr.mv[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (threadIdx.x == 0) clr.bflag[blk] = 0u;
+        }
+        if (zero_word && blockIdx.x == 0 && threadIdx.x == 0) *zero_word = 0;
+    }
     if (body && threadIdx.x == 0) {
         if (!sv.sc) {
             if (blockIdx.x == 0) body_begin(body, col, d0, max_blocks, err);
@@ -546,11 +559,13 @@ void launch_p2g_ref(const PView& P, const GridBuf& g, uint8_t* bever, const Doma
 }
 void launch_update(const GridBuf& g, int nblocks, const Domain& d, const Phys& ph, double qscale, float dt,
                    hipStream_t s, bm_body* body, ColliderDev* col, int64_t max_blocks, int* err, SceneView sv,
-                   const int* has_body) {
+                   const int* has_body, const GridBuf* clear_next, int* zero_word) {
     int grid = nblocks < 16384 ? (nblocks > 0 ? nblocks : 1) : 16384;
     if (grid < sv.n_scenes) grid = sv.n_scenes;
+    GridBuf clr{};
+    if (clear_next) clr = *clear_next;
     hipLaunchKernelGGL(k_update, dim3(grid), dim3(64), 0, s, g, d, ph, qscale, dt, body, col, (long long)max_blocks,
-                       err, sv, has_body);
+                       err, sv, has_body, clr, zero_word);
 }
 void launch_g2p_ref(const PView& P, const GridBuf& g, const Domain& d, const Phys& ph, float dt, int* err,
                     hipStream_t s) {
