@@ -607,11 +607,16 @@ int bm_batch_download_scene(bm_batch_handle b, int32_t id, bm_particles* p) {
     if (!b->stage.ensure((sizeof(float) * PCOMP + sizeof(int)) * n)) return fail(BM_ERR_OOM, "pinned staging");
     float* st = b->stage.f();
     int* order = reinterpret_cast<int*>(st + PCOMP * n);
-    HIP_TRY(hipMemcpy2D(st, sizeof(float) * n, b->pbuf + S.p0, sizeof(float) * b->capP, sizeof(float) * n, PCOMP,
-                        hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(order, b->pid + S.p0, sizeof(int) * n, hipMemcpyDeviceToHost));
-    // slot i holds the scene's particle order[i] - p0 (insertion index)
-    if (!particles_from_soa(st, n, order, S.p0, p)) return fail(BM_ERR_HIP, "particle order corrupted");
+    // slot p0 + i holds the scene's particle pid[p0 + i] (insertion index +
+    // p0): the GPU puts them back in insertion order in the re-binning buffer
+    // (free between re-binnings), so the host copy is a plain transpose
+    launch_unpermute(b->pview(), b->pbuf2, b->pid, S.p0, n, b->stream);
+    HIP_TRY(hipMemcpyAsync(order, b->pid + S.p0, sizeof(int) * n, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipMemcpy2DAsync(st, sizeof(float) * n, b->pbuf2 + S.p0, sizeof(float) * b->capP, sizeof(float) * n, PCOMP,
+                             hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (!order_in_range(order, n, S.p0)) return fail(BM_ERR_HIP, "particle order corrupted");
+    particles_from_soa(st, n, nullptr, 0, p);
     return BM_OK;
 }
 

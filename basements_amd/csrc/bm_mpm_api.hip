@@ -464,11 +464,20 @@ int bm_mpm_download_particles(bm_mpm_handle h, bm_particles* p) {
     if (!h->stage.ensure((sizeof(float) * PCOMP + sizeof(int)) * n)) return fail(BM_ERR_OOM, "pinned staging");
     float* st = h->stage.f();
     int* order = reinterpret_cast<int*>(st + PCOMP * n);
-    HIP_TRY(hipMemcpy2D(st, sizeof(float) * n, h->pbuf, sizeof(float) * h->cap, sizeof(float) * n, PCOMP,
-                        hipMemcpyDeviceToHost));
-    // fast mode stores particles binned by block: slot i holds particle pid[i]
-    if (h->fast) HIP_TRY(hipMemcpy(order, h->pid, sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (!particles_from_soa(st, n, h->fast ? order : nullptr, 0, p)) return fail(BM_ERR_HIP, "particle order corrupted");
+    // fast mode stores particles binned by block (slot i holds particle
+    // pid[i]): the GPU puts them back in insertion order in the re-binning
+    // buffer (free between re-binnings), so the host copy is a plain transpose
+    const float* src = h->pbuf;
+    if (h->fast) {
+        launch_unpermute(h->pview(), h->pbuf2, h->pid, 0, n, h->stream);
+        HIP_TRY(hipMemcpyAsync(order, h->pid, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+        src = h->pbuf2;
+    }
+    HIP_TRY(hipMemcpy2DAsync(st, sizeof(float) * n, src, sizeof(float) * h->cap, sizeof(float) * n, PCOMP,
+                             hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->fast && !order_in_range(order, n, 0)) return fail(BM_ERR_HIP, "particle order corrupted");
+    particles_from_soa(st, n, nullptr, 0, p);
     return BM_OK;
 }
 

@@ -12,6 +12,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -67,30 +68,37 @@ struct PinnedStage {
 };
 
 // Caller layout -> component-major st[c * n + i].  Null fields are skipped.
-// Returns false when any value is not finite.
+// Returns false when any value is not finite.  Blocks of 1024 particles, one
+// component row at a time (each output row is one sequential stream); the
+// finiteness test is an exponent check folded into the same pass.
 inline bool soa_from_particles(const float* x, const float* v, const float* F, const float* Cm, const float* m,
                                const float* V0, int64_t n, float* st) {
     std::atomic<bool> bad{false};
     parallel_ranges(n, [&](int64_t lo, int64_t hi) {
-        float acc = 0.0f;  // NaN and Inf survive the sum (scaled so finite values cannot overflow it)
-        for (int64_t i = lo; i < hi; ++i) {
-            for (int k = 0; k < 3; ++k) {
-                if (x) st[(PX + k) * n + i] = x[3 * i + k];
-                if (v) st[(PV + k) * n + i] = v[3 * i + k];
+        uint32_t nonfinite = 0;
+        auto rows = [&](const float* src, int c0, int w, int64_t b0, int64_t b1) {
+            if (!src) return;
+            for (int k = 0; k < w; ++k) {
+                float* o = st + (int64_t)(c0 + k) * n;
+                for (int64_t i = b0; i < b1; ++i) {
+                    const float a = src[(int64_t)w * i + k];
+                    uint32_t u;
+                    std::memcpy(&u, &a, sizeof u);
+                    nonfinite |= (uint32_t)((u & 0x7f800000u) == 0x7f800000u);
+                    o[i] = a;
+                }
             }
-            for (int k = 0; k < 9; ++k) {
-                if (F) st[(PF + k) * n + i] = F[9 * i + k];
-                if (Cm) st[(PC + k) * n + i] = Cm[9 * i + k];
-            }
-            if (m) st[PM * n + i] = m[i];
-            if (V0) st[PV0 * n + i] = V0[i];
+        };
+        for (int64_t b0 = lo; b0 < hi; b0 += 1024) {
+            const int64_t b1 = std::min(hi, b0 + 1024);
+            rows(x, PX, 3, b0, b1);
+            rows(v, PV, 3, b0, b1);
+            rows(F, PF, 9, b0, b1);
+            rows(Cm, PC, 9, b0, b1);
+            rows(m, PM, 1, b0, b1);
+            rows(V0, PV0, 1, b0, b1);
         }
-        for (int64_t i = lo; i < hi; ++i)
-            for (int c = 0; c < PCOMP; ++c) {
-                const bool present = (c < PV ? x : c < PF ? v : c < PC ? F : c < PM ? Cm : c == PM ? m : V0) != nullptr;
-                if (present) acc += st[c * n + i] * 0x1p-100f;
-            }
-        if (!std::isfinite(acc)) bad.store(true, std::memory_order_relaxed);
+        if (nonfinite) bad.store(true, std::memory_order_relaxed);
     });
     return !bad.load();
 }
@@ -99,6 +107,23 @@ inline bool soa_from_particles(const float* x, const float* v, const float* F, c
 // (order == nullptr: slot i is particle i).  Returns false when an order
 // entry is out of range.  Null fields are skipped.
 inline bool particles_from_soa(const float* st, int64_t n, const int* order, int64_t base, bm_particles* p) {
+    if (!order) {  // insertion order already: sequential streams both ways
+        parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i) {
+                for (int k = 0; k < 3; ++k) {
+                    if (p->x) p->x[3 * i + k] = st[(PX + k) * n + i];
+                    if (p->v) p->v[3 * i + k] = st[(PV + k) * n + i];
+                }
+                for (int k = 0; k < 9; ++k) {
+                    if (p->F) p->F[9 * i + k] = st[(PF + k) * n + i];
+                    if (p->C) p->C[9 * i + k] = st[(PC + k) * n + i];
+                }
+                if (p->m) p->m[i] = st[PM * n + i];
+                if (p->V0) p->V0[i] = st[PV0 * n + i];
+            }
+        });
+        return true;
+    }
     std::atomic<bool> bad{false};
     parallel_ranges(n, [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; ++i) {
@@ -118,6 +143,17 @@ inline bool particles_from_soa(const float* st, int64_t n, const int* order, int
             if (p->m) p->m[o] = st[PM * n + i];
             if (p->V0) p->V0[o] = st[PV0 * n + i];
         }
+    });
+    return !bad.load();
+}
+
+// True when every order entry lies in [base, base + n).
+inline bool order_in_range(const int* order, int64_t n, int64_t base) {
+    std::atomic<bool> bad{false};
+    parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+        bool b = false;
+        for (int64_t i = lo; i < hi; ++i) b |= (int64_t)order[i] < base || (int64_t)order[i] >= base + n;
+        if (b) bad.store(true, std::memory_order_relaxed);
     });
     return !bad.load();
 }

@@ -82,6 +82,19 @@ __global__ void k_reorder(PView src, float* dst, const int* key, const int* slot
     pid_dst[j] = pid_src[i];
 }
 
+// Back to insertion order for a download: slot base + i holds particle
+// pid[base + i] (an index in [base, base + n)); every component moves to that
+// particle's slot of dst.  Entries out of range are left out (the host checks
+// the ids it reads back and reports a corrupted order).
+__global__ void k_unpermute(PView src, float* dst, const int* pid, int64_t base, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t j = (int64_t)pid[base + i];
+    if (j < base || j >= base + n) return;
+#pragma unroll
+    for (int c = 0; c < PCOMP; ++c) dst[c * src.cap + j] = src.p[c * src.cap + base + i];
+}
+
 __global__ void k_units_count(const int* bcnt, int nblocks, int unit_p, int* ucnt) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b < nblocks) ucnt[b] = (bcnt[b] + unit_p - 1) / unit_p;
@@ -222,6 +235,10 @@ int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, 
     if (hipMemcpyAsync(n_units_host, bs.n_units, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
     return 0;
+}
+
+void launch_unpermute(const PView& P, float* dst, const int* pid, int64_t base, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_unpermute, dim3(grid_for(n, 256)), dim3(256), 0, s, P, dst, pid, base, n);
 }
 
 void launch_iota(int* p, int64_t n, int64_t first, hipStream_t s) {

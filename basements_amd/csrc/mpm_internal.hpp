@@ -260,6 +260,9 @@ int launch_rebin(const PView& P, float* pdst, const int* pid_src, int* pid_dst, 
                  int unit_p, BinScratch& bs, hipStream_t s, int* n_units_host, SceneView sv = SceneView{},
                  int n_chunks = 0, bool with_vc = true);
 void launch_iota(int* p, int64_t n, int64_t first, hipStream_t s);
+// Slots [base, base + n) of P into insertion order in dst (same pitch):
+// dst[c * cap + pid[base + i]] = P[c * cap + base + i].
+void launch_unpermute(const PView& P, float* dst, const int* pid, int64_t base, int64_t n, hipStream_t s);
 void launch_fused(bool do_g2p, const PView& P, const int4* units, int n_units, const GridBuf& gin,
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, int variant,
