@@ -875,6 +875,14 @@ int bm_mpm_upload_grid(bm_mpm_handle h, const int32_t org[3], const int32_t dims
     return BM_OK;
 }
 
+int bm_mpm_grid_settle(bm_mpm_handle h, float dvy) {
+    if (int rc = check_handle(h)) return rc;
+    if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "grid_settle needs an updated grid");
+    launch_grid_settle(h->grid[h->cur], (int)std::min<int64_t>(h->nblocks, 1 << 30), dvy, h->stream);
+    HIP_TRY(hipGetLastError());
+    return BM_OK;
+}
+
 int bm_mpm_grid_mass_total(bm_mpm_handle h, int64_t* total) {
     if (int rc = check_handle(h)) return rc;
     if (!total) return fail(BM_ERR_INVALID_ARG, "null output");

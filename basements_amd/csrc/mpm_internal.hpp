@@ -227,6 +227,7 @@ void launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridB
 void launch_couple_batch(const int* methods, const int* has_body, bm_body* bodies, const ColliderDev* cols,
                          const GridBuf& g, uint8_t* bever, SceneView sv, float dt, double* partials, double* out7,
                          hipStream_t s);
+void launch_grid_settle(const GridBuf& g, int nblocks, float dvy, hipStream_t s);
 void launch_grid_gather(const GridBuf& g, uint8_t* bever, const Domain& d, const int* org, const int* dims,
                         double qscale, int updated, long long* o_mq, float* o_mass, float* o_vel, float* o_force,
                         uint8_t* o_act, uint8_t* o_alloc, hipStream_t s);

@@ -519,6 +519,24 @@ __global__ void k_grid_scatter(GridBuf g, uint8_t* bever, Domain d, int org0, in
     touch_block(g, bever, (int)(nid >> 6));
 }
 
+// bench_compare_all.cpp:117-126: v_y += dvy, clamped at 0, on active nodes
+// (mass > 0 after the update; w holds the node mass).
+__global__ void __launch_bounds__(64) k_grid_settle(GridBuf g, float dvy) {
+    const int cnt = *g.bcount;
+    for (int b = blockIdx.x; b < cnt; b += gridDim.x) {
+        const int64_t nid = (int64_t)g.blist[b] * 64 + threadIdx.x;
+        float4 q = g.mv[nid];
+        if (!(q.w > 0.0f)) continue;
+        q.y = q.y + dvy;
+        if (q.y < 0.0f) q.y = 0.0f;
+        g.mv[nid] = q;
+    }
+}
+void launch_grid_settle(const GridBuf& g, int nblocks, float dvy, hipStream_t s) {
+    const int grid = nblocks < 4096 ? (nblocks > 0 ? nblocks : 1) : 4096;
+    hipLaunchKernelGGL(k_grid_settle, dim3(grid), dim3(64), 0, s, g, dvy);
+}
+
 __global__ void __launch_bounds__(256) k_mass_total(GridBuf g, unsigned long long* out) {
     const int cnt = *g.bcount;
     const int64_t tot = (int64_t)cnt * 64;

@@ -1,0 +1,159 @@
+"""Grid-only coupler loops of the reference's research harness on the device.
+
+The reference's ``bench_compare_all`` drives its rigid-body couplers on a
+seeded sand grid with no particles: every step applies gravity to the body,
+couples it (M0 or M1), adds the reaction, integrates the body, then lets the
+grid settle (v_y += -g dt, clamped at 0) and logs one time-series row
+(``research/mpm_rigid_coupling/bench_compare_all.cpp:72-134`` box drop,
+``:614-689`` rover wheel; the row format is ``FrameExporter::log_step``,
+``frame_exporter.h:44-57``).  Here the coupling, reaction, integration and
+settling run on the GPU (``bm_mpm_couple_body``, ``bm_mpm_grid_settle``);
+the host reads the body and the grid back only to log.
+
+Scenario constants follow ``scenarios/scenario_box_drop.h:14-78`` and
+``scenarios/scenario_rover_wheel.h:19-176``.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+COLUMNS = ("t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y", "p_sum_y", "KE", "nodes_inside")
+
+
+@dataclass
+class GridOnlyScenario:
+    """A seeded sand patch (nodes i in [-ei, ei], j in [0, layers), k in [-ek,
+    ek], each with mass ``node_mass`` at rest) and one dynamic body."""
+
+    name: str
+    dx: float
+    extent_i: int
+    extent_k: int
+    layers_y: int
+    node_mass: float
+    shape: int            # SHAPE_BOX or SHAPE_CYLINDER
+    half_extents: tuple
+    body_mass: float
+    position: tuple
+    orientation: tuple
+    linear_velocity: tuple
+    angular_velocity: tuple
+    friction: float = 0.5
+    dt: float = 1.0 / 240.0
+    total_time: float = 3.0
+    gravity_g: float = 9.81
+    # the node box the handle holds: covers the patch, the body's reach and
+    # block alignment with the reference's 4^3 blocks (origin a multiple of 4)
+    origin: tuple = (-16, -16, -16)
+    dims: tuple = (32, 32, 32)
+
+    def num_steps(self) -> int:
+        # scenario_box_drop.h:37-39 (same in the rover wheel)
+        return int(np.float32(self.total_time) / np.float32(self.dt) + np.float32(0.5))
+
+
+def box_drop() -> GridOnlyScenario:
+    """ScenarioBoxDrop (scenario_box_drop.h:14-78): a 1 kg box, half extents
+    0.2 m, dropped from y = 0.55 m on a 7 x 3 x 7 node patch at dx = 0.1 m.
+    make_box sets inertia_tensor directly, so the inverse inertia stays the
+    identity (rigid_body.h:122)."""
+    from .mpm import SHAPE_BOX
+
+    return GridOnlyScenario(name="box_drop", dx=0.1, extent_i=3, extent_k=3, layers_y=3, node_mass=0.01,
+                            shape=SHAPE_BOX, half_extents=(0.2, 0.2, 0.2), body_mass=1.0, position=(0.0, 0.55, 0.0),
+                            orientation=(1.0, 0.0, 0.0, 0.0), linear_velocity=(0.0, 0.0, 0.0),
+                            angular_velocity=(0.0, 0.0, 0.0))
+
+
+def rover_wheel() -> GridOnlyScenario:
+    """ScenarioRoverWheel in ROLL mode (scenario_rover_wheel.h:19-176): a 1 kg
+    cylinder R = 0.15 m, half length 0.05 m, its axis turned onto world Z
+    (Quaternion::from_axis_angle((1,0,0), 1.5707963f) in FP32, quaternion.h:36-47),
+    spinning at -10 rad/s about Z, on a 13 x 3 x 9 node patch."""
+    from .mpm import SHAPE_CYLINDER
+
+    half = np.float32(1.5707963) * np.float32(0.5)
+    s, c = np.float32(math.sin(float(half))), np.float32(math.cos(float(half)))
+    return GridOnlyScenario(name="rover_wheel", dx=0.1, extent_i=6, extent_k=4, layers_y=3, node_mass=0.01,
+                            shape=SHAPE_CYLINDER, half_extents=(0.15, 0.05, 0.15), body_mass=1.0,
+                            position=(0.0, 0.55, 0.0), orientation=(float(c), float(s), 0.0, 0.0),
+                            linear_velocity=(0.0, 0.0, 0.0), angular_velocity=(0.0, 0.0, -10.0))
+
+
+SCENARIOS = {"box_drop": box_drop, "rover_wheel": rover_wheel}
+
+
+def _seed(sc: GridOnlyScenario):
+    """seed_sand / seed_sand_for_wheel as a dense box at the handle's origin."""
+    dims = tuple(sc.dims)
+    mass = np.zeros(dims, np.float32)
+    vel = np.zeros(dims + (3,), np.float32)
+    o = sc.origin
+    for i in range(-sc.extent_i, sc.extent_i + 1):
+        for j in range(sc.layers_y):
+            for k in range(-sc.extent_k, sc.extent_k + 1):
+                mass[i - o[0], j - o[1], k - o[2]] = np.float32(sc.node_mass)
+    return mass, vel, mass > 0
+
+
+def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool = True):
+    """The grid-only coupled loop on a fresh handle.  Returns (columns,
+    seconds): the time-series rows of FrameExporter::log_step (dict of
+    arrays) and the wall time of the stepping alone when log is False (the
+    per-step host reads of the logging otherwise dominate)."""
+    from .mpm import BODY_DYNAMIC, MODE_REFERENCE, Body, Solver
+
+    n = sc.num_steps() if steps is None else int(steps)
+    S = Solver(sc.dx, sc.origin, sc.dims, 1, MODE_REFERENCE)
+    mass, vel, active = _seed(sc)
+    S.upload_grid(sc.origin, mass, vel, active)
+    m = float(sc.body_mass)
+    # RigidBody::apply_force(gravity) each step: a constant applied force;
+    # make_dynamic(m) sets inv_mass = 1 / m; damping 0 as make_box / make_wheel
+    S.set_body(Body(position=sc.position, orientation=sc.orientation, linear_velocity=sc.linear_velocity,
+                    angular_velocity=sc.angular_velocity, inv_mass=1.0 / m, linear_damping=0.0,
+                    angular_damping=0.0, integrate=BODY_DYNAMIC,
+                    applied_force=(0.0, float(np.float32(-sc.gravity_g) * np.float32(m)), 0.0),
+                    half_extents=sc.half_extents, friction=sc.friction, mass=m, shape=sc.shape))
+    dvy = float(np.float32(-sc.gravity_g) * np.float32(sc.dt))
+    dt = float(np.float32(sc.dt))
+    rows = {k: np.zeros(n, np.float64) for k in COLUMNS}
+    t0 = time.perf_counter()
+    for i in range(n):
+        S.couple_body(method, dt)
+        S.grid_settle(dvy)
+        if not log:
+            continue
+        body, (force, torque, inside) = S.get_body()
+        G = S.download_grid()
+        act = G["active"].astype(bool) & (G["mass"] > 0)
+        p_grid_y = float(np.sum(G["vel"][..., 1][act].astype(np.float64) * G["mass"][act]))
+        v = np.asarray(body.linear_velocity, np.float32)
+        p_body_y = float(v[1] * np.float32(m))
+        rows["t"][i] = float(np.float32(i + 1) * np.float32(sc.dt))
+        rows["box_x"][i], rows["box_y"][i], rows["box_z"][i] = body.position
+        rows["box_vy"][i] = float(v[1])
+        rows["p_body_y"][i] = p_body_y
+        rows["p_grid_y"][i] = p_grid_y
+        rows["p_sum_y"][i] = p_body_y + p_grid_y
+        rows["KE"][i] = float(np.float32(0.5) * np.float32(m) * np.float32(np.dot(v, v)))
+        rows["nodes_inside"][i] = inside
+    S.synchronize()
+    secs = time.perf_counter() - t0
+    S.close()
+    return rows, secs
+
+
+def write_timeseries_csv(path: str, rows: dict) -> None:
+    """FrameExporter's time-series format (frame_exporter.h:25-27, 44-57)."""
+    with open(path, "w") as f:
+        f.write(",".join(COLUMNS) + "\n")
+        for i in range(len(rows["t"])):
+            f.write("%.6f,%.6e,%.6e,%.6e,%.6e,%.8e,%.8e,%.8e,%.6e,%d\n" % (
+                rows["t"][i], rows["box_x"][i], rows["box_y"][i], rows["box_z"][i], rows["box_vy"][i],
+                rows["p_body_y"][i], rows["p_grid_y"][i], rows["p_sum_y"][i], rows["KE"][i],
+                int(rows["nodes_inside"][i])))

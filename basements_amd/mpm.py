@@ -274,6 +274,7 @@ _SIGS = {
                                        C.c_void_p, C.c_void_p, C.c_void_p]),
     "bm_mpm_upload_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bm_mpm_grid_mass_total": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "bm_mpm_grid_settle": (C.c_int, [C.c_void_p, C.c_float]),
     "bm_mpm_mass_quantum_exp": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "bm_mpm_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "bm_mpm_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -481,6 +482,11 @@ class Solver:
         vel = _f32(vel)
         act = np.ascontiguousarray(active, np.uint8)
         _check(lib().bm_mpm_upload_grid(self.h, _ptr(org), _ptr(dims), _ptr(mass), _ptr(vel), _ptr(act)))
+
+    def grid_settle(self, dvy):
+        """The research harness's grid step (bm_mpm_grid_settle): v_y += dvy on
+        active nodes, then clamped at 0."""
+        _check(lib().bm_mpm_grid_settle(self.h, float(dvy)))
 
     def grid_mass_total(self):
         t = C.c_int64()

@@ -74,9 +74,11 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="wheel_bed", choices=["wheel_bed", "box_drop", "batch", "serve", "adjoint"],
+    ap.add_argument("--config", default="wheel_bed",
+                    choices=["wheel_bed", "box_drop", "batch", "serve", "adjoint", "coupler_loops"],
                     help="wheel_bed (the default workload), box_drop, or batch: 64 uneven tool scenes stepped "
-                         "together against the same scenes one at a time on the single-scene path")
+                         "together against the same scenes one at a time on the single-scene path; "
+                         "coupler_loops: the research harness's grid-only M0 / M1 loops")
     ap.add_argument("--scenes", type=int, default=64, help="scenes of --config batch")
     ap.add_argument("--graph", type=int, default=1, help="--config batch: replay captured substeps")
     ap.add_argument("--serial", type=int, default=1, help="--config batch/serve: also time the work one at a time")
@@ -273,6 +275,33 @@ def bench_serve(a):
     return 0
 
 
+def bench_coupler_loops(a):
+    """The research harness's grid-only coupler loops (box drop and rover
+    wheel, M0 and M1; 720 steps each: body gravity, coupler, reaction,
+    integration, grid settling, all on the device), timed without the
+    per-step logging reads.  One JSON line: steps/s over the four loops."""
+    from basements_amd import M0, M1, harness
+
+    runs = []
+    total_steps, total_s = 0, 0.0
+    for scen in ("box_drop", "rover_wheel"):
+        for name, m in (("M0", M0), ("M1", M1)):
+            sc = harness.SCENARIOS[scen]()
+            harness.run(sc, m, steps=min(a.warmup, sc.num_steps()), log=False)  # warm-up
+            _, secs = harness.run(sc, m, log=False)
+            n = sc.num_steps()
+            runs.append({"run": f"{scen}_{name}", "steps": n, "us_per_step": secs / n * 1e6})
+            total_steps += n
+            total_s += secs
+    out = {"metric": "coupler-loop steps/s (grid-only M0/M1 + rigid body)", "value": total_steps / total_s,
+           "unit": "steps/s", "n_gpus": 1, "steps": total_steps, "warmup": a.warmup,
+           "ms_per_step": total_s / total_steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32", "data": "the reference's scenario constants (no external data)",
+           "config": {"workload": "coupler_loops", "dt": 1.0 / 240.0}, "runs": runs}
+    print(json.dumps(out))
+    return 0
+
+
 def bench_adjoint(a):
     """Differentiable tine rollout: 32 x 4 x 32 cells at dx = 1/32 (32,768
     particles), 400 substeps, checkpoint every 16; loss = mean squared final
@@ -441,6 +470,8 @@ def main():
         return bench_serve(a)
     if a.config == "adjoint":
         return bench_adjoint(a)
+    if a.config == "coupler_loops":
+        return bench_coupler_loops(a)
 
     from basements_amd import M0, M1, MODE_FAST, MODE_REFERENCE, Solver
     from basements_amd.scenes import box_drop, wheel_bed

@@ -228,6 +228,11 @@ int bm_mpm_download_grid(bm_mpm_handle h, const int32_t org[3], const int32_t di
  * Leaves the grid in the post-update phase. */
 int bm_mpm_upload_grid(bm_mpm_handle h, const int32_t org[3], const int32_t dims[3], const float* mass,
                        const float* vel, const uint8_t* active);
+/* The research harness's grid step of its grid-only coupler loops
+ * (bench_compare_all.cpp:117-126 box drop, :666-675 rover wheel): on every
+ * active node with mass > 0 of the current (post-update) grid, v_y += dvy,
+ * then v_y = 0 if negative.  dvy is the caller's -g * dt. */
+int bm_mpm_grid_settle(bm_mpm_handle h, float dvy);
 /* Sum of the fixed-point node mass of the last P2G (exactly the sum of the
  * particles' quanta when every particle was inside the domain). */
 int bm_mpm_grid_mass_total(bm_mpm_handle h, int64_t* mass_q_total);

@@ -1,0 +1,38 @@
+"""Write tests/golden/coupler_loops.npz from the reference's committed outputs.
+
+The reference's research harness committed the time series of its grid-only
+coupler loops (research/mpm_rigid_coupling/outputs/csv/{box_drop,rover_wheel}_
+{M0,M1}.csv, written by FrameExporter::log_step, frame_exporter.h:44-57, from
+bench_compare_all.cpp:72-134 and :614-689).  This copies their numbers into
+one .npz (no pickles): key "<scenario>_<method>" -> (steps, 10) float64 in
+the CSV's column order.  tests/test_gpu_coupler_loops.py compares the device
+loops with them.  Runs only where the reference tree is present.
+
+    python tests/golden/make_coupler_loops.py
+"""
+from __future__ import annotations
+
+import csv
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_CSV = "/root/reference/research/mpm_rigid_coupling/outputs/csv"
+RUNS = ("box_drop_M0", "box_drop_M1", "rover_wheel_M0", "rover_wheel_M1")
+COLUMNS = ("t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y", "p_sum_y", "KE", "nodes_inside")
+
+
+def main():
+    out = {}
+    for run in RUNS:
+        with open(os.path.join(REF_CSV, run + ".csv")) as f:
+            rows = list(csv.DictReader(f))
+        out[run] = np.array([[float(r[c]) for c in COLUMNS] for r in rows], np.float64)
+    out["columns"] = np.array(COLUMNS)
+    np.savez_compressed(os.path.join(HERE, "coupler_loops.npz"), **out)
+    print({k: v.shape for k, v in out.items()})
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,75 @@
+"""Host side of the grid-only coupler loops (basements_amd.harness): the
+scenario constants against the reference's scenario headers, the committed
+fixture's shape, and the time-series CSV format of FrameExporter::log_step
+(frame_exporter.h:25-27, 44-57).  The device loops themselves are checked in
+tests/test_gpu_coupler_loops.py."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "coupler_loops.npz")
+
+
+def test_scenarios_follow_the_reference_headers():
+    from basements_amd import harness
+
+    b = harness.box_drop()
+    # scenario_box_drop.h:14-40: 3 s at 1/240 s, 7 x 3 x 7 nodes of 0.01 kg at dx 0.1
+    assert b.num_steps() == 720
+    assert (b.extent_i, b.extent_k, b.layers_y, b.node_mass, b.dx) == (3, 3, 3, 0.01, 0.1)
+    assert b.half_extents == (0.2, 0.2, 0.2) and b.position == (0.0, 0.55, 0.0)
+    w = harness.rover_wheel()
+    # scenario_rover_wheel.h:19-40: R 0.15, half length 0.05, 13 x 3 x 9 nodes, spin -10 rad/s about Z
+    assert w.num_steps() == 720
+    assert (w.extent_i, w.extent_k, w.layers_y) == (6, 4, 3)
+    assert w.half_extents == (0.15, 0.05, 0.15) and w.angular_velocity == (0.0, 0.0, -10.0)
+    # Quaternion::from_axis_angle((1,0,0), 1.5707963f): a quarter turn about X
+    q = np.asarray(w.orientation)
+    assert abs(np.dot(q, q) - 1.0) < 1e-6 and abs(q[0] - q[1]) < 1e-7 and q[2] == q[3] == 0.0
+    # both patches and every pose they reach stay block-aligned with the reference's 4^3 blocks
+    for sc in (b, w):
+        assert all(o % 4 == 0 for o in sc.origin) and all(d % 4 == 0 for d in sc.dims)
+
+
+def test_seeded_patch_matches_seed_sand():
+    from basements_amd import harness
+
+    sc = harness.box_drop()
+    mass, vel, active = harness._seed(sc)
+    assert int(active.sum()) == 7 * 3 * 7
+    assert np.all(mass[active] == np.float32(0.01)) and not vel.any()
+    ii, jj, kk = np.nonzero(active)
+    o = sc.origin
+    assert (ii + o[0]).min() == -3 and (ii + o[0]).max() == 3
+    assert (jj + o[1]).min() == 0 and (jj + o[1]).max() == 2
+
+
+def test_fixture_holds_the_four_committed_series():
+    ref = np.load(GOLDEN)
+    assert [str(c) for c in ref["columns"]] == ["t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y",
+                                                "p_sum_y", "KE", "nodes_inside"]
+    for run in ("box_drop_M0", "box_drop_M1", "rover_wheel_M0", "rover_wheel_M1"):
+        R = ref[run]
+        assert R.shape == (720, 10)
+        np.testing.assert_allclose(R[:, 0], (np.arange(720) + 1) / 240.0, atol=1e-6)
+        # free fall before the first contact: v_y = -g t, as the integrator gives
+        k = int(np.argmax(R[:, 9] > 0))
+        n = min(k, 5)  # the printed t is rounded to 1e-6: use (i + 1) dt itself
+        np.testing.assert_allclose(R[:n, 4], -9.81 * (np.arange(n) + 1) / 240.0, rtol=1e-5)
+
+
+def test_csv_writer_matches_the_reference_format(tmp_path):
+    from basements_amd import harness
+
+    rows = {k: np.zeros(2) for k in harness.COLUMNS}
+    rows["t"][:] = (1 / 240, 2 / 240)
+    rows["nodes_inside"][:] = (0, 3)
+    p = tmp_path / "ts.csv"
+    harness.write_timeseries_csv(str(p), rows)
+    lines = p.read_text().splitlines()
+    assert lines[0] == "t,box_x,box_y,box_z,box_vy,p_body_y,p_grid_y,p_sum_y,KE,nodes_inside"
+    assert lines[1] == ("0.004167,0.000000e+00,0.000000e+00,0.000000e+00,0.000000e+00,0.00000000e+00,"
+                        "0.00000000e+00,0.00000000e+00,0.000000e+00,0")
+    assert lines[2].endswith(",3")
