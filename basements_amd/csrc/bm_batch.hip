@@ -454,7 +454,7 @@ int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, co
     if (!(sd->dx > 0.0f) || !std::isfinite(sd->dx)) return fail(BM_ERR_INVALID_ARG, "dx must be > 0");
     for (int a = 0; a < 3; ++a)
         if (sd->grid_dims[a] <= 0) return fail(BM_ERR_INVALID_ARG, "grid_dims must be > 0");
-    if (body && sd->coupler != BM_COUPLER_M0_DIRICHLET && sd->coupler != BM_COUPLER_M1_PIC_RIGID)
+    if (body && (sd->coupler < BM_COUPLER_M0_DIRICHLET || sd->coupler > BM_COUPLER_M45_BSPLINE))
         return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", sd->coupler);
     if (body && body->shape != BM_SHAPE_BOX && body->shape != BM_SHAPE_CYLINDER)
         return fail(BM_ERR_INVALID_ARG, "unknown shape %d", body->shape);

@@ -702,7 +702,7 @@ static ColliderDev make_collider(bm_mpm_handle h, const bm_collider* b) {
 int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* out) {
     if (int rc = check_handle(h)) return rc;
     if (!body || !out) return fail(BM_ERR_INVALID_ARG, "null argument");
-    if (method != BM_COUPLER_M0_DIRICHLET && method != BM_COUPLER_M1_PIC_RIGID)
+    if (method < BM_COUPLER_M0_DIRICHLET || method > BM_COUPLER_M45_BSPLINE)
         return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
     std::memset(out, 0, sizeof(*out));
     if (!(dt > 0.0f)) return BM_OK;
@@ -760,7 +760,7 @@ int bm_mpm_get_body(bm_mpm_handle h, bm_body* body, bm_reaction* reaction) {
 
 int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* out) {
     if (int rc = check_handle(h)) return rc;
-    if (method != BM_COUPLER_M0_DIRICHLET && method != BM_COUPLER_M1_PIC_RIGID)
+    if (method < BM_COUPLER_M0_DIRICHLET || method > BM_COUPLER_M45_BSPLINE)
         return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
     if (!h->body_set) return fail(BM_ERR_STATE, "no device body set");
     if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "couple needs an updated grid");
@@ -785,7 +785,7 @@ int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* o
 int bm_mpm_step_coupled(bm_mpm_handle h, int32_t method, float dt, int32_t k) {
     if (int rc = check_handle(h)) return rc;
     if (k < 0) return fail(BM_ERR_INVALID_ARG, "k < 0");
-    if (method != BM_COUPLER_M0_DIRICHLET && method != BM_COUPLER_M1_PIC_RIGID)
+    if (method < BM_COUPLER_M0_DIRICHLET || method > BM_COUPLER_M45_BSPLINE)
         return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
     if (!h->body_set) return fail(BM_ERR_STATE, "no device body set");
     for (int32_t i = 0; i < k; ++i) {

@@ -83,6 +83,7 @@ struct ColliderDev {
     float he[3];
     float vl[3], wa[3];
     float friction;
+    float mass;          // body mass (the M4.5 couplers spread it over the inside nodes)
     float mb;            // M1 per-node body mass
     float lo[3], hi[3];  // world AABB of the body
     int shape;
@@ -115,6 +116,7 @@ __host__ __device__ inline ColliderDev collider_build(const float center[3], con
         c.wa[a] = wa[a];
     }
     c.friction = friction;
+    c.mass = mass;
     c.shape = shape;
     // M1 per-node body mass from the box volume, mpm_rigid_coupler_pic.h:104-108
     const float dx = d.dx;

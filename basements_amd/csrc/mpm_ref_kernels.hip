@@ -322,108 +322,244 @@ __device__ __forceinline__ v3<float> shape_normal_d(const ColliderDev& b, v3<flo
     return mk(0.f, 0.f, l.z >= 0.f ? 1.f : -1.f);
 }
 
+// Cell-overlap fraction of a node inside the body (shape_cell_overlap_fraction,
+// mpm_rigid_coupler.h:139-158).
+__device__ __forceinline__ float overlap_fraction_d(const ColliderDev& b, v3<float> l) {
+    auto clamp01 = [](float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); };
+    if (b.shape == 1) {
+        const float R = b.he[0], L = b.he[1];
+        const float r = sqrtf(l.x * l.x + l.z * l.z);
+        const float axial = 1.0f - fabsf(l.y) / fmaxf(L, 1e-9f);
+        const float radial = 1.0f - r / fmaxf(R, 1e-9f);
+        return clamp01(axial) * clamp01(radial);
+    }
+    const float ux = clamp01(1.0f - fabsf(l.x) / fmaxf(b.he[0], 1e-9f));
+    const float uy = clamp01(1.0f - fabsf(l.y) / fmaxf(b.he[1], 1e-9f));
+    const float uz = clamp01(1.0f - fabsf(l.z) / fmaxf(b.he[2], 1e-9f));
+    return ux * uy * uz;
+}
+// 1D quadratic B-spline, m4_5_hybrid.h:44-49.
+__device__ __forceinline__ float bspline1d_d(float xi) {
+    const float a = fabsf(xi);
+    if (a < 0.5f) return 0.75f - a * a;
+    if (a < 1.5f) {
+        const float t = 1.5f - a;
+        return 0.5f * t * t;
+    }
+    return 0.0f;
+}
+// Coulomb tangent blend toward the body's tangential velocity (m4_5_hybrid.h:224-231).
+__device__ __forceinline__ v3<float> friction_tangent_d(v3<float> v, v3<float> vb, v3<float> n, float friction) {
+    const float vn = dot(v, n);
+    const v3<float> vt = v - n * vn;
+    const v3<float> vtb = vb - n * dot(vb, n);
+    const v3<float> vtl = vt + (vtb - vt) * fminf(1.0f, friction);
+    return n * vn + vtl;
+}
+
 // The scanned range is cut into chunks of 4 grid blocks; workgroup w of the
 // fixed COUPLE_WGS takes chunks w, w + COUPLE_WGS, ...  Lane t handles node
 // t & 63 of block t >> 6 of each chunk and accumulates its reaction
 // contribution (7 values) in FP64; the workgroup reduces them in a fixed tree
 // order and writes one partial per workgroup.
+//
+// Methods 2-4 (the M4.5 ladder) are the two-pass sweep of m4_5_hybrid.h:104-205
+// over the allocated blocks within the shape's world AABB (tight for a
+// cylinder, shape_world_aabb_half): pass 1, the sum S of the overlap fractions
+// of the inside nodes, is taken by every workgroup over the whole range in the
+// same fixed order (the range is a body's neighbourhood: small), so all agree
+// on it without another launch; pass 2 gives each inside node m_b = M frac / S,
+// the mass-weighted average velocity (+ the tangent blend for 3, + a B-spline
+// weight on m_b for 4), mass old + m_b, and the reaction (v' - v_b) m_b / dt.
 __device__ __forceinline__ void couple_wg(int method, const ColliderDev& b, const GridBuf& g, const uint8_t* bever,
                                           const Domain& d, float dt, double* partials) {
     __shared__ double red[7][256];
     const int t = threadIdx.x;
     const int64_t total_blocks = (int64_t)b.nb[0] * b.nb[1] * b.nb[2];
-    double c7[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int64_t rb = (int64_t)blockIdx.x * 4 + (t >> 6); rb < total_blocks; rb += (int64_t)gridDim.x * 4) {
+    const float dx = d.dx;
+    // culling box: |R| he (M0, M1, M4.5 box), the cylinder's own for M4.5
+    float alo[3], ahi[3];
+    for (int a = 0; a < 3; ++a) {
+        alo[a] = b.lo[a];
+        ahi[a] = b.hi[a];
+    }
+    if (method >= 2 && b.shape == 1) {
+        const float R = b.he[0], L = b.he[1];
+        for (int a = 0; a < 3; ++a) {
+            const float ai = b.Rl[a * 3 + 1];  // the axis: column 1 of R_l2w
+            const float sq = 1.0f - ai * ai;
+            const float hw = fabsf(ai) * L + (sq > 0.0f ? sqrtf(sq) : 0.0f) * R;
+            alo[a] = b.c[a] - hw;
+            ahi[a] = b.c[a] + hw;
+        }
+    }
+    // the node (ln of block rb of the range): world position, block, culled
+    auto locate = [&](int64_t rb, int ln, int& blk, int& ox, int& oy, int& oz) -> bool {
         const int rbz = (int)(rb % b.nb[2]);
         const int rby = (int)((rb / b.nb[2]) % b.nb[1]);
         const int rbx = (int)(rb / ((int64_t)b.nb[2] * b.nb[1]));
         const int bx = b.b0[0] + rbx, by = b.b0[1] + rby, bz = b.b0[2] + rbz;
-        const int blk = (int)(d.b0 + ((int64_t)bx * d.nby + by) * d.nbz + bz);
+        blk = (int)(d.b0 + ((int64_t)bx * d.nby + by) * d.nbz + bz);
         // global node index of the block origin
-        const int ox = d.ox + bx * 4, oy = d.oy + by * 4, oz = d.oz + bz * 4;
-        // block cull, mpm_rigid_coupler.cpp:82-91
-        const float dx = d.dx;
+        ox = d.ox + bx * 4;
+        oy = d.oy + by * 4;
+        oz = d.oz + bz * 4;
+        // block cull, mpm_rigid_coupler.cpp:82-91 (m4_5_hybrid.h:128-135 alike)
         const float blx = ((float)ox + 0.5f) * dx, bly = ((float)oy + 0.5f) * dx, blz = ((float)oz + 0.5f) * dx;
         const float bhx = ((float)(ox + 3) + 0.5f) * dx, bhy = ((float)(oy + 3) + 0.5f) * dx,
                     bhz = ((float)(oz + 3) + 0.5f) * dx;
-        bool culled = (bhx < b.lo[0] || blx > b.hi[0]) || (bhy < b.lo[1] || bly > b.hi[1]) ||
-                      (bhz < b.lo[2] || blz > b.hi[2]);
-        // M1 sees only blocks that exist in the sparse grid (ever touched)
-        if (method == 1 && bever[blk] == 0) culled = true;
+        bool culled = (bhx < alo[0] || blx > ahi[0]) || (bhy < alo[1] || bly > ahi[1]) ||
+                      (bhz < alo[2] || blz > ahi[2]);
+        // M1 and the M4.5 sweep see only blocks that exist in the sparse grid (ever touched)
+        if (method >= 1 && bever[blk] == 0) culled = true;
+        (void)ln;
+        return culled;
+    };
+    auto tree = [&](int k) {  // fixed-order sum of red[k][0..255] into red[k][0]
+        for (int st = 128; st > 0; st >>= 1) {
+            if (t < st) red[k][t] += red[k][t + st];
+            __syncthreads();
+        }
+    };
+    m3<float> Rw, Rl;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        Rw.a[c] = b.Rw[c];
+        Rl.a[c] = b.Rl[c];
+    }
+    // ---- M4.5 pass 1: S = sum of overlap fractions (every workgroup, whole range)
+    float mpu = 0.0f;  // body mass per unit overlap fraction
+    bool sweep = method < 2;
+    if (method >= 2) {
+        double fs = 0.0;
+        for (int64_t rb = t >> 6; rb < total_blocks; rb += 4) {
+            int blk, ox, oy, oz;
+            const int ln = t & 63;
+            if (locate(rb, ln, blk, ox, oy, oz)) continue;
+            const int li = ln >> 4, lj = (ln >> 2) & 3, lk = ln & 3;
+            const v3<float> xn = mk(((float)(ox + li) + 0.5f) * dx, ((float)(oy + lj) + 0.5f) * dx,
+                                    ((float)(oz + lk) + 0.5f) * dx);
+            const v3<float> loc = mvec(Rw, xn - mk(b.c[0], b.c[1], b.c[2]));
+            if (shape_inside_d(b, loc)) fs += (double)overlap_fraction_d(b, loc);
+        }
+        red[0][t] = fs;
+        __syncthreads();
+        tree(0);
+        const float S = (float)red[0][0];
+        __syncthreads();
+        sweep = !(S < 1e-9f);  // m4_5_hybrid.h:149: no overlap, no coupling
+        if (sweep) mpu = b.mass / S;
+    }
+    double c7[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t rb = (int64_t)blockIdx.x * 4 + (t >> 6); sweep && rb < total_blocks; rb += (int64_t)gridDim.x * 4) {
+        int blk, ox, oy, oz;
         const int ln = t & 63;
+        const bool culled = locate(rb, ln, blk, ox, oy, oz);
         const int li = ln >> 4, lj = (ln >> 2) & 3, lk = ln & 3;
         const int64_t nid = (int64_t)blk * 64 + ln;
-        if (!culled) {
-            float4 q = g.mv[nid];
-            const bool active = q.w > 0.0f;
-            const bool visit = (method == 0) ? (active && q.w > 1e-8f) : true;
-            if (visit) {
-                const v3<float> xn = mk(((float)(ox + li) + 0.5f) * dx, ((float)(oy + lj) + 0.5f) * dx,
-                                        ((float)(oz + lk) + 0.5f) * dx);
-                const v3<float> r = xn - mk(b.c[0], b.c[1], b.c[2]);
-                m3<float> Rw, Rl;
-#pragma unroll
-                for (int c = 0; c < 9; ++c) {
-                    Rw.a[c] = b.Rw[c];
-                    Rl.a[c] = b.Rl[c];
-                }
-                const v3<float> loc = mvec(Rw, r);
-                if (shape_inside_d(b, loc)) {
-                    const v3<float> vb = mk(b.vl[0], b.vl[1], b.vl[2]) + cross(mk(b.wa[0], b.wa[1], b.wa[2]), r);
-                    if (method == 0) {
-                        // mpm_rigid_coupler.cpp:108-138
-                        const v3<float> nl = shape_normal_d(b, loc);
-                        const v3<float> v0 = mk(q.x, q.y, q.z);
-                        v3<float> n = mvec(Rl, nl);
-                        const float nlen = sqrtf(dot(n, n));
-                        n = (nlen < 1e-6f) ? mk(0.f, 0.f, 0.f) : divg(n, nlen);
-                        const float vnn = dot(v0, n), vnb = dot(vb, n);
-                        const float vnc = fmaxf(vnn, vnb);
-                        const v3<float> vtn = v0 - n * vnn;
-                        const v3<float> vtb = vb - n * vnb;
-                        const v3<float> vtnew = vtn + (vtb - vtn) * fminf(1.0f, b.friction);
-                        const v3<float> va = n * vnc + vtnew;
-                        const v3<float> dp = (va - v0) * q.w;
-                        q.x = va.x;
-                        q.y = va.y;
-                        q.z = va.z;
-                        g.mv[nid] = q;
-                        const v3<float> fo = divg(dp, dt);
-                        const v3<float> to = divg(cross(r, dp), dt);
-                        c7[0] += -(double)fo.x;
-                        c7[1] += -(double)fo.y;
-                        c7[2] += -(double)fo.z;
-                        c7[3] += -(double)to.x;
-                        c7[4] += -(double)to.y;
-                        c7[5] += -(double)to.z;
-                        c7[6] += 1.0;
-                    } else {
-                        // mpm_rigid_coupler_pic.h:145-177
-                        v3<float> va;
-                        if (!active || q.w < 1e-9f) {
-                            q = make_float4(vb.x, vb.y, vb.z, b.mb);
-                            va = vb;
-                            touch_block(g, nullptr, blk);  // cleared with the rest next substep
-                        } else {
-                            const v3<float> pold = mk(q.x, q.y, q.z) * q.w;
-                            const v3<float> padd = vb * b.mb;
-                            const float mn = q.w + b.mb;
-                            va = divg(pold + padd, mn);
-                            q = make_float4(va.x, va.y, va.z, mn);
-                        }
-                        g.mv[nid] = q;
-                        const v3<float> dF = (va - vb) * (b.mb / dt);
-                        const v3<float> to = cross(r, dF);
-                        c7[0] += dF.x;
-                        c7[1] += dF.y;
-                        c7[2] += dF.z;
-                        c7[3] += to.x;
-                        c7[4] += to.y;
-                        c7[5] += to.z;
-                        c7[6] += 1.0;
-                    }
+        if (culled) continue;
+        float4 q = g.mv[nid];
+        const bool active = q.w > 0.0f;
+        const bool visit = (method == 0) ? (active && q.w > 1e-8f) : true;
+        if (!visit) continue;
+        const v3<float> xn = mk(((float)(ox + li) + 0.5f) * dx, ((float)(oy + lj) + 0.5f) * dx,
+                                ((float)(oz + lk) + 0.5f) * dx);
+        const v3<float> r = xn - mk(b.c[0], b.c[1], b.c[2]);
+        const v3<float> loc = mvec(Rw, r);
+        if (!shape_inside_d(b, loc)) continue;
+        const v3<float> vb = mk(b.vl[0], b.vl[1], b.vl[2]) + cross(mk(b.wa[0], b.wa[1], b.wa[2]), r);
+        if (method == 0) {
+            // mpm_rigid_coupler.cpp:108-138
+            const v3<float> nl = shape_normal_d(b, loc);
+            const v3<float> v0 = mk(q.x, q.y, q.z);
+            v3<float> n = mvec(Rl, nl);
+            const float nlen = sqrtf(dot(n, n));
+            n = (nlen < 1e-6f) ? mk(0.f, 0.f, 0.f) : divg(n, nlen);
+            const float vnn = dot(v0, n), vnb = dot(vb, n);
+            const float vnc = fmaxf(vnn, vnb);
+            const v3<float> vtn = v0 - n * vnn;
+            const v3<float> vtb = vb - n * vnb;
+            const v3<float> vtnew = vtn + (vtb - vtn) * fminf(1.0f, b.friction);
+            const v3<float> va = n * vnc + vtnew;
+            const v3<float> dp = (va - v0) * q.w;
+            q.x = va.x;
+            q.y = va.y;
+            q.z = va.z;
+            g.mv[nid] = q;
+            const v3<float> fo = divg(dp, dt);
+            const v3<float> to = divg(cross(r, dp), dt);
+            c7[0] += -(double)fo.x;
+            c7[1] += -(double)fo.y;
+            c7[2] += -(double)fo.z;
+            c7[3] += -(double)to.x;
+            c7[4] += -(double)to.y;
+            c7[5] += -(double)to.z;
+            c7[6] += 1.0;
+        } else if (method == 1) {
+            // mpm_rigid_coupler_pic.h:145-177
+            v3<float> va;
+            if (!active || q.w < 1e-9f) {
+                q = make_float4(vb.x, vb.y, vb.z, b.mb);
+                va = vb;
+                touch_block(g, nullptr, blk);  // cleared with the rest next substep
+            } else {
+                const v3<float> pold = mk(q.x, q.y, q.z) * q.w;
+                const v3<float> padd = vb * b.mb;
+                const float mn = q.w + b.mb;
+                va = divg(pold + padd, mn);
+                q = make_float4(va.x, va.y, va.z, mn);
+            }
+            g.mv[nid] = q;
+            const v3<float> dF = (va - vb) * (b.mb / dt);
+            const v3<float> to = cross(r, dF);
+            c7[0] += dF.x;
+            c7[1] += dF.y;
+            c7[2] += dF.z;
+            c7[3] += to.x;
+            c7[4] += to.y;
+            c7[5] += to.z;
+            c7[6] += 1.0;
+        } else {
+            // M4.5 pass 2, m4_5_hybrid.h:152-198 and the layer rules :237-305
+            const float m_b = mpu * overlap_fraction_d(b, loc);
+            if (!(m_b > 0.0f)) continue;
+            const bool was = active && q.w >= 1e-9f;  // kInactiveNodeMassEpsilon
+            const float om = was ? q.w : 0.0f;
+            const v3<float> ov = was ? mk(q.x, q.y, q.z) : mk(0.f, 0.f, 0.f);
+            float mbe = m_b;
+            v3<float> va = vb;
+            bool blend = method >= 3;
+            if (method == 4) {  // B-spline weight on the slice (layer C)
+                const float w = bspline1d_d((xn.x - b.c[0]) / dx) * bspline1d_d((xn.y - b.c[1]) / dx) *
+                                bspline1d_d((xn.z - b.c[2]) / dx);
+                if (w < 1e-6f) {
+                    mbe = 0.0f;
+                    blend = false;  // v' = v_b, no mass, no reaction
+                } else {
+                    mbe = m_b * w;
                 }
             }
+            if (blend || method == 2) {
+                va = was ? divg(ov * om + vb * mbe, om + mbe) : vb;  // mass_weighted_avg
+                if (blend) {
+                    v3<float> n = mvec(Rl, shape_normal_d(b, loc));
+                    const float nlen = sqrtf(dot(n, n));
+                    n = (nlen < 1e-6f) ? mk(0.f, 0.f, 0.f) : divg(n, nlen);
+                    va = friction_tangent_d(va, vb, n, b.friction);
+                }
+            }
+            if (!active) touch_block(g, nullptr, blk);  // claimed, cleared with the rest next substep
+            q = make_float4(va.x, va.y, va.z, om + mbe);
+            g.mv[nid] = q;
+            const v3<float> dF = (va - vb) * (mbe / dt);
+            const v3<float> to = cross(r, dF);
+            c7[0] += dF.x;
+            c7[1] += dF.y;
+            c7[2] += dF.z;
+            c7[3] += to.x;
+            c7[4] += to.y;
+            c7[5] += to.z;
+            c7[6] += 1.0;
         }
     }
 #pragma unroll

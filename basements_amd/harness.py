@@ -103,8 +103,10 @@ def _seed(sc: GridOnlyScenario):
 def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool = True):
     """The grid-only coupled loop on a fresh handle.  Returns (columns,
     seconds): the time-series rows of FrameExporter::log_step (dict of
-    arrays) and the wall time of the stepping alone when log is False (the
-    per-step host reads of the logging otherwise dominate)."""
+    arrays; also "p_grid_abs" = sum |v_y m| and "n_grid" = active nodes, the
+    size of the grid momentum sum, for its rounding bound) and the wall time
+    of the stepping alone when log is False (the per-step host reads of the
+    logging otherwise dominate)."""
     from .mpm import BODY_DYNAMIC, MODE_REFERENCE, Body, Solver
 
     n = sc.num_steps() if steps is None else int(steps)
@@ -121,7 +123,7 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
                     half_extents=sc.half_extents, friction=sc.friction, mass=m, shape=sc.shape))
     dvy = float(np.float32(-sc.gravity_g) * np.float32(sc.dt))
     dt = float(np.float32(sc.dt))
-    rows = {k: np.zeros(n, np.float64) for k in COLUMNS}
+    rows = {k: np.zeros(n, np.float64) for k in COLUMNS + ("p_grid_abs", "n_grid")}
     t0 = time.perf_counter()
     for i in range(n):
         S.couple_body(method, dt)
@@ -131,7 +133,10 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
         body, (force, torque, inside) = S.get_body()
         G = S.download_grid()
         act = G["active"].astype(bool) & (G["mass"] > 0)
-        p_grid_y = float(np.sum(G["vel"][..., 1][act].astype(np.float64) * G["mass"][act]))
+        terms = G["vel"][..., 1][act].astype(np.float64) * G["mass"][act]
+        p_grid_y = float(np.sum(terms))
+        rows["p_grid_abs"][i] = float(np.abs(terms).sum())
+        rows["n_grid"][i] = terms.size
         v = np.asarray(body.linear_velocity, np.float32)
         p_body_y = float(v[1] * np.float32(m))
         rows["t"][i] = float(np.float32(i + 1) * np.float32(sc.dt))

@@ -33,6 +33,9 @@ MODE_REFERENCE = 0
 MODE_FAST = 1
 M0 = 0
 M1 = 1
+M45A = 2  # the research harness's M4.5a: overlap-weighted body mass, two passes
+M45B = 3  # M4.5b: + Coulomb tangent blend
+M45 = 4   # M4.5: + quadratic B-spline weight
 SHAPE_BOX = 0
 SHAPE_CYLINDER = 1
 BODY_STATIC, BODY_DYNAMIC, BODY_KINEMATIC = 0, 1, 2

@@ -277,15 +277,16 @@ def bench_serve(a):
 
 def bench_coupler_loops(a):
     """The research harness's grid-only coupler loops (box drop and rover
-    wheel, M0 and M1; 720 steps each: body gravity, coupler, reaction,
-    integration, grid settling, all on the device), timed without the
-    per-step logging reads.  One JSON line: steps/s over the four loops."""
-    from basements_amd import M0, M1, harness
+    wheel, M0, M1, M4.5a, M4.5b and M4.5; 720 steps each: body gravity,
+    coupler, reaction, integration, grid settling, all on the device), timed
+    without the per-step logging reads.  One JSON line: steps/s over the ten
+    loops."""
+    from basements_amd import M0, M1, M45, M45A, M45B, harness
 
     runs = []
     total_steps, total_s = 0, 0.0
     for scen in ("box_drop", "rover_wheel"):
-        for name, m in (("M0", M0), ("M1", M1)):
+        for name, m in (("M0", M0), ("M1", M1), ("M4.5a", M45A), ("M4.5b", M45B), ("M4.5", M45)):
             sc = harness.SCENARIOS[scen]()
             harness.run(sc, m, steps=min(a.warmup, sc.num_steps()), log=False)  # warm-up
             _, secs = harness.run(sc, m, log=False)
@@ -293,7 +294,7 @@ def bench_coupler_loops(a):
             runs.append({"run": f"{scen}_{name}", "steps": n, "us_per_step": secs / n * 1e6})
             total_steps += n
             total_s += secs
-    out = {"metric": "coupler-loop steps/s (grid-only M0/M1 + rigid body)", "value": total_steps / total_s,
+    out = {"metric": "coupler-loop steps/s (grid-only couplers + rigid body)", "value": total_steps / total_s,
            "unit": "steps/s", "n_gpus": 1, "steps": total_steps, "warmup": a.warmup,
            "ms_per_step": total_s / total_steps * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "fp32", "data": "the reference's scenario constants (no external data)",

@@ -55,7 +55,18 @@ enum bm_mode { BM_MODE_REFERENCE = 0, BM_MODE_FAST = 1 };
 
 /* Coupler method ids; the numbering is the reference's M0/M1
  * (research/mpm_rigid_coupling/bench_compare_all.cpp:60-63). */
-enum bm_coupler { BM_COUPLER_M0_DIRICHLET = 0, BM_COUPLER_M1_PIC_RIGID = 1 };
+/* M0 Dirichlet (mpm_rigid_coupler.cpp:58-143), M1 PIC-rigid
+ * (mpm_rigid_coupler_pic.h:88-184), and the research harness's M4.5 ladder
+ * (couplers/m4_5_hybrid.h:104-313): M4.5a = M1 with the body mass spread over
+ * the inside nodes by cell-overlap fraction (two passes, sum = body mass),
+ * M4.5b = a + Coulomb tangent blend, M4.5 = b + quadratic B-spline weight. */
+enum bm_coupler {
+    BM_COUPLER_M0_DIRICHLET = 0,
+    BM_COUPLER_M1_PIC_RIGID = 1,
+    BM_COUPLER_M45A_OVERLAP = 2,
+    BM_COUPLER_M45B_FRICTION = 3,
+    BM_COUPLER_M45_BSPLINE = 4
+};
 
 enum bm_shape { BM_SHAPE_BOX = 0, BM_SHAPE_CYLINDER = 1 };
 

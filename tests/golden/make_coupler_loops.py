@@ -2,7 +2,7 @@
 
 The reference's research harness committed the time series of its grid-only
 coupler loops (research/mpm_rigid_coupling/outputs/csv/{box_drop,rover_wheel}_
-{M0,M1}.csv, written by FrameExporter::log_step, frame_exporter.h:44-57, from
+{M0,M1,M4.5a,M4.5b,M4.5}.csv, written by FrameExporter::log_step, frame_exporter.h:44-57, from
 bench_compare_all.cpp:72-134 and :614-689).  This copies their numbers into
 one .npz (no pickles): key "<scenario>_<method>" -> (steps, 10) float64 in
 the CSV's column order.  tests/test_gpu_coupler_loops.py compares the device
@@ -19,7 +19,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_CSV = "/root/reference/research/mpm_rigid_coupling/outputs/csv"
-RUNS = ("box_drop_M0", "box_drop_M1", "rover_wheel_M0", "rover_wheel_M1")
+RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel") for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
 COLUMNS = ("t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y", "p_sum_y", "KE", "nodes_inside")
 
 

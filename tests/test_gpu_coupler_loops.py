@@ -1,7 +1,7 @@
 """The reference's grid-only coupler loops on the device against the time
 series the reference committed (research/mpm_rigid_coupling/outputs/csv/
-{box_drop,rover_wheel}_{M0,M1}.csv, copied into tests/golden/coupler_loops.npz
-by tests/golden/make_coupler_loops.py).
+{box_drop,rover_wheel}_{M0,M1,M4.5a,M4.5b,M4.5}.csv, copied into
+tests/golden/coupler_loops.npz by tests/golden/make_coupler_loops.py).
 
 Each loop runs 720 steps of: body gravity, M0 or M1 on the seeded sand grid,
 reaction into the body, TimeIntegrator::integrate, grid settling -- the
@@ -11,7 +11,15 @@ own print precision (%.6e / %.8e: 7 significant digits), i.e. 1e-6 of the
 column family's scale (positions: the largest |coordinate| of the run,
 velocities and momenta: the largest |v_y|, energy: the largest KE).  The
 reaction sums run in a different order than the reference's (a fixed-order
-FP64 tree against hash-map order in FP32), which stays far below that."""
+FP64 tree against hash-map order in FP32), which stays below that for the
+body.  The logged grid momentum (p_grid_y, p_sum_y) is a sum over the ~360
+persistent nodes that the reference takes in FP32 (hash-map order) and the
+harness in FP64, and the nodes carry that reaction-order perturbation from
+step to step: its bar is 5e-6 of scale plus the recursive-summation bound of
+the FP32 sum, (n - 1) 2^-24 sum |v_y m| (Higham, Accuracy and Stability,
+eq. 4.4).  Measured worst case: 2.0e-6 of scale (rover wheel M4.5b, step 475,
+the wheel rebounding under the tangent blend), with the body's v_y there
+within 6e-8 of scale and nodes_inside equal throughout."""
 from __future__ import annotations
 
 import os
@@ -22,18 +30,19 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "coupler_loops.npz")
-RUNS = ("box_drop_M0", "box_drop_M1", "rover_wheel_M0", "rover_wheel_M1")
+RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel") for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
 
 
 @pytest.mark.parametrize("run", RUNS)
 def test_grid_only_loop_reproduces_the_committed_series(run):
-    from basements_amd import M0, M1, harness
+    from basements_amd import M0, M1, M45, M45A, M45B, harness
 
     ref = np.load(GOLDEN)
     cols = [str(c) for c in ref["columns"]]
     R = ref[run]
     scen, method = run.rsplit("_", 1)
-    rows, _ = harness.run(harness.SCENARIOS[scen](), {"M0": M0, "M1": M1}[method])
+    rows, _ = harness.run(harness.SCENARIOS[scen](), {"M0": M0, "M1": M1, "M4.5a": M45A, "M4.5b": M45B,
+                                                       "M4.5": M45}[method])
     G = np.stack([rows[c] for c in cols], 1)
     assert G.shape == R.shape
     np.testing.assert_allclose(G[:, 0], R[:, 0], atol=1e-6)  # t, printed %.6f
@@ -43,8 +52,50 @@ def test_grid_only_loop_reproduces_the_committed_series(run):
     vel = max(np.abs(R[:, c["box_vy"]]).max(), np.abs(R[:, c["p_grid_y"]]).max())
     families = {"box_x": pos, "box_y": pos, "box_z": pos, "box_vy": vel, "p_body_y": vel, "p_grid_y": vel,
                 "p_sum_y": vel, "KE": np.abs(R[:, c["KE"]]).max()}
+    fp32_sum = np.maximum(rows["n_grid"] - 1, 0) * 2.0 ** -24 * rows["p_grid_abs"]
     for name, scale in families.items():
         d = np.abs(G[:, c[name]] - R[:, c[name]])
-        k = int(np.argmax(d))
-        assert d[k] <= 1e-6 * scale, (run, name, k, G[k, c[name]], R[k, c[name]], scale)
+        grid = name in ("p_grid_y", "p_sum_y")
+        bar = (5e-6 * scale + fp32_sum) if grid else 1e-6 * scale
+        k = int(np.argmax(d - bar))
+        assert d[k] <= bar[k] if np.ndim(bar) else d[k] <= bar, (run, name, k, G[k, c[name]], R[k, c[name]], scale)
 
+
+
+@pytest.mark.parametrize("method", ["M4.5a", "M4.5b", "M4.5"])
+def test_m45_couplers_in_the_coupled_substep_batch_equals_alone(method):
+    """The M4.5 methods as the coupler of the particle substep
+    (bm_mpm_step_coupled) and of a batch scene: the box presses into the soil
+    (nodes inside, finite reaction and state), and the scene in a batch
+    evolves as it does alone (the batch tests' 1e-4 bar)."""
+    import math
+
+    from basements_amd import BODY_DYNAMIC, M45, M45A, M45B, SHAPE_BOX, Batch, Body, SceneDesc
+    from _util import assert_close_field, gpu_solver, params_from, random_scene
+
+    m = {"M4.5a": M45A, "M4.5b": M45B, "M4.5": M45}[method]
+    P = random_scene(41, dx=1 / 32, cells_lo=(6, 0, 6), cells_hi=(16, 8, 16), vel=0.1, strain=0.01, cgrad=0.5)
+    prm = params_from(E=3.5e5, alpha=0.386)
+    q = (math.cos(0.1), 0.0, math.sin(0.1), 0.0)
+    body = Body(position=(0.35, 0.27, 0.35), orientation=q, linear_velocity=(0.0, -0.3, 0.0),
+                angular_velocity=(0.0, 0.2, 0.0), inv_mass=0.5, linear_damping=0.0, angular_damping=0.0,
+                applied_force=(0.0, -9.81 * 2.0, 0.0), half_extents=(0.08, 0.05, 0.06), friction=0.5, mass=2.0,
+                shape=SHAPE_BOX, integrate=BODY_DYNAMIC)
+    sd = SceneDesc(dx=1 / 32, origin=(-4, -4, -4), dims=(32, 20, 32), E=prm.E, nu=prm.nu, rho=prm.rho,
+                   alpha=prm.alpha, mu_floor=prm.mu_floor, coupler=m)
+    k, dt = 12, 2e-4
+    S = gpu_solver(P, sd.dx, sd.origin, sd.dims, prm, mode=1)
+    S.set_body(body)
+    S.step_coupled(m, dt, k)
+    alone = S.download()
+    b_alone, (force, torque, inside) = S.get_body()
+    assert inside > 0 and np.isfinite(force).all() and np.isfinite(torque).all()
+    for key in ("x", "v", "F", "C"):
+        assert np.isfinite(alone[key]).all(), key
+    B = Batch(max_particles=100_000, max_blocks=20_000, max_scenes=2)
+    sid = B.add_scene(sd, P, body)
+    B.step(dt, k)
+    got = B.download_scene(sid)
+    for key in ("x", "v", "F", "C"):
+        assert_close_field(got[key], alone[key], 1e-4, f"{method} batch {key}")
+    assert_close_field(B.get_body(sid)[0].position, b_alone.position, 1e-5, f"{method} body position")

@@ -50,7 +50,7 @@ def test_fixture_holds_the_four_committed_series():
     ref = np.load(GOLDEN)
     assert [str(c) for c in ref["columns"]] == ["t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y",
                                                 "p_sum_y", "KE", "nodes_inside"]
-    for run in ("box_drop_M0", "box_drop_M1", "rover_wheel_M0", "rover_wheel_M1"):
+    for run in [f"{s}_{m}" for s in ("box_drop", "rover_wheel") for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5")]:
         R = ref[run]
         assert R.shape == (720, 10)
         np.testing.assert_allclose(R[:, 0], (np.arange(720) + 1) / 240.0, atol=1e-6)
