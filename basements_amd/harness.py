@@ -86,6 +86,75 @@ def rover_wheel() -> GridOnlyScenario:
 
 SCENARIOS = {"box_drop": box_drop, "rover_wheel": rover_wheel}
 
+METRIC_COLUMNS = ("max_abs_total_p_y", "avg_abs_total_p_y", "max_abs_energy_drift", "step_time_avg_us",
+                  "max_penetration_m", "avg_penetration_m", "dwell_fraction", "ratio_p_y_vs_M0")
+PENETRATION_DWELL_REL = np.float32(0.30)  # kPenetrationDwellRelThreshold, metrics.h
+
+
+class RunMetrics:
+    """The harness's run summary (RunMetrics, metrics.h): peak and mean |total
+    p_y|, peak relative energy drift against the first step, peak and mean
+    penetration depth and the fraction of steps deeper than 0.3 of the smallest
+    half extent; FP32 accumulators as there."""
+
+    def __init__(self):
+        self.max_p = np.float32(0)
+        self.sum_p = np.float32(0)
+        self.max_drift = np.float32(0)
+        self.e0 = None
+        self.time_us = 0.0
+        self.n = 0
+        self.max_pen = np.float32(0)
+        self.sum_pen = np.float32(0)
+        self.dwell = 0
+
+    def observe(self, total_p_y, e_total, pen, min_half, step_us):
+        e = np.float32(e_total)
+        if self.e0 is None:
+            self.e0 = e
+        drift = (np.float32(abs(e - self.e0)) / np.float32(abs(self.e0)) if abs(self.e0) > np.float32(1e-9)
+                 else np.float32(abs(e - self.e0)))
+        ap = np.float32(abs(np.float32(total_p_y)))
+        self.max_p = max(self.max_p, ap)
+        self.sum_p = np.float32(self.sum_p + ap)
+        self.max_drift = max(self.max_drift, drift)
+        pen = np.float32(pen)
+        self.max_pen = max(self.max_pen, pen)
+        self.sum_pen = np.float32(self.sum_pen + pen)
+        if pen > np.float32(min_half) * PENETRATION_DWELL_REL:
+            self.dwell += 1
+        self.time_us += step_us
+        self.n += 1
+
+    def summary(self) -> dict:
+        n = max(self.n, 1)
+        return {"max_abs_total_p_y": float(self.max_p), "avg_abs_total_p_y": float(np.float32(self.sum_p / n)),
+                "max_abs_energy_drift": float(self.max_drift), "step_time_avg_us": self.time_us / n,
+                "max_penetration_m": float(self.max_pen), "avg_penetration_m": float(np.float32(self.sum_pen / n)),
+                "dwell_fraction": self.dwell / n, "ratio_p_y_vs_M0": 1.0}
+
+
+def _rotation(q) -> np.ndarray:
+    """Matrix3::from_quaternion (matrix3.h:432-448) in FP32, row-major."""
+    w, x, y, z = (np.float32(v) for v in q)
+    two, one = np.float32(2), np.float32(1)
+    return np.array([[one - two * (y * y + z * z), two * (x * y - w * z), two * (x * z + w * y)],
+                     [two * (x * y + w * z), one - two * (x * x + z * z), two * (y * z - w * x)],
+                     [two * (x * z - w * y), two * (y * z + w * x), one - two * (x * x + y * y)]], np.float32)
+
+
+def penetration(sc: "GridOnlyScenario", position, orientation, nodes_xyz, mass) -> float:
+    """measure_penetration (metrics.h): the deepest node (mass > 0) inside the
+    body's half-extent box, depth = the smallest distance to a face."""
+    if nodes_xyz.shape[0] == 0:
+        return 0.0
+    R = _rotation(orientation)
+    local = (nodes_xyz - np.asarray(position, np.float32)) @ R  # R_w2l (x - c)
+    he = np.asarray(sc.half_extents, np.float32)
+    d = he - np.abs(local)
+    ok = (d >= 0).all(axis=1) & (mass > 0)
+    return float(d[ok].min(axis=1).max()) if ok.any() else 0.0
+
 
 def _seed(sc: GridOnlyScenario):
     """seed_sand / seed_sand_for_wheel as a dense box at the handle's origin."""
@@ -124,8 +193,18 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
     dvy = float(np.float32(-sc.gravity_g) * np.float32(sc.dt))
     dt = float(np.float32(sc.dt))
     rows = {k: np.zeros(n, np.float64) for k in COLUMNS + ("p_grid_abs", "n_grid")}
+    metrics = RunMetrics()
+    g32 = np.float32(sc.gravity_g)
+    min_half = float(min(np.float32(h) for h in sc.half_extents))
+    ii, jj, kk = np.meshgrid(*(np.arange(o, o + d) for o, d in zip(sc.origin, sc.dims)), indexing="ij")
+    node_xyz = ((np.stack([ii, jj, kk], -1).astype(np.float32) + np.float32(0.5)) * np.float32(sc.dx))
+    # the harness measures penetration with its collider state, whose centre it
+    # updates after the integration but whose orientation it leaves at the
+    # step's start (bench_compare_all.cpp:126-134, 676-680)
+    q_step = tuple(sc.orientation)
     t0 = time.perf_counter()
     for i in range(n):
+        ts = time.perf_counter()
         S.couple_body(method, dt)
         S.grid_settle(dvy)
         if not log:
@@ -147,10 +226,31 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
         rows["p_sum_y"][i] = p_body_y + p_grid_y
         rows["KE"][i] = float(np.float32(0.5) * np.float32(m) * np.float32(np.dot(v, v)))
         rows["nodes_inside"][i] = inside
+        # RunMetrics inputs (bench_compare_all.cpp:137-156, 677-690)
+        vn = G["vel"][act].astype(np.float64)
+        e_grid = float(np.sum(0.5 * G["mass"][act] * np.einsum("ij,ij->i", vn, vn)))
+        e_total = float(np.float32(rows["KE"][i]) + np.float32(m) * g32 * np.float32(body.position[1])) + e_grid
+        pen = penetration(sc, body.position, q_step, node_xyz[act], G["mass"][act])
+        q_step = tuple(body.orientation)
+        metrics.observe(rows["p_sum_y"][i], e_total, pen, min_half, (time.perf_counter() - ts) * 1e6)
     S.synchronize()
     secs = time.perf_counter() - t0
     S.close()
+    if log:
+        rows["metrics"] = metrics.summary()
     return rows, secs
+
+
+def write_metrics_csv(path: str, method_name: str, summary: dict) -> None:
+    """The run summary line the harness writes beside the time series
+    (bench_compare_all.cpp:160-176)."""
+    s = summary
+    with open(path, "w") as f:
+        f.write(",".join(("method",) + METRIC_COLUMNS) + "\n")
+        f.write("%s,%.6e,%.6e,%.6e,%.3f,%.6e,%.6e,%.4f,%.6f\n" % (
+            method_name, s["max_abs_total_p_y"], s["avg_abs_total_p_y"], s["max_abs_energy_drift"],
+            s["step_time_avg_us"], s["max_penetration_m"], s["avg_penetration_m"], s["dwell_fraction"],
+            s["ratio_p_y_vs_M0"]))
 
 
 def write_timeseries_csv(path: str, rows: dict) -> None:

@@ -5,7 +5,9 @@ coupler loops (research/mpm_rigid_coupling/outputs/csv/{box_drop,rover_wheel}_
 {M0,M1,M4.5a,M4.5b,M4.5}.csv, written by FrameExporter::log_step, frame_exporter.h:44-57, from
 bench_compare_all.cpp:72-134 and :614-689).  This copies their numbers into
 one .npz (no pickles): key "<scenario>_<method>" -> (steps, 10) float64 in
-the CSV's column order.  tests/test_gpu_coupler_loops.py compares the device
+the CSV's column order, and "<scenario>_<method>_metrics" -> the 8 numbers of
+the run summary the harness wrote beside it (<run>_metrics.csv,
+bench_compare_all.cpp:160-176; its method name dropped).  tests/test_gpu_coupler_loops.py compares the device
 loops with them.  Runs only where the reference tree is present.
 
     python tests/golden/make_coupler_loops.py
@@ -21,6 +23,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_CSV = "/root/reference/research/mpm_rigid_coupling/outputs/csv"
 RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel") for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
 COLUMNS = ("t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y", "p_sum_y", "KE", "nodes_inside")
+METRICS = ("max_abs_total_p_y", "avg_abs_total_p_y", "max_abs_energy_drift", "step_time_avg_us",
+           "max_penetration_m", "avg_penetration_m", "dwell_fraction", "ratio_p_y_vs_M0")
 
 
 def main():
@@ -29,7 +33,11 @@ def main():
         with open(os.path.join(REF_CSV, run + ".csv")) as f:
             rows = list(csv.DictReader(f))
         out[run] = np.array([[float(r[c]) for c in COLUMNS] for r in rows], np.float64)
+        with open(os.path.join(REF_CSV, run + "_metrics.csv")) as f:
+            m = list(csv.DictReader(f))[0]
+        out[run + "_metrics"] = np.array([float(m[c]) for c in METRICS], np.float64)
     out["columns"] = np.array(COLUMNS)
+    out["metric_columns"] = np.array(METRICS)
     np.savez_compressed(os.path.join(HERE, "coupler_loops.npz"), **out)
     print({k: v.shape for k, v in out.items()})
 

@@ -3,7 +3,7 @@ series the reference committed (research/mpm_rigid_coupling/outputs/csv/
 {box_drop,rover_wheel}_{M0,M1,M4.5a,M4.5b,M4.5}.csv, copied into
 tests/golden/coupler_loops.npz by tests/golden/make_coupler_loops.py).
 
-Each loop runs 720 steps of: body gravity, M0 or M1 on the seeded sand grid,
+Each loop runs 720 steps of: body gravity, the coupler on the seeded sand grid,
 reaction into the body, TimeIntegrator::integrate, grid settling -- the
 couplers, the integrator and the settling on the GPU (basements_amd.harness).
 Bars: nodes_inside equal at every step; every logged float within the CSV's
@@ -59,6 +59,16 @@ def test_grid_only_loop_reproduces_the_committed_series(run):
         bar = (5e-6 * scale + fp32_sum) if grid else 1e-6 * scale
         k = int(np.argmax(d - bar))
         assert d[k] <= bar[k] if np.ndim(bar) else d[k] <= bar, (run, name, k, G[k, c[name]], R[k, c[name]], scale)
+    # the run summary (RunMetrics, metrics.h) against the committed <run>_metrics.csv: maxima are single
+    # logged values (print precision of %.6e plus the series bars above), the means and the energy drift
+    # FP32 accumulations over 720 steps (720 * 2^-24 ~ 4e-5 relative), the dwell fraction a count of
+    # steps (one step = 1/720) printed with %.4f; the wall-clock step time is not compared
+    got = rows["metrics"]
+    want = dict(zip([str(x) for x in ref["metric_columns"]], ref[run + "_metrics"]))
+    for name, rel in (("max_abs_total_p_y", 1e-5), ("max_penetration_m", 1e-5), ("avg_abs_total_p_y", 1e-4),
+                      ("avg_penetration_m", 1e-4), ("max_abs_energy_drift", 1e-4)):
+        assert abs(got[name] - want[name]) <= rel * abs(want[name]) + 1e-7, (run, name, got[name], want[name])
+    assert abs(got["dwell_fraction"] - want["dwell_fraction"]) <= 1.5 / 720 + 5e-5, (run, got, want)
 
 
 

@@ -60,6 +60,24 @@ def test_fixture_holds_the_four_committed_series():
         np.testing.assert_allclose(R[:n, 4], -9.81 * (np.arange(n) + 1) / 240.0, rtol=1e-5)
 
 
+def test_metrics_writer_matches_the_reference_format(tmp_path):
+    from basements_amd import harness
+
+    m = harness.RunMetrics()
+    m.observe(1.0, 2.0, 0.1, 0.2, 5.0)   # first step: energy reference
+    m.observe(-3.0, 2.2, 0.05, 0.2, 7.0)  # drift 0.1; 0.05 < 0.3 * 0.2: no dwell
+    s = m.summary()
+    assert s["max_abs_total_p_y"] == 3.0 and s["avg_abs_total_p_y"] == 2.0
+    assert abs(s["max_abs_energy_drift"] - 0.1) < 1e-6 and s["dwell_fraction"] == 0.5
+    p = tmp_path / "m.csv"
+    harness.write_metrics_csv(str(p), "M0", s)
+    lines = p.read_text().splitlines()
+    assert lines[0] == ("method,max_abs_total_p_y,avg_abs_total_p_y,max_abs_energy_drift,step_time_avg_us,"
+                        "max_penetration_m,avg_penetration_m,dwell_fraction,ratio_p_y_vs_M0")
+    assert lines[1].startswith("M0,3.000000e+00,2.000000e+00,1.000000e-01,6.000,1.000000e-01,") and \
+        lines[1].endswith(",0.5000,1.000000")
+
+
 def test_csv_writer_matches_the_reference_format(tmp_path):
     from basements_amd import harness
 
