@@ -46,6 +46,7 @@ class GridOnlyScenario:
     dt: float = 1.0 / 240.0
     total_time: float = 3.0
     gravity_g: float = 9.81
+    frame_every_n: int = 5  # a grid snapshot every n steps (scenario_box_drop.h:35)
     # the node box the handle holds: covers the patch, the body's reach and
     # block alignment with the reference's 4^3 blocks (origin a multiple of 4)
     origin: tuple = (-16, -16, -16)
@@ -169,13 +170,16 @@ def _seed(sc: GridOnlyScenario):
     return mass, vel, mass > 0
 
 
-def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool = True):
+def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool = True, frames: bool = False):
     """The grid-only coupled loop on a fresh handle.  Returns (columns,
     seconds): the time-series rows of FrameExporter::log_step (dict of
     arrays; also "p_grid_abs" = sum |v_y m| and "n_grid" = active nodes, the
     size of the grid momentum sum, for its rounding bound) and the wall time
     of the stepping alone when log is False (the per-step host reads of the
-    logging otherwise dominate)."""
+    logging otherwise dominate).  frames: also the snapshots of
+    FrameExporter::log_frame every frame_every_n steps, rows["frames"] = list
+    of (t, body position, body velocity, nodes) with nodes an (n, 7) array of
+    x, y, z (node centres), vx, vy, vz, mass of the nodes with mass > 0."""
     from .mpm import BODY_DYNAMIC, MODE_REFERENCE, Body, Solver
 
     n = sc.num_steps() if steps is None else int(steps)
@@ -202,6 +206,7 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
     # updates after the integration but whose orientation it leaves at the
     # step's start (bench_compare_all.cpp:126-134, 676-680)
     q_step = tuple(sc.orientation)
+    snaps = []
     t0 = time.perf_counter()
     for i in range(n):
         ts = time.perf_counter()
@@ -232,13 +237,30 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
         e_total = float(np.float32(rows["KE"][i]) + np.float32(m) * g32 * np.float32(body.position[1])) + e_grid
         pen = penetration(sc, body.position, q_step, node_xyz[act], G["mass"][act])
         q_step = tuple(body.orientation)
+        if frames and i % sc.frame_every_n == 0:  # bench_compare_all.cpp:132
+            nodes = np.concatenate([node_xyz[act], G["vel"][act], G["mass"][act][:, None]], 1).astype(np.float32)
+            snaps.append((rows["t"][i], tuple(body.position), tuple(body.linear_velocity), nodes))
         metrics.observe(rows["p_sum_y"][i], e_total, pen, min_half, (time.perf_counter() - ts) * 1e6)
     S.synchronize()
     secs = time.perf_counter() - t0
     S.close()
     if log:
         rows["metrics"] = metrics.summary()
+    if frames:
+        rows["frames"] = snaps
     return rows, secs
+
+
+def write_frames_csv(path: str, frames: list) -> None:
+    """FrameExporter::log_frame's format (frame_exporter.h:59-94): per frame a
+    body row and one row per node with mass > 0 (here in block-linear order;
+    the reference's follows its hash map)."""
+    with open(path, "w") as f:
+        f.write("t,kind,id,x,y,z,vx,vy,vz,mass\n")
+        for t, pos, vel, nodes in frames:
+            f.write("%.6f,B,0,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e\n" % ((t,) + tuple(pos) + tuple(vel) + (0.0,)))
+            for k, r in enumerate(nodes):
+                f.write("%.6f,N,%d,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e\n" % ((t, k) + tuple(float(v) for v in r)))
 
 
 def write_metrics_csv(path: str, method_name: str, summary: dict) -> None:

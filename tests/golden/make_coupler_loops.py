@@ -7,7 +7,10 @@ bench_compare_all.cpp:72-134 and :614-689).  This copies their numbers into
 one .npz (no pickles): key "<scenario>_<method>" -> (steps, 10) float64 in
 the CSV's column order, and "<scenario>_<method>_metrics" -> the 8 numbers of
 the run summary the harness wrote beside it (<run>_metrics.csv,
-bench_compare_all.cpp:160-176; its method name dropped).  tests/test_gpu_coupler_loops.py compares the device
+bench_compare_all.cpp:160-176; its method name dropped), and for the box
+drop "<run>_frames" -> float32 rows (frame, x, y, z, vx, vy, vz, mass) of the
+node rows of every FRAME_STRIDE-th grid snapshot of <run>_frames.csv
+(FrameExporter::log_frame, frame_exporter.h:59-94).  tests/test_gpu_coupler_loops.py compares the device
 loops with them.  Runs only where the reference tree is present.
 
     python tests/golden/make_coupler_loops.py
@@ -23,6 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_CSV = "/root/reference/research/mpm_rigid_coupling/outputs/csv"
 RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel") for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
 COLUMNS = ("t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y", "p_sum_y", "KE", "nodes_inside")
+FRAME_STRIDE = 8
 METRICS = ("max_abs_total_p_y", "avg_abs_total_p_y", "max_abs_energy_drift", "step_time_avg_us",
            "max_penetration_m", "avg_penetration_m", "dwell_fraction", "ratio_p_y_vs_M0")
 
@@ -36,6 +40,14 @@ def main():
         with open(os.path.join(REF_CSV, run + "_metrics.csv")) as f:
             m = list(csv.DictReader(f))[0]
         out[run + "_metrics"] = np.array([float(m[c]) for c in METRICS], np.float64)
+        fpath = os.path.join(REF_CSV, run + "_frames.csv")
+        if run.startswith("box_drop") and os.path.exists(fpath):
+            with open(fpath) as f:
+                fr = list(csv.reader(f))[1:]
+            times = sorted({float(r[0]) for r in fr})
+            keep = {t: k for k, t in enumerate(times) if k % FRAME_STRIDE == 0}
+            out[run + "_frames"] = np.array([[keep[float(r[0])]] + [float(v) for v in r[3:10]]
+                                             for r in fr if r[1] == "N" and float(r[0]) in keep], np.float32)
     out["columns"] = np.array(COLUMNS)
     out["metric_columns"] = np.array(METRICS)
     np.savez_compressed(os.path.join(HERE, "coupler_loops.npz"), **out)

@@ -42,7 +42,7 @@ def test_grid_only_loop_reproduces_the_committed_series(run):
     R = ref[run]
     scen, method = run.rsplit("_", 1)
     rows, _ = harness.run(harness.SCENARIOS[scen](), {"M0": M0, "M1": M1, "M4.5a": M45A, "M4.5b": M45B,
-                                                       "M4.5": M45}[method])
+                                                       "M4.5": M45}[method], frames=True)
     G = np.stack([rows[c] for c in cols], 1)
     assert G.shape == R.shape
     np.testing.assert_allclose(G[:, 0], R[:, 0], atol=1e-6)  # t, printed %.6f
@@ -69,6 +69,23 @@ def test_grid_only_loop_reproduces_the_committed_series(run):
                       ("avg_penetration_m", 1e-4), ("max_abs_energy_drift", 1e-4)):
         assert abs(got[name] - want[name]) <= rel * abs(want[name]) + 1e-7, (run, name, got[name], want[name])
     assert abs(got["dwell_fraction"] - want["dwell_fraction"]) <= 1.5 / 720 + 5e-5, (run, got, want)
+    # grid snapshots (FrameExporter::log_frame; box drop only, every 8th committed frame): the same nodes
+    # (mass > 0) at the same centres, velocities within the grid-momentum bar above (5e-6 of the run's
+    # velocity scale: nodes at rest carry ~1e-11 m/s of FP32 noise there), masses to print precision
+    if run + "_frames" not in ref.files:
+        return
+    F = ref[run + "_frames"].astype(np.float64)
+    vmax = max(np.abs(F[:, 4:7]).max(), np.abs(R[:, c["box_vy"]]).max())
+    for k in np.unique(F[:, 0]).astype(int):
+        want_n = F[F[:, 0] == k][:, 1:]
+        got_n = rows["frames"][k][3].astype(np.float64)
+        assert got_n.shape == want_n.shape, (run, k, got_n.shape, want_n.shape)
+        order = lambda a: np.lexsort((a[:, 2], a[:, 1], a[:, 0]))  # noqa: E731
+        g, w = got_n[order(got_n)], want_n[order(want_n)]
+        np.testing.assert_allclose(g[:, :3], w[:, :3], rtol=1e-6, atol=1e-7, err_msg=f"{run} frame {k} centres")
+        np.testing.assert_allclose(g[:, 3:6], w[:, 3:6], rtol=1e-6, atol=5e-6 * vmax,
+                                   err_msg=f"{run} frame {k} velocities")
+        np.testing.assert_allclose(g[:, 6], w[:, 6], rtol=1e-6, atol=1e-9, err_msg=f"{run} frame {k} masses")
 
 
 

@@ -78,6 +78,21 @@ def test_metrics_writer_matches_the_reference_format(tmp_path):
         lines[1].endswith(",0.5000,1.000000")
 
 
+def test_frames_writer_matches_the_reference_format(tmp_path):
+    from basements_amd import harness
+
+    nodes = np.array([[-0.25, 0.05, -0.25, 0.0, 0.0, 0.0, 0.01]], np.float32)
+    p = tmp_path / "fr.csv"
+    harness.write_frames_csv(str(p), [(1 / 240, (0.0, 0.5498297, 0.0), (0.0, -0.040875, 0.0), nodes)])
+    lines = p.read_text().splitlines()
+    assert lines[0] == "t,kind,id,x,y,z,vx,vy,vz,mass"
+    # the first rows of research/mpm_rigid_coupling/outputs/csv/box_drop_M1_frames.csv
+    assert lines[1] == "0.004167,B,0,0.000000e+00,5.498297e-01,0.000000e+00,0.000000e+00,-4.087500e-02," \
+                       "0.000000e+00,0.000000e+00"
+    assert lines[2] == "0.004167,N,0,-2.500000e-01,5.000000e-02,-2.500000e-01,0.000000e+00,0.000000e+00," \
+                       "0.000000e+00,1.000000e-02"
+
+
 def test_csv_writer_matches_the_reference_format(tmp_path):
     from basements_amd import harness
 
