@@ -85,7 +85,26 @@ def rover_wheel() -> GridOnlyScenario:
                             linear_velocity=(0.0, 0.0, 0.0), angular_velocity=(0.0, 0.0, -10.0))
 
 
-SCENARIOS = {"box_drop": box_drop, "rover_wheel": rover_wheel}
+def box_drop_dense() -> GridOnlyScenario:
+    """The box drop with a 9 kg box (bench_compare_all.cpp:505-509:
+    S.box_mass = 9, density-matched to the wheel)."""
+    sc = box_drop()
+    sc.name, sc.body_mass = "box_drop_dense", 9.0
+    return sc
+
+
+def foot_step() -> GridOnlyScenario:
+    """ScenarioFootStep (scenario_foot_step.h): a 1 kg thin box foot, half
+    extents (0.10, 0.02, 0.10) m, dropped from y = 0.55 m on the 7 x 3 x 7
+    patch, 2 s.  make_foot sets inertia_tensor directly: the inverse inertia
+    stays the identity."""
+    sc = box_drop()
+    sc.name, sc.half_extents, sc.total_time = "foot_step", (0.10, 0.02, 0.10), 2.0
+    return sc
+
+
+SCENARIOS = {"box_drop": box_drop, "rover_wheel": rover_wheel, "box_drop_dense": box_drop_dense,
+             "foot_step": foot_step}
 
 METRIC_COLUMNS = ("max_abs_total_p_y", "avg_abs_total_p_y", "max_abs_energy_drift", "step_time_avg_us",
                   "max_penetration_m", "avg_penetration_m", "dwell_fraction", "ratio_p_y_vs_M0")

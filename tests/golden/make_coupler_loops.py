@@ -1,9 +1,10 @@
 """Write tests/golden/coupler_loops.npz from the reference's committed outputs.
 
 The reference's research harness committed the time series of its grid-only
-coupler loops (research/mpm_rigid_coupling/outputs/csv/{box_drop,rover_wheel}_
-{M0,M1,M4.5a,M4.5b,M4.5}.csv, written by FrameExporter::log_step, frame_exporter.h:44-57, from
-bench_compare_all.cpp:72-134 and :614-689).  This copies their numbers into
+coupler loops (research/mpm_rigid_coupling/outputs/csv/{box_drop,rover_wheel,
+box_drop_dense,foot_step}_{M0,M1,M4.5a,M4.5b,M4.5}.csv, written by
+FrameExporter::log_step, frame_exporter.h:44-57, from bench_compare_all.cpp:72-134,
+:374-427, :505-560 and :614-689).  This copies their numbers into
 one .npz (no pickles): key "<scenario>_<method>" -> (steps, 10) float64 in
 the CSV's column order, and "<scenario>_<method>_metrics" -> the 8 numbers of
 the run summary the harness wrote beside it (<run>_metrics.csv,
@@ -24,7 +25,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_CSV = "/root/reference/research/mpm_rigid_coupling/outputs/csv"
-RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel") for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
+RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel", "box_drop_dense", "foot_step")
+             for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
 COLUMNS = ("t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y", "p_sum_y", "KE", "nodes_inside")
 FRAME_STRIDE = 8
 METRICS = ("max_abs_total_p_y", "avg_abs_total_p_y", "max_abs_energy_drift", "step_time_avg_us",
@@ -41,7 +43,7 @@ def main():
             m = list(csv.DictReader(f))[0]
         out[run + "_metrics"] = np.array([float(m[c]) for c in METRICS], np.float64)
         fpath = os.path.join(REF_CSV, run + "_frames.csv")
-        if run.startswith("box_drop") and os.path.exists(fpath):
+        if run.startswith("box_drop_M") and os.path.exists(fpath):
             with open(fpath) as f:
                 fr = list(csv.reader(f))[1:]
             times = sorted({float(r[0]) for r in fr})

@@ -1,15 +1,16 @@
 """The reference's grid-only coupler loops on the device against the time
 series the reference committed (research/mpm_rigid_coupling/outputs/csv/
-{box_drop,rover_wheel}_{M0,M1,M4.5a,M4.5b,M4.5}.csv, copied into
+{box_drop,rover_wheel,box_drop_dense,foot_step}_{M0,M1,M4.5a,M4.5b,M4.5}.csv, copied into
 tests/golden/coupler_loops.npz by tests/golden/make_coupler_loops.py).
 
-Each loop runs 720 steps of: body gravity, the coupler on the seeded sand grid,
+Each loop runs 720 steps (480 for the foot step) of: body gravity, the coupler on the seeded sand grid,
 reaction into the body, TimeIntegrator::integrate, grid settling -- the
 couplers, the integrator and the settling on the GPU (basements_amd.harness).
 Bars: nodes_inside equal at every step; every logged float within the CSV's
 own print precision (%.6e / %.8e: 7 significant digits), i.e. 1e-6 of the
 column family's scale (positions: the largest |coordinate| of the run,
-velocities and momenta: the largest |v_y|, energy: the largest KE).  The
+velocity: the largest |v_y|, momenta: the largest |p_y| logged, energy: the
+largest KE).  The
 reaction sums run in a different order than the reference's (a fixed-order
 FP64 tree against hash-map order in FP32), which stays below that for the
 body.  The logged grid momentum (p_grid_y, p_sum_y) is a sum over the ~360
@@ -30,7 +31,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "coupler_loops.npz")
-RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel") for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
+SCENES = ("box_drop", "rover_wheel", "box_drop_dense", "foot_step")
+RUNS = tuple(f"{s}_{m}" for s in SCENES for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
 
 
 @pytest.mark.parametrize("run", RUNS)
@@ -49,9 +51,10 @@ def test_grid_only_loop_reproduces_the_committed_series(run):
     assert np.array_equal(G[:, 9], R[:, 9]), np.nonzero(G[:, 9] != R[:, 9])[0][:10]
     c = {k: cols.index(k) for k in cols}
     pos = np.abs(R[:, [c["box_x"], c["box_y"], c["box_z"]]]).max()
-    vel = max(np.abs(R[:, c["box_vy"]]).max(), np.abs(R[:, c["p_grid_y"]]).max())
-    families = {"box_x": pos, "box_y": pos, "box_z": pos, "box_vy": vel, "p_body_y": vel, "p_grid_y": vel,
-                "p_sum_y": vel, "KE": np.abs(R[:, c["KE"]]).max()}
+    vel = np.abs(R[:, c["box_vy"]]).max()
+    mom = np.abs(R[:, [c["p_body_y"], c["p_grid_y"], c["p_sum_y"]]]).max()
+    families = {"box_x": pos, "box_y": pos, "box_z": pos, "box_vy": vel, "p_body_y": mom, "p_grid_y": mom,
+                "p_sum_y": mom, "KE": np.abs(R[:, c["KE"]]).max()}
     fp32_sum = np.maximum(rows["n_grid"] - 1, 0) * 2.0 ** -24 * rows["p_grid_abs"]
     for name, scale in families.items():
         d = np.abs(G[:, c[name]] - R[:, c[name]])

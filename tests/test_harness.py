@@ -50,10 +50,13 @@ def test_fixture_holds_the_four_committed_series():
     ref = np.load(GOLDEN)
     assert [str(c) for c in ref["columns"]] == ["t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y",
                                                 "p_sum_y", "KE", "nodes_inside"]
-    for run in [f"{s}_{m}" for s in ("box_drop", "rover_wheel") for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5")]:
+    runs = [(s, f"{s}_{m}") for s in ("box_drop", "rover_wheel", "box_drop_dense", "foot_step")
+            for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5")]
+    for s, run in runs:
         R = ref[run]
-        assert R.shape == (720, 10)
-        np.testing.assert_allclose(R[:, 0], (np.arange(720) + 1) / 240.0, atol=1e-6)
+        steps = 480 if s == "foot_step" else 720  # 2 s / 3 s at 1/240 s
+        assert R.shape == (steps, 10) and ref[run + "_metrics"].shape == (8,)
+        np.testing.assert_allclose(R[:, 0], (np.arange(steps) + 1) / 240.0, atol=1e-6)
         # free fall before the first contact: v_y = -g t, as the integrator gives
         k = int(np.argmax(R[:, 9] > 0))
         n = min(k, 5)  # the printed t is rounded to 1e-6: use (i + 1) dt itself
