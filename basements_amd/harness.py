@@ -282,6 +282,62 @@ def write_frames_csv(path: str, frames: list) -> None:
                 f.write("%.6f,N,%d,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e\n" % ((t, k) + tuple(float(v) for v in r)))
 
 
+TINE_COLUMNS = ("t", "tine_x", "reaction_fx", "reaction_fy", "reaction_fz")
+
+
+def run_tine_drag(method: int, steps: int | None = None):
+    """ScenarioTineDrag (scenario_tine_drag.h, run_tine_drag in
+    bench_compare_all.cpp:215-292): a 10 x 100 x 80 mm tine dragged at
+    0.2 m/s through a 13 x 4 x 7 node soil bed (dx 5 cm, 0.1875 kg per node),
+    6 cm deep, kinematically; per step the tine moves, the coupler acts and
+    the grid settles.  Returns (rows of every 4th step: t, tine_x and the
+    reaction, the summary: mean draft -F_x and vertical F_y after 0.5 s,
+    specific draught per metre of tine width, samples).  On the device the
+    tine is a kinematic body, which moves after its coupling: it starts one
+    step ahead, so it couples at the reference's positions, accumulated the
+    same way in FP32."""
+    from .mpm import BODY_KINEMATIC, MODE_REFERENCE, SHAPE_BOX, Body, Solver
+
+    f32 = np.float32
+    dx, dt, g, v = f32(0.05), f32(1.0) / f32(240.0), f32(9.81), f32(0.2)
+    he = (0.005, 0.05, 0.04)
+    n = int(f32(2.0) / dt + f32(0.5)) if steps is None else int(steps)
+    origin, dims = (-8, -8, -8), (32, 16, 16)
+    mass = np.zeros(dims, np.float32)
+    for i in range(-2, 11):
+        for j in range(4):
+            for k in range(-3, 4):
+                mass[i - origin[0], j - origin[1], k - origin[2]] = f32(0.1875)
+    S = Solver(float(dx), origin, dims, 1, MODE_REFERENCE)
+    S.upload_grid(origin, mass, np.zeros(dims + (3,), np.float32), mass > 0)
+    y0 = (f32(4 - 1) + f32(0.5)) * dx - f32(0.06) + f32(he[1])  # tine_center_y_init
+    x1 = f32(-0.05) + v * dt  # tine.position.x += v dt comes first in each reference step
+    S.set_body(Body(position=(float(x1), float(y0), 0.0), orientation=(1.0, 0.0, 0.0, 0.0),
+                    linear_velocity=(float(v), 0.0, 0.0), angular_velocity=(0.0, 0.0, 0.0), inv_mass=2.0,
+                    linear_damping=0.0, angular_damping=0.0, integrate=BODY_KINEMATIC, half_extents=he,
+                    friction=0.55, mass=0.5, shape=SHAPE_BOX))
+    dvy = float(-g * dt)
+    rows, fx_sum, fy_sum, samples = [], 0.0, 0.0, 0
+    for i in range(n):
+        t = float(f32(i + 1) * dt)
+        x_i = S.get_body()[0].position[0] if i % 4 == 0 else None  # where this step couples
+        S.couple_body(method, float(dt))
+        S.grid_settle(dvy)
+        force = S.get_body()[1][0]
+        if t > 0.5:  # measurement_start_time
+            fx_sum += -float(np.float32(force[0]))
+            fy_sum += float(np.float32(force[1]))
+            samples += 1
+        if i % 4 == 0:
+            rows.append((t, x_i, float(force[0]), float(force[1]), float(force[2])))
+    S.close()
+    fx = float(f32(fx_sum / samples)) if samples else 0.0
+    fy = float(f32(fy_sum / samples)) if samples else 0.0
+    summary = {"avg_draft_fx_n": fx, "avg_vertical_fy_n": fy,
+               "specific_draught_n_per_m": float(f32(fx) / (f32(2.0) * f32(he[2]))), "samples": samples}
+    return np.array(rows, np.float64), summary
+
+
 def write_metrics_csv(path: str, method_name: str, summary: dict) -> None:
     """The run summary line the harness writes beside the time series
     (bench_compare_all.cpp:160-176)."""

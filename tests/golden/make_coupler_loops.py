@@ -29,6 +29,7 @@ RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel", "box_drop_dense", "
              for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
 COLUMNS = ("t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y", "p_sum_y", "KE", "nodes_inside")
 FRAME_STRIDE = 8
+TINE_SUMMARY = ("avg_draft_fx_n", "avg_vertical_fy_n", "specific_draught_n_per_m", "samples")
 METRICS = ("max_abs_total_p_y", "avg_abs_total_p_y", "max_abs_energy_drift", "step_time_avg_us",
            "max_penetration_m", "avg_penetration_m", "dwell_fraction", "ratio_p_y_vs_M0")
 
@@ -50,6 +51,13 @@ def main():
             keep = {t: k for k, t in enumerate(times) if k % FRAME_STRIDE == 0}
             out[run + "_frames"] = np.array([[keep[float(r[0])]] + [float(v) for v in r[3:10]]
                                              for r in fr if r[1] == "N" and float(r[0]) in keep], np.float32)
+    for m in ("M0", "M1", "M4.5a", "M4.5b"):  # run_tine_drag, bench_compare_all.cpp:215-292
+        with open(os.path.join(REF_CSV, f"tine_drag_{m}.csv")) as f:
+            rows = list(csv.reader(f))[1:]
+        out[f"tine_drag_{m}"] = np.array([[float(v) for v in r] for r in rows], np.float64)
+        with open(os.path.join(REF_CSV, f"tine_drag_{m}_summary.csv")) as f:
+            s = list(csv.DictReader(f))[0]
+        out[f"tine_drag_{m}_summary"] = np.array([float(s[c]) for c in TINE_SUMMARY], np.float64)
     out["columns"] = np.array(COLUMNS)
     out["metric_columns"] = np.array(METRICS)
     np.savez_compressed(os.path.join(HERE, "coupler_loops.npz"), **out)

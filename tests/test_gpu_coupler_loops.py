@@ -129,3 +129,29 @@ def test_m45_couplers_in_the_coupled_substep_batch_equals_alone(method):
     for key in ("x", "v", "F", "C"):
         assert_close_field(got[key], alone[key], 1e-4, f"{method} batch {key}")
     assert_close_field(B.get_body(sid)[0].position, b_alone.position, 1e-5, f"{method} body position")
+
+
+@pytest.mark.parametrize("method", ["M0", "M1", "M4.5a", "M4.5b"])
+def test_tine_drag_reproduces_the_committed_series(method):
+    """run_tine_drag (bench_compare_all.cpp:215-292): a kinematic tine dragged
+    through the soil bed, the reaction logged every 4th step and averaged
+    after 0.5 s, against tine_drag_<method>.csv and its summary.  The tine's
+    path is prescribed, so its position matches to print precision; the
+    reaction comes from the persistent grid, which carries the reaction-order
+    perturbation from step to step: the same 5e-6-of-scale bar as the grid
+    momentum of the other loops."""
+    from basements_amd import M0, M1, M45A, M45B, harness
+
+    ref = np.load(GOLDEN)
+    R = ref[f"tine_drag_{method}"]
+    rows, summary = harness.run_tine_drag({"M0": M0, "M1": M1, "M4.5a": M45A, "M4.5b": M45B}[method])
+    assert rows.shape == R.shape
+    np.testing.assert_allclose(rows[:, 0], R[:, 0], atol=5e-5)  # t, printed %.4f
+    np.testing.assert_allclose(rows[:, 1], R[:, 1], rtol=1e-6, atol=1e-7)
+    fscale = max(np.abs(R[:, 2:5]).max(), 1e-30)
+    np.testing.assert_allclose(rows[:, 2:5], R[:, 2:5], rtol=1e-6, atol=5e-6 * fscale, err_msg=method)
+    want = dict(zip(("avg_draft_fx_n", "avg_vertical_fy_n", "specific_draught_n_per_m", "samples"),
+                    ref[f"tine_drag_{method}_summary"]))
+    assert summary["samples"] == int(want["samples"])
+    for k in ("avg_draft_fx_n", "avg_vertical_fy_n", "specific_draught_n_per_m"):
+        assert abs(summary[k] - want[k]) <= 1e-5 * abs(want[k]) + 5e-6 * fscale, (method, k, summary[k], want[k])

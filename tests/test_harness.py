@@ -63,6 +63,18 @@ def test_fixture_holds_the_four_committed_series():
         np.testing.assert_allclose(R[:n, 4], -9.81 * (np.arange(n) + 1) / 240.0, rtol=1e-5)
 
 
+def test_fixture_holds_the_tine_drag_series():
+    ref = np.load(GOLDEN)
+    for m in ("M0", "M1", "M4.5a", "M4.5b"):
+        R = ref[f"tine_drag_{m}"]
+        # run_tine_drag logs every 4th of 480 steps: t = (4k + 1) / 240 s, the tine at -0.05 + 0.2 t
+        assert R.shape == (120, 5)
+        t = (4 * np.arange(120) + 1) / 240.0
+        np.testing.assert_allclose(R[:, 0], t, atol=5e-5)
+        np.testing.assert_allclose(R[:, 1], -0.05 + 0.2 * t, atol=1e-6)
+        assert ref[f"tine_drag_{m}_summary"][3] == 360  # samples after 0.5 s
+
+
 def test_metrics_writer_matches_the_reference_format(tmp_path):
     from basements_amd import harness
 
