@@ -285,6 +285,16 @@ def write_frames_csv(path: str, frames: list) -> None:
 TINE_COLUMNS = ("t", "tine_x", "reaction_fx", "reaction_fy", "reaction_fz")
 
 
+def method_id(name: str) -> int:
+    """The coupler of a harness method name.  The harness's M2 (ASFLIP), M3
+    (MLS) and M4 (Lagrange) are skeletons that delegate to M1
+    (couplers/m2_asflip.h, m3_mls_mpm.h, m4_lagrange.h: apply returns
+    MPMRigidCouplerPIC::apply), so their committed runs are M1's."""
+    from .mpm import M0, M1, M45, M45A, M45B
+
+    return {"M0": M0, "M1": M1, "M2": M1, "M3": M1, "M4": M1, "M4.5a": M45A, "M4.5b": M45B, "M4.5": M45}[name]
+
+
 def run_tine_drag(method: int, steps: int | None = None):
     """ScenarioTineDrag (scenario_tine_drag.h, run_tine_drag in
     bench_compare_all.cpp:215-292): a 10 x 100 x 80 mm tine dragged at

@@ -58,6 +58,14 @@ def main():
         with open(os.path.join(REF_CSV, f"tine_drag_{m}_summary.csv")) as f:
             s = list(csv.DictReader(f))[0]
         out[f"tine_drag_{m}_summary"] = np.array([float(s[c]) for c in TINE_SUMMARY], np.float64)
+    # the harness's M2 / M3 / M4 are skeletons delegating to M1 (couplers/m2_asflip.h, m3_mls_mpm.h,
+    # m4_lagrange.h): their committed series equal M1's, which is what basements_amd.harness.method_id maps
+    for s in ("box_drop", "rover_wheel", "box_drop_dense", "foot_step"):
+        for m in ("M2", "M3", "M4"):
+            with open(os.path.join(REF_CSV, f"{s}_{m}.csv")) as f:
+                rows = list(csv.DictReader(f))
+            alias = np.array([[float(r[c]) for c in COLUMNS] for r in rows], np.float64)
+            assert np.array_equal(alias, out[f"{s}_M1"]), (s, m)
     out["columns"] = np.array(COLUMNS)
     out["metric_columns"] = np.array(METRICS)
     np.savez_compressed(os.path.join(HERE, "coupler_loops.npz"), **out)

@@ -37,14 +37,13 @@ RUNS = tuple(f"{s}_{m}" for s in SCENES for m in ("M0", "M1", "M4.5a", "M4.5b", 
 
 @pytest.mark.parametrize("run", RUNS)
 def test_grid_only_loop_reproduces_the_committed_series(run):
-    from basements_amd import M0, M1, M45, M45A, M45B, harness
+    from basements_amd import harness
 
     ref = np.load(GOLDEN)
     cols = [str(c) for c in ref["columns"]]
     R = ref[run]
     scen, method = run.rsplit("_", 1)
-    rows, _ = harness.run(harness.SCENARIOS[scen](), {"M0": M0, "M1": M1, "M4.5a": M45A, "M4.5b": M45B,
-                                                       "M4.5": M45}[method], frames=True)
+    rows, _ = harness.run(harness.SCENARIOS[scen](), harness.method_id(method), frames=True)
     G = np.stack([rows[c] for c in cols], 1)
     assert G.shape == R.shape
     np.testing.assert_allclose(G[:, 0], R[:, 0], atol=1e-6)  # t, printed %.6f
@@ -140,11 +139,11 @@ def test_tine_drag_reproduces_the_committed_series(method):
     reaction comes from the persistent grid, which carries the reaction-order
     perturbation from step to step: the same 5e-6-of-scale bar as the grid
     momentum of the other loops."""
-    from basements_amd import M0, M1, M45A, M45B, harness
+    from basements_amd import harness
 
     ref = np.load(GOLDEN)
     R = ref[f"tine_drag_{method}"]
-    rows, summary = harness.run_tine_drag({"M0": M0, "M1": M1, "M4.5a": M45A, "M4.5b": M45B}[method])
+    rows, summary = harness.run_tine_drag(harness.method_id(method))
     assert rows.shape == R.shape
     np.testing.assert_allclose(rows[:, 0], R[:, 0], atol=5e-5)  # t, printed %.4f
     np.testing.assert_allclose(rows[:, 1], R[:, 1], rtol=1e-6, atol=1e-7)

@@ -33,6 +33,15 @@ def test_scenarios_follow_the_reference_headers():
         assert all(o % 4 == 0 for o in sc.origin) and all(d % 4 == 0 for d in sc.dims)
 
 
+def test_method_names_map_to_couplers():
+    from basements_amd import M0, M1, M45, M45A, M45B, harness
+
+    assert [harness.method_id(m) for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5")] == [M0, M1, M45A, M45B, M45]
+    # the harness's M2 / M3 / M4 delegate to M1 (their committed series equal M1's,
+    # checked by tests/golden/make_coupler_loops.py where the reference is present)
+    assert {harness.method_id(m) for m in ("M2", "M3", "M4")} == {M1}
+
+
 def test_seeded_patch_matches_seed_sand():
     from basements_amd import harness
 
