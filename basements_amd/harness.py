@@ -47,6 +47,7 @@ class GridOnlyScenario:
     total_time: float = 3.0
     gravity_g: float = 9.81
     frame_every_n: int = 5  # a grid snapshot every n steps (scenario_box_drop.h:35)
+    settle_time: float | None = None  # foot step: rebound samples start here (scenario_foot_step.h:49)
     # the node box the handle holds: covers the patch, the body's reach and
     # block alignment with the reference's 4^3 blocks (origin a multiple of 4)
     origin: tuple = (-16, -16, -16)
@@ -100,11 +101,53 @@ def foot_step() -> GridOnlyScenario:
     stays the identity."""
     sc = box_drop()
     sc.name, sc.half_extents, sc.total_time = "foot_step", (0.10, 0.02, 0.10), 2.0
+    sc.settle_time = 0.05
     return sc
 
 
 SCENARIOS = {"box_drop": box_drop, "rover_wheel": rover_wheel, "box_drop_dense": box_drop_dense,
              "foot_step": foot_step}
+
+class ReboundTracker:
+    """The foot step's rebound coefficient (ReboundTracker,
+    scenario_foot_step.h:91-137), observed on the body's v_y after each step
+    from settle_time on: the most negative v_y while descending is
+    v_down_peak; once v_y turns non-negative after it, the largest v_y until
+    v_y is non-positive again is v_up_peak; e = v_up_peak / |v_down_peak|,
+    0 when the body never descended or never rose, capped at 2.  All in FP32
+    as the harness keeps it."""
+
+    DESCENT, REBOUND, DONE = 0, 1, 2
+    COLUMNS = ("v_down_peak", "v_up_peak", "rebound_coefficient_e")
+
+    def __init__(self):
+        self.state = self.DESCENT
+        self.v_down = np.float32(0.0)
+        self.v_up = np.float32(0.0)
+
+    def observe(self, t: float, v_y: float, settle_time: float) -> None:
+        v = np.float32(v_y)
+        if np.float32(t) < np.float32(settle_time):
+            return
+        if self.state == self.DESCENT:
+            self.v_down = min(self.v_down, v)
+            if v >= 0 and self.v_down < 0:
+                self.state = self.REBOUND
+        if self.state == self.REBOUND:
+            self.v_up = max(self.v_up, v)
+            if v <= 0 and self.v_up > 0:
+                self.state = self.DONE
+
+    def coefficient(self) -> float:
+        if self.v_down >= 0:
+            return 0.0
+        e = self.v_up / -self.v_down
+        return float(min(max(e, np.float32(0.0)), np.float32(2.0)))
+
+    def summary(self) -> dict:
+        return {"v_down_peak": float(self.v_down), "v_up_peak": float(self.v_up),
+                "rebound_coefficient_e": self.coefficient()}
+
 
 METRIC_COLUMNS = ("max_abs_total_p_y", "avg_abs_total_p_y", "max_abs_energy_drift", "step_time_avg_us",
                   "max_penetration_m", "avg_penetration_m", "dwell_fraction", "ratio_p_y_vs_M0")
@@ -198,7 +241,9 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
     logging otherwise dominate).  frames: also the snapshots of
     FrameExporter::log_frame every frame_every_n steps, rows["frames"] = list
     of (t, body position, body velocity, nodes) with nodes an (n, 7) array of
-    x, y, z (node centres), vx, vy, vz, mass of the nodes with mass > 0."""
+    x, y, z (node centres), vx, vy, vz, mass of the nodes with mass > 0.
+    A scenario with a settle_time (the foot step) also returns
+    rows["rebound"], the ReboundTracker summary of the logged run."""
     from .mpm import BODY_DYNAMIC, MODE_REFERENCE, Body, Solver
 
     n = sc.num_steps() if steps is None else int(steps)
@@ -217,6 +262,7 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
     dt = float(np.float32(sc.dt))
     rows = {k: np.zeros(n, np.float64) for k in COLUMNS + ("p_grid_abs", "n_grid")}
     metrics = RunMetrics()
+    rebound = ReboundTracker() if sc.settle_time is not None else None
     g32 = np.float32(sc.gravity_g)
     min_half = float(min(np.float32(h) for h in sc.half_extents))
     ii, jj, kk = np.meshgrid(*(np.arange(o, o + d) for o, d in zip(sc.origin, sc.dims)), indexing="ij")
@@ -250,6 +296,8 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
         rows["p_sum_y"][i] = p_body_y + p_grid_y
         rows["KE"][i] = float(np.float32(0.5) * np.float32(m) * np.float32(np.dot(v, v)))
         rows["nodes_inside"][i] = inside
+        if rebound is not None:  # bench_compare_all.cpp:431, on the body's v_y after the step
+            rebound.observe(rows["t"][i], v[1], sc.settle_time)
         # RunMetrics inputs (bench_compare_all.cpp:137-156, 677-690)
         vn = G["vel"][act].astype(np.float64)
         e_grid = float(np.sum(0.5 * G["mass"][act] * np.einsum("ij,ij->i", vn, vn)))
@@ -265,6 +313,8 @@ def run(sc: GridOnlyScenario, method: int, steps: int | None = None, log: bool =
     S.close()
     if log:
         rows["metrics"] = metrics.summary()
+        if rebound is not None:
+            rows["rebound"] = rebound.summary()
     if frames:
         rows["frames"] = snaps
     return rows, secs
@@ -358,6 +408,14 @@ def write_metrics_csv(path: str, method_name: str, summary: dict) -> None:
             method_name, s["max_abs_total_p_y"], s["avg_abs_total_p_y"], s["max_abs_energy_drift"],
             s["step_time_avg_us"], s["max_penetration_m"], s["avg_penetration_m"], s["dwell_fraction"],
             s["ratio_p_y_vs_M0"]))
+
+
+def write_rebound_csv(path: str, method_name: str, summary: dict) -> None:
+    """The foot step's rebound summary (bench_compare_all.cpp:481-491)."""
+    s = summary
+    with open(path, "w") as f:
+        f.write(",".join(("method",) + ReboundTracker.COLUMNS) + "\n")
+        f.write("%s,%.6e,%.6e,%.6f\n" % (method_name, s["v_down_peak"], s["v_up_peak"], s["rebound_coefficient_e"]))
 
 
 def write_timeseries_csv(path: str, rows: dict) -> None:

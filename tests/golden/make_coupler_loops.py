@@ -11,7 +11,9 @@ the run summary the harness wrote beside it (<run>_metrics.csv,
 bench_compare_all.cpp:160-176; its method name dropped), and for the box
 drop "<run>_frames" -> float32 rows (frame, x, y, z, vx, vy, vz, mass) of the
 node rows of every FRAME_STRIDE-th grid snapshot of <run>_frames.csv
-(FrameExporter::log_frame, frame_exporter.h:59-94).  tests/test_gpu_coupler_loops.py compares the device
+(FrameExporter::log_frame, frame_exporter.h:59-94), and for the foot step
+"<run>_rebound" -> v_down_peak, v_up_peak, e of <run>_rebound.csv
+(bench_compare_all.cpp:481-491).  tests/test_gpu_coupler_loops.py compares the device
 loops with them.  Runs only where the reference tree is present.
 
     python tests/golden/make_coupler_loops.py
@@ -29,6 +31,7 @@ RUNS = tuple(f"{s}_{m}" for s in ("box_drop", "rover_wheel", "box_drop_dense", "
              for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"))
 COLUMNS = ("t", "box_x", "box_y", "box_z", "box_vy", "p_body_y", "p_grid_y", "p_sum_y", "KE", "nodes_inside")
 FRAME_STRIDE = 8
+REBOUND = ("v_down_peak", "v_up_peak", "rebound_coefficient_e")
 TINE_SUMMARY = ("avg_draft_fx_n", "avg_vertical_fy_n", "specific_draught_n_per_m", "samples")
 METRICS = ("max_abs_total_p_y", "avg_abs_total_p_y", "max_abs_energy_drift", "step_time_avg_us",
            "max_penetration_m", "avg_penetration_m", "dwell_fraction", "ratio_p_y_vs_M0")
@@ -43,6 +46,10 @@ def main():
         with open(os.path.join(REF_CSV, run + "_metrics.csv")) as f:
             m = list(csv.DictReader(f))[0]
         out[run + "_metrics"] = np.array([float(m[c]) for c in METRICS], np.float64)
+        if run.startswith("foot_step_"):
+            with open(os.path.join(REF_CSV, run + "_rebound.csv")) as f:
+                r = list(csv.DictReader(f))[0]
+            out[run + "_rebound"] = np.array([float(r[c]) for c in REBOUND], np.float64)
         fpath = os.path.join(REF_CSV, run + "_frames.csv")
         if run.startswith("box_drop_M") and os.path.exists(fpath):
             with open(fpath) as f:

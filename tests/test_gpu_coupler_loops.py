@@ -71,6 +71,13 @@ def test_grid_only_loop_reproduces_the_committed_series(run):
                       ("avg_penetration_m", 1e-4), ("max_abs_energy_drift", 1e-4)):
         assert abs(got[name] - want[name]) <= rel * abs(want[name]) + 1e-7, (run, name, got[name], want[name])
     assert abs(got["dwell_fraction"] - want["dwell_fraction"]) <= 1.5 / 720 + 5e-5, (run, got, want)
+    # the foot step's rebound summary (ReboundTracker, <run>_rebound.csv): its peaks are logged v_y values
+    # (the velocity bar, plus %.6e print precision), e printed with %.6f
+    if run + "_rebound" in ref.files:
+        rb, want_rb = rows["rebound"], ref[run + "_rebound"]
+        for j, name in enumerate(("v_down_peak", "v_up_peak")):
+            assert abs(rb[name] - want_rb[j]) <= 1e-6 * vel + 5e-7 * abs(want_rb[j]), (run, name, rb, want_rb)
+        assert abs(rb["rebound_coefficient_e"] - want_rb[2]) <= 1e-6, (run, rb, want_rb)
     # grid snapshots (FrameExporter::log_frame; box drop only, every 8th committed frame): the same nodes
     # (mass > 0) at the same centres, velocities within the grid-momentum bar above (5e-6 of the run's
     # velocity scale: nodes at rest carry ~1e-11 m/s of FP32 noise there), masses to print precision

@@ -130,3 +130,52 @@ def test_csv_writer_matches_the_reference_format(tmp_path):
     assert lines[1] == ("0.004167,0.000000e+00,0.000000e+00,0.000000e+00,0.000000e+00,0.00000000e+00,"
                         "0.00000000e+00,0.00000000e+00,0.000000e+00,0")
     assert lines[2].endswith(",3")
+
+
+def test_rebound_tracker_replays_the_committed_foot_step_summaries():
+    """ReboundTracker fed the committed foot-step series' body v_y (%.6e, the
+    same FP32 values the reference observed, printed) gives the committed
+    <run>_rebound.csv: v_down_peak the most negative v_y from 0.05 s on."""
+    from basements_amd import harness
+
+    ref = np.load(GOLDEN)
+    dt = np.float32(1.0 / 240.0)
+    for m in ("M0", "M1", "M4.5a", "M4.5b", "M4.5"):
+        R = ref[f"foot_step_{m}"]
+        tr = harness.ReboundTracker()
+        for i in range(R.shape[0]):
+            tr.observe(float(np.float32(i + 1) * dt), R[i, 4], 0.05)
+        want = ref[f"foot_step_{m}_rebound"]
+        got = tr.summary()
+        assert np.float32(got["v_down_peak"]) == np.float32(want[0]), (m, got, want)
+        assert got["v_up_peak"] == want[1] and abs(got["rebound_coefficient_e"] - want[2]) <= 5e-7, (m, got, want)
+
+
+def test_rebound_tracker_phases():
+    """The tracker's three phases on hand-made series: samples before the
+    settle time are ignored, the up peak ends at the first non-positive v_y
+    after it, e is v_up / |v_down| capped at 2, and 0 without a descent."""
+    from basements_amd import harness
+
+    def track(vs, settle=0.0):
+        tr = harness.ReboundTracker()
+        for k, v in enumerate(vs):
+            tr.observe(0.01 * (k + 1), v, settle)
+        return tr.summary()
+
+    s = track([-1.0, -4.0, -2.0, 0.5, 1.0, 0.2, -0.1, 3.0])  # the late 3.0 is after the rebound ended
+    assert (s["v_down_peak"], s["v_up_peak"], s["rebound_coefficient_e"]) == (-4.0, 1.0, 0.25)
+    assert track([-9.0, -1.0, -2.0], settle=0.015)["v_down_peak"] == -2.0  # -9 falls before the settle time
+    assert track([-1.0, 0.0, 5.0, 1.0])["rebound_coefficient_e"] == 2.0
+    assert track([0.5, 1.0])["rebound_coefficient_e"] == 0.0
+    assert track([-3.0, -1.0])["v_up_peak"] == 0.0
+
+
+def test_rebound_writer_matches_the_reference_format(tmp_path):
+    from basements_amd import harness
+
+    p = tmp_path / "reb.csv"
+    harness.write_rebound_csv(str(p), "M1", {"v_down_peak": -1.921124, "v_up_peak": 0.0, "rebound_coefficient_e": 0.0})
+    # research/mpm_rigid_coupling/outputs/csv/foot_step_M1_rebound.csv
+    assert p.read_text().splitlines() == ["method,v_down_peak,v_up_peak,rebound_coefficient_e",
+                                          "M1,-1.921124e+00,0.000000e+00,0.000000"]
