@@ -468,12 +468,20 @@ __device__ __forceinline__ NodeFwd<R> node_forward(R m, v3<R> q, int j_node, v3<
     return o;
 }
 
+// Each thread owns one node and reads only that node's (m, q), so it may
+// clear them for the next P2G once read (clear_mq: the grid's last reader
+// this substep).
 template <class R>
-__global__ void __launch_bounds__(256) k_update(Box b, Par<R> P, R dt, const R* gm, const R* gq, R* gu) {
+__global__ void __launch_bounds__(256) k_update(Box b, Par<R> P, R dt, R* gm, R* gq, R* gu, int clear_mq) {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nn = (int64_t)b.nx * b.ny * b.nz;
     if (id >= nn) return;
     const R m = gm[id];
+    const v3<R> q = mk(gq[3 * id], gq[3 * id + 1], gq[3 * id + 2]);
+    if (clear_mq) {
+        gm[id] = R(0);
+        gq[3 * id] = gq[3 * id + 1] = gq[3 * id + 2] = R(0);
+    }
     if (m == R(0)) {
         gu[3 * id] = gu[3 * id + 1] = gu[3 * id + 2] = R(0);
         return;
@@ -481,7 +489,7 @@ __global__ void __launch_bounds__(256) k_update(Box b, Par<R> P, R dt, const R* 
     const int k = (int)(id % b.nz), j = (int)((id / b.nz) % b.ny), i = (int)(id / ((int64_t)b.nz * b.ny));
     const int gi = b.ox + i, gj = b.oy + j, gk = b.oz + k;
     const v3<R> xc = mk(((R)gi + R(0.5)) * P.dx, ((R)gj + R(0.5)) * P.dx, ((R)gk + R(0.5)) * P.dx);
-    const NodeFwd<R> f = node_forward(m, mk(gq[3 * id], gq[3 * id + 1], gq[3 * id + 2]), gj, xc, P, dt);
+    const NodeFwd<R> f = node_forward(m, q, gj, xc, P, dt);
     gu[3 * id] = f.u2.x;
     gu[3 * id + 1] = f.u2.y;
     gu[3 * id + 2] = f.u2.z;
@@ -692,13 +700,21 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
 // Grid: from the node-velocity adjoint to the adjoint of (m, q); the tool
 // velocity's adjoint from the M0 nodes.
 template <class R>
-__global__ void __launch_bounds__(256) k_update_adj(Box b, Par<R> P, R dt, const R* gm, const R* gq, const R* gub,
+__global__ void __launch_bounds__(256) k_update_adj(Box b, Par<R> P, R dt, R* gm, R* gq, R* gub,
                                                     R* gqb, R* gmb, double* grad) {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nn = (int64_t)b.nx * b.ny * b.nz;
     v3<R> vtb = mk(R(0), R(0), R(0));
     if (id < nn) {
+        // the substep's last reader of (m, q) and of the velocity adjoint:
+        // each thread reads its own node and leaves it cleared for the next
+        // P2G / G2P adjoint that accumulates into it
         const R m = gm[id];
+        const v3<R> q = mk(gq[3 * id], gq[3 * id + 1], gq[3 * id + 2]);
+        const v3<R> u2b_in = mk(gub[3 * id], gub[3 * id + 1], gub[3 * id + 2]);
+        gm[id] = R(0);
+        gq[3 * id] = gq[3 * id + 1] = gq[3 * id + 2] = R(0);
+        gub[3 * id] = gub[3 * id + 1] = gub[3 * id + 2] = R(0);
         gqb[3 * id] = gqb[3 * id + 1] = gqb[3 * id + 2] = R(0);
         gmb[id] = R(0);
         if (m != R(0)) {
@@ -706,9 +722,8 @@ __global__ void __launch_bounds__(256) k_update_adj(Box b, Par<R> P, R dt, const
             const int gj = b.oy + j;
             const v3<R> xc = mk(((R)(b.ox + i) + R(0.5)) * P.dx, ((R)gj + R(0.5)) * P.dx,
                                 ((R)(b.oz + k) + R(0.5)) * P.dx);
-            const v3<R> q = mk(gq[3 * id], gq[3 * id + 1], gq[3 * id + 2]);
             const NodeFwd<R> f = node_forward(m, q, gj, xc, P, dt);
-            v3<R> u2b = mk(gub[3 * id], gub[3 * id + 1], gub[3 * id + 2]);
+            v3<R> u2b = u2b_in;
             v3<R> u1b = u2b;
             if (f.m0) {
                 const R nb = dot(u2b, f.n);
@@ -875,7 +890,9 @@ struct bm_diff {
     void *m = nullptr, *V0 = nullptr, *x0 = nullptr, *tgt = nullptr, *init = nullptr;
     void *gm = nullptr, *gq = nullptr, *gu = nullptr, *gub = nullptr, *gqb = nullptr, *gmb = nullptr;
     void *ck = nullptr, *seg = nullptr, *abar = nullptr, *abar2 = nullptr;
-    int64_t ck_slots = 0, seg_slots = 0;
+    // the backward's grids of one segment: (m, q) and u of each of its substeps
+    void *sgmq = nullptr, *sgu = nullptr;
+    int64_t ck_slots = 0, seg_slots = 0, grid_slots = 0;
     double* grad = nullptr;
     int* err = nullptr;
     // last forward
@@ -937,16 +954,23 @@ template <class R> static PS<R> slot(void* base, int64_t cap, int64_t k) {
     return PS<R>{static_cast<R*>(base) + k * QCOMP * cap, cap};
 }
 
-// One forward substep from state `in` to state `out` (grid left in gm/gq/gu).
+// Zero (m, q): once per forward or backward call.  Every substep then leaves
+// them zero for the next one (k_update or k_update_adj clears each node after
+// its last read).
+template <class R> static void clear_mass_momentum(bm_diff_handle h) {
+    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
+    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
+}
+
+// One forward substep from state `in` to state `out` (grid velocity left in
+// gu; m and q zero on entry and on return).
 template <class R> static void substep(bm_diff_handle h, PS<R> in, PS<R> out, int step) {
     const Par<R> P = params_at<R>(h, step);
     const R dt = (R)h->dt;
-    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
-    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
     hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, (const R*)h->m,
                        (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err);
-    hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
-                       (const R*)h->gm, (const R*)h->gq, (R*)h->gu);
+    hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt, (R*)h->gm,
+                       (R*)h->gq, (R*)h->gu, 1);
     hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, out, h->n, h->box, P, dt,
                        (const R*)h->gu, h->err);
 }
@@ -983,6 +1007,7 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
     }
     HIP_TRY(hipMemcpyAsync(h->ck, h->init, h->rs * QCOMP * h->cap, hipMemcpyDeviceToDevice, h->stream));
     PS<R> cur = slot<R>(h->ck, h->cap, 0);
+    clear_mass_momentum<R>(h);
     int pp = 0;  // states between checkpoints alternate between two scratch slots
     for (int s = 0; s < substeps; ++s) {
         const int ns = s + 1;
@@ -1013,32 +1038,51 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
     const int K = h->K;
     const int T = h->substeps;
     const int nseg = (T + K - 1) / K;
+    const int64_t nn = h->nn;
+    if (h->grid_slots < K) {
+        if (int rc = ensure(&h->sgmq, h->rs * 4 * nn * K)) return rc;
+        if (int rc = ensure(&h->sgu, h->rs * 3 * nn * K)) return rc;
+        h->grid_slots = K;
+    }
+    R* const gmq = static_cast<R*>(h->sgmq);
+    R* const gus = static_cast<R*>(h->sgu);
     HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
+    // the slots' (m, q) and the velocity adjoint start at zero; the update's
+    // adjoint clears each after its last read, so they stay zero between uses
+    for (int j = 0; j < K; ++j)
+        hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(4 * nn)), dim3(256), 0, h->stream, gmq + 4 * nn * j, 4 * nn);
+    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * nn)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn);
     PS<R> a = slot<R>(h->abar, h->cap, 0), a2 = slot<R>(h->abar2, h->cap, 0);
     const R dt = (R)h->dt;
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * K, len = std::min(K, T - s0);
-        // recompute the segment's states from its checkpoint
+        // recompute the segment from its checkpoint, keeping each substep's
+        // grid (m, q, u) for the reverse walk; the last substep's particle
+        // update is the next checkpoint, already stored
         hipMemcpyAsync(slot<R>(h->seg, h->cap, 0).p, slot<R>(h->ck, h->cap, sg).p, h->rs * QCOMP * h->cap,
                        hipMemcpyDeviceToDevice, h->stream);
-        for (int j = 0; j < len - 1; ++j)
-            substep<R>(h, slot<R>(h->seg, h->cap, j), slot<R>(h->seg, h->cap, j + 1), s0 + j);
+        for (int j = 0; j < len; ++j) {
+            const Par<R> P = params_at<R>(h, s0 + j);
+            R* const gm = gmq + 4 * nn * j;
+            R* const gu = gus + 3 * nn * j;
+            hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, slot<R>(h->seg, h->cap, j),
+                               (const R*)h->m, (const R*)h->V0, h->n, h->box, P, dt, gm, gm + nn, h->err);
+            hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
+                               gm + nn, gu, 0);
+            if (j + 1 < len)
+                hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream,
+                                   slot<R>(h->seg, h->cap, j), slot<R>(h->seg, h->cap, j + 1), h->n, h->box, P, dt,
+                                   (const R*)gu, h->err);
+        }
         for (int j = len - 1; j >= 0; --j) {
             const PS<R> sj = slot<R>(h->seg, h->cap, j);
             const Par<R> P = params_at<R>(h, s0 + j);
-            // the grid of substep s0 + j
-            hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
-            hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
-            hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
-                               (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err);
-            hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
-                               (const R*)h->gm, (const R*)h->gq, (R*)h->gu);
-            hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gub,
-                               3 * h->nn);
+            R* const gm = gmq + 4 * nn * j;
+            R* const gu = gus + 3 * nn * j;
             hipLaunchKernelGGL(k_g2p_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, a, a2, h->n,
-                               h->box, P, dt, (const R*)h->gu, (R*)h->gub, h->grad);
-            hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt,
-                               (const R*)h->gm, (const R*)h->gq, (const R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad);
+                               h->box, P, dt, (const R*)gu, (R*)h->gub, h->grad);
+            hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
+                               gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad);
             hipLaunchKernelGGL(k_p2g_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
                                (const R*)h->V0, h->n, h->box, P, dt, (const R*)h->gqb, (const R*)h->gmb, a2, h->grad);
             std::swap(a, a2);
@@ -1155,7 +1199,7 @@ int bm_diff_destroy(bm_diff_handle h) {
     if (!h) return BM_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : {h->m, h->V0, h->x0, h->tgt, h->init, h->gm, h->gq, h->gu, h->gub, h->gqb, h->gmb, h->ck, h->seg, h->abar,
-                    h->abar2, (void*)h->grad, (void*)h->err})
+                    h->abar2, h->sgmq, h->sgu, (void*)h->grad, (void*)h->err})
         if (p) hipFree(p);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
