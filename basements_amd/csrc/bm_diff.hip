@@ -893,6 +893,8 @@ struct bm_diff {
     // the backward's grids of one segment: (m, q) and u of each of its substeps
     void *sgmq = nullptr, *sgu = nullptr;
     int64_t ck_slots = 0, seg_slots = 0, grid_slots = 0;
+    // device slot -> the caller's particle index (upload reorders, download restores)
+    std::vector<int64_t> order;
     double* grad = nullptr;
     int* err = nullptr;
     // last forward
@@ -1101,33 +1103,62 @@ template <class R> static int download_t(bm_diff_handle h, bm_particles* out) {
     HIP_TRY(hipMemcpy2D(st.data(), sizeof(R) * n, slot<R>(h->ck, h->cap, last).p, sizeof(R) * h->cap, sizeof(R) * n,
                         QCOMP, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i) {
+        const int64_t o = h->order[(size_t)i];
         for (int k = 0; k < 3; ++k) {
-            if (out->x) out->x[3 * i + k] = (float)st[(size_t)((QX + k) * n + i)];
-            if (out->v) out->v[3 * i + k] = (float)st[(size_t)((QV + k) * n + i)];
+            if (out->x) out->x[3 * o + k] = (float)st[(size_t)((QX + k) * n + i)];
+            if (out->v) out->v[3 * o + k] = (float)st[(size_t)((QV + k) * n + i)];
         }
         for (int k = 0; k < 9; ++k) {
-            if (out->F) out->F[9 * i + k] = (float)st[(size_t)((QF + k) * n + i)];
-            if (out->C) out->C[9 * i + k] = (float)st[(size_t)((QC + k) * n + i)];
+            if (out->F) out->F[9 * o + k] = (float)st[(size_t)((QF + k) * n + i)];
+            if (out->C) out->C[9 * o + k] = (float)st[(size_t)((QC + k) * n + i)];
         }
     }
     return BM_OK;
 }
 
+// The device order of the particles: grouped by 2 x 2 x 2 stencil bases
+// (floor(x / dx - 1/2) per axis), the caller's order kept inside a base.
+// A 64-lane wave of a bed with 8 particles per cell then covers a 4^3-node
+// tile, so the scatter kernels' LDS tiles stay small and their flush
+// atomics few; every result is a sum over particles or restored to the
+// caller's order on download.
+static std::vector<int64_t> device_order(const bm_diff_handle h, int64_t n, const float* x) {
+    const double inv_dx = 1.0 / (double)h->desc.dx;
+    const int ob[3] = {h->box.ox, h->box.oy, h->box.oz};
+    const int nb[3] = {h->box.nx, h->box.ny, h->box.nz};
+    std::vector<int64_t> key((size_t)n), ord((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t g = 0, sub = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double f = std::floor((double)x[3 * i + a] * inv_dx - 0.5) - ob[a];
+            const int64_t c = f < 0 ? 0 : (f > nb[a] ? nb[a] : (int64_t)f);
+            g = g * (nb[a] / 2 + 1) + (c >> 1);
+            sub = sub * 2 + (c & 1);
+        }
+        key[(size_t)i] = g * 8 + sub;
+        ord[(size_t)i] = i;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return key[(size_t)a] < key[(size_t)b]; });
+    return ord;
+}
+
 template <class R> static int upload_t(bm_diff_handle h, int64_t n, const bm_particles* p, const float* tgt) {
     std::vector<R> st((size_t)(QCOMP * n)), mm((size_t)n), vv((size_t)n), x0((size_t)(3 * n)), tg((size_t)(3 * n));
+    h->order = device_order(h, n, p->x);
     for (int64_t i = 0; i < n; ++i) {
+        const int64_t o = h->order[(size_t)i];  // slot i holds the caller's particle o
         for (int k = 0; k < 3; ++k) {
-            st[(size_t)((QX + k) * n + i)] = (R)p->x[3 * i + k];
-            st[(size_t)((QV + k) * n + i)] = (R)p->v[3 * i + k];
-            x0[(size_t)(3 * i + k)] = (R)p->x[3 * i + k];
-            tg[(size_t)(3 * i + k)] = tgt ? (R)tgt[3 * i + k] : R(0);
+            st[(size_t)((QX + k) * n + i)] = (R)p->x[3 * o + k];
+            st[(size_t)((QV + k) * n + i)] = (R)p->v[3 * o + k];
+            x0[(size_t)(3 * i + k)] = (R)p->x[3 * o + k];
+            tg[(size_t)(3 * i + k)] = tgt ? (R)tgt[3 * o + k] : R(0);
         }
         for (int k = 0; k < 9; ++k) {
-            st[(size_t)((QC + k) * n + i)] = (R)p->C[9 * i + k];
-            st[(size_t)((QF + k) * n + i)] = (R)p->F[9 * i + k];
+            st[(size_t)((QC + k) * n + i)] = (R)p->C[9 * o + k];
+            st[(size_t)((QF + k) * n + i)] = (R)p->F[9 * o + k];
         }
-        mm[(size_t)i] = (R)p->m[i];
-        vv[(size_t)i] = (R)p->V0[i];
+        mm[(size_t)i] = (R)p->m[o];
+        vv[(size_t)i] = (R)p->V0[o];
     }
     // stream-ordered with the handle's (non-blocking) stream; the host
     // vectors live until the synchronisation
