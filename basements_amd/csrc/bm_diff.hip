@@ -338,7 +338,7 @@ __device__ __forceinline__ void tile_flush(const TileBox& t, R (*tl)[STILE], con
 // ------------------------------------------------------------------ forward
 template <class R>
 __global__ void __launch_bounds__(64) k_p2g(PS<R> s, const R* m, const R* V0, int64_t n, Box b, Par<R> P, R dt,
-                                            R* gm, R* gq, int* err) {
+                                            R* gm, R* gq, int* err, const R* ks) {
     __shared__ R tl[4][STILE];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < n;
@@ -360,7 +360,13 @@ __global__ void __launch_bounds__(64) k_p2g(PS<R> s, const R* m, const R* V0, in
         const m3<R> C = load_m(s, QC, p), F = load_m(s, QF, p);
         const R mp = m[p];
         const R kf = -R(4) * P.inv_dx * P.inv_dx * V0[p];
-        m3<R> K = stress(F, P.mu, P.lam, P.ref_svd);
+        m3<R> K;
+        if (ks) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) K.a[c] = ks[c * s.cap + p];
+        } else {
+            K = stress(F, P.mu, P.lam, P.ref_svd);
+        }
 #pragma unroll
         for (int c = 0; c < 9; ++c) K.a[c] *= kf;
         for (int i = 0; i < 3; ++i)
@@ -547,7 +553,7 @@ __device__ __forceinline__ void g2p_forward(const R x[3], const m3<R>& F, const 
 
 template <class R>
 __global__ void __launch_bounds__(256) k_g2p(PS<R> s, PS<R> out, int64_t n, Box b, Par<R> P, R dt, const R* gu,
-                                             int* err) {
+                                             int* err, R* ks) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
@@ -567,6 +573,26 @@ __global__ void __launch_bounds__(256) k_g2p(PS<R> s, PS<R> out, int64_t n, Box 
     out.at(QV + 2, p) = vn.z;
     store_m(out, QC, p, f.Cn);
     store_m(out, QF, p, usv(f.U, f.dp.h, f.V));
+    if (ks) {
+        // the new F is U diag(s') V^T (s' = f.dp.h), so the Kirchhoff stress the
+        // next deposit needs, (U diag(hk) V^T) F^T, is U diag(hk s') U^T: no
+        // second SVD of the same F
+        const Hh<R> H = hencky_h(f.dp.h, P.mu, P.lam);
+        const m3<R> K = usv(f.U, mk(H.h.x * f.dp.h.x, H.h.y * f.dp.h.y, H.h.z * f.dp.h.z), f.U);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) ks[c * out.cap + p] = K.a[c];
+    }
+}
+
+// The Kirchhoff stress of each particle's F (component c of particle p at
+// ks[c * cap + p]): the first deposit after a checkpoint reads it.
+template <class R>
+__global__ void __launch_bounds__(256) k_stress(PS<R> s, int64_t n, Par<R> P, R* ks) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const m3<R> K = stress(load_m(s, QF, p), P.mu, P.lam, P.ref_svd);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) ks[c * s.cap + p] = K.a[c];
 }
 
 // ------------------------------------------------------------------ loss
@@ -892,6 +918,10 @@ struct bm_diff {
     void *ck = nullptr, *seg = nullptr, *abar = nullptr, *abar2 = nullptr;
     // the backward's grids of one segment: (m, q) and u of each of its substeps
     void *sgmq = nullptr, *sgu = nullptr;
+    // per-particle Kirchhoff stress handed from each G2P to the next P2G
+    // (unused with the reference's SVD stopping rule, whose comparisons
+    // factor every F as the reference does)
+    void* ks = nullptr;
     int64_t ck_slots = 0, seg_slots = 0, grid_slots = 0;
     // device slot -> the caller's particle index (upload reorders, download restores)
     std::vector<int64_t> order;
@@ -956,6 +986,19 @@ template <class R> static PS<R> slot(void* base, int64_t cap, int64_t k) {
     return PS<R>{static_cast<R*>(base) + k * QCOMP * cap, cap};
 }
 
+// The stress buffer the deposits read, or null where each deposit factors F
+// itself (the reference's SVD stopping rule).
+template <class R> static R* stress_buffer(bm_diff_handle h) {
+    return h->desc.reference_svd ? nullptr : static_cast<R*>(h->ks);
+}
+
+// The stress of state `st` into the buffer, before its first deposit.
+template <class R> static void seed_stress(bm_diff_handle h, PS<R> st, int step) {
+    if (R* ks = stress_buffer<R>(h))
+        hipLaunchKernelGGL(k_stress<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, st, h->n,
+                           params_at<R>(h, step), ks);
+}
+
 // Zero (m, q): once per forward or backward call.  Every substep then leaves
 // them zero for the next one (k_update or k_update_adj clears each node after
 // its last read).
@@ -969,12 +1012,13 @@ template <class R> static void clear_mass_momentum(bm_diff_handle h) {
 template <class R> static void substep(bm_diff_handle h, PS<R> in, PS<R> out, int step) {
     const Par<R> P = params_at<R>(h, step);
     const R dt = (R)h->dt;
+    R* const ks = stress_buffer<R>(h);
     hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, (const R*)h->m,
-                       (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err);
+                       (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err, (const R*)ks);
     hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt, (R*)h->gm,
                        (R*)h->gq, (R*)h->gu, 1);
     hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, out, h->n, h->box, P, dt,
-                       (const R*)h->gu, h->err);
+                       (const R*)h->gu, h->err, ks);
 }
 
 static int ensure(void** p, size_t bytes) {
@@ -1010,6 +1054,7 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
     HIP_TRY(hipMemcpyAsync(h->ck, h->init, h->rs * QCOMP * h->cap, hipMemcpyDeviceToDevice, h->stream));
     PS<R> cur = slot<R>(h->ck, h->cap, 0);
     clear_mass_momentum<R>(h);
+    seed_stress<R>(h, cur, 0);
     int pp = 0;  // states between checkpoints alternate between two scratch slots
     for (int s = 0; s < substeps; ++s) {
         const int ns = s + 1;
@@ -1063,18 +1108,21 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
         // update is the next checkpoint, already stored
         hipMemcpyAsync(slot<R>(h->seg, h->cap, 0).p, slot<R>(h->ck, h->cap, sg).p, h->rs * QCOMP * h->cap,
                        hipMemcpyDeviceToDevice, h->stream);
+        seed_stress<R>(h, slot<R>(h->seg, h->cap, 0), s0);
+        R* const ks = stress_buffer<R>(h);
         for (int j = 0; j < len; ++j) {
             const Par<R> P = params_at<R>(h, s0 + j);
             R* const gm = gmq + 4 * nn * j;
             R* const gu = gus + 3 * nn * j;
             hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, slot<R>(h->seg, h->cap, j),
-                               (const R*)h->m, (const R*)h->V0, h->n, h->box, P, dt, gm, gm + nn, h->err);
+                               (const R*)h->m, (const R*)h->V0, h->n, h->box, P, dt, gm, gm + nn, h->err,
+                               (const R*)ks);
             hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
                                gm + nn, gu, 0);
             if (j + 1 < len)
                 hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream,
                                    slot<R>(h->seg, h->cap, j), slot<R>(h->seg, h->cap, j + 1), h->n, h->box, P, dt,
-                                   (const R*)gu, h->err);
+                                   (const R*)gu, h->err, ks);
         }
         for (int j = len - 1; j >= 0; --j) {
             const PS<R> sj = slot<R>(h->seg, h->cap, j);
@@ -1216,6 +1264,7 @@ int bm_diff_create(const bm_diff_desc* desc, bm_diff_handle* out) {
     if (!rc) rc = ensure(&h->gub, r * 3 * h->nn);
     if (!rc) rc = ensure(&h->gqb, r * 3 * h->nn);
     if (!rc) rc = ensure(&h->gmb, r * h->nn);
+    if (!rc) rc = ensure(&h->ks, r * 9 * h->cap);
     if (!rc) rc = ensure(&h->abar, r * QCOMP * h->cap);
     if (!rc) rc = ensure(&h->abar2, r * QCOMP * h->cap);
     if (!rc) rc = ensure((void**)&h->grad, sizeof(double) * GN);
@@ -1230,7 +1279,7 @@ int bm_diff_destroy(bm_diff_handle h) {
     if (!h) return BM_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : {h->m, h->V0, h->x0, h->tgt, h->init, h->gm, h->gq, h->gu, h->gub, h->gqb, h->gmb, h->ck, h->seg, h->abar,
-                    h->abar2, h->sgmq, h->sgu, (void*)h->grad, (void*)h->err})
+                    h->abar2, h->sgmq, h->sgu, h->ks, (void*)h->grad, (void*)h->err})
         if (p) hipFree(p);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
