@@ -272,6 +272,23 @@ template <class R> __device__ __forceinline__ void wave_add(double* dst, R v) {
     if ((threadIdx.x & 63) == 0 && x != 0.0) atomicAdd(dst, x);
 }
 
+// ------------------------------------------------------------------ scatter order
+// A scatter lane walks its 27 stencil nodes starting at node `rot` (lane
+// index mod 27): the particles of one cell share a stencil base and sit in
+// consecutive lanes, so with a common start they would add to the same LDS
+// address at every step, and same-address LDS atomics serialise.
+__device__ __forceinline__ void rotated_node(int t, int rot, int& i, int& j, int& k) {
+    int q = t + rot;
+    q = q >= 27 ? q - 27 : q;
+    i = q / 9;
+    j = (q / 3) % 3;
+    k = q % 3;
+}
+// a[i] for a run-time i without an indexed (scratch) access
+template <class R> __device__ __forceinline__ R pick3(const R (&a)[3], int i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+}
+
 // ------------------------------------------------------------------ scatter tile
 // The scatter kernels run one wave per workgroup over consecutive particles,
 // which sit close together: the wave's deposits are summed in an LDS tile
@@ -369,10 +386,11 @@ __global__ void __launch_bounds__(64) k_p2g(PS<R> s, const R* m, const R* V0, in
         }
 #pragma unroll
         for (int c = 0; c < 9; ++c) K.a[c] *= kf;
-        for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j)
-            for (int k = 0; k < 3; ++k) {
-                const R w = st.w[0][i] * st.w[1][j] * st.w[2][k];
+        const int rot = (int)(threadIdx.x % 27u);
+        for (int t = 0; t < 27; ++t) {
+                int i, j, k;
+                rotated_node(t, rot, i, j, k);
+                const R w = pick3(st.w[0], i) * pick3(st.w[1], j) * pick3(st.w[2], k);
                 const int ni = st.b[0] + i, nj = st.b[1] + j, nk = st.b[2] + k;
                 const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
                 const v3<R> a = (v + mvec(C, d)) * mp + mvec(K, d) * dt;
@@ -676,13 +694,14 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
         vb = vb + xb * dt;
         v3<R> xo = xb;
         const R kC = R(4) * P.inv_dx * P.inv_dx;
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
-                for (int k = 0; k < 3; ++k) {
-                    const R w = f.st.w[0][i] * f.st.w[1][j] * f.st.w[2][k];
-                    const v3<R> gw = mk(f.st.dw[0][i] * f.st.w[1][j] * f.st.w[2][k],
-                                        f.st.w[0][i] * f.st.dw[1][j] * f.st.w[2][k],
-                                        f.st.w[0][i] * f.st.w[1][j] * f.st.dw[2][k]);
+        const int rot = (int)(threadIdx.x % 27u);
+        for (int t = 0; t < 27; ++t) {
+                    int i, j, k;
+                    rotated_node(t, rot, i, j, k);
+                    const R wi = pick3(f.st.w[0], i), wj = pick3(f.st.w[1], j), wk = pick3(f.st.w[2], k);
+                    const R w = wi * wj * wk;
+                    const v3<R> gw = mk(pick3(f.st.dw[0], i) * wj * wk, wi * pick3(f.st.dw[1], j) * wk,
+                                        wi * wj * pick3(f.st.dw[2], k));
                     const int ni = f.st.b[0] + i, nj = f.st.b[1] + j, nk = f.st.b[2] + k;
                     const int64_t id = node_index<R>(b, ni, nj, nk);
                     if (id < 0) continue;
