@@ -273,8 +273,8 @@ template <class R> __device__ __forceinline__ void wave_add(double* dst, R v) {
 }
 
 // ------------------------------------------------------------------ scatter order
-// A scatter lane walks its 27 stencil nodes starting at node `rot` (lane
-// index mod 27): the particles of one cell share a stencil base and sit in
+// The deposit's lanes walk their 27 stencil nodes starting at node `rot`
+// (lane index mod 27): the particles of one cell share a stencil base and sit in
 // consecutive lanes, so with a common start they would add to the same LDS
 // address at every step, and same-address LDS atomics serialise.
 __device__ __forceinline__ void rotated_node(int t, int rot, int& i, int& j, int& k) {
@@ -694,14 +694,15 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
         vb = vb + xb * dt;
         v3<R> xo = xb;
         const R kC = R(4) * P.inv_dx * P.inv_dx;
-        const int rot = (int)(threadIdx.x % 27u);
-        for (int t = 0; t < 27; ++t) {
-                    int i, j, k;
-                    rotated_node(t, rot, i, j, k);
-                    const R wi = pick3(f.st.w[0], i), wj = pick3(f.st.w[1], j), wk = pick3(f.st.w[2], k);
-                    const R w = wi * wj * wk;
-                    const v3<R> gw = mk(pick3(f.st.dw[0], i) * wj * wk, wi * pick3(f.st.dw[1], j) * wk,
-                                        wi * wj * pick3(f.st.dw[2], k));
+        // a fixed node order here: the run-time rotation of the deposit costs
+        // this heavier loop more than its LDS collisions do
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    const R w = f.st.w[0][i] * f.st.w[1][j] * f.st.w[2][k];
+                    const v3<R> gw = mk(f.st.dw[0][i] * f.st.w[1][j] * f.st.w[2][k],
+                                        f.st.w[0][i] * f.st.dw[1][j] * f.st.w[2][k],
+                                        f.st.w[0][i] * f.st.w[1][j] * f.st.dw[2][k]);
                     const int ni = f.st.b[0] + i, nj = f.st.b[1] + j, nk = f.st.b[2] + k;
                     const int64_t id = node_index<R>(b, ni, nj, nk);
                     if (id < 0) continue;
