@@ -938,6 +938,12 @@ struct bm_diff {
     void *ck = nullptr, *seg = nullptr, *abar = nullptr, *abar2 = nullptr;
     // the backward's grids of one segment: (m, q) and u of each of its substeps
     void *sgmq = nullptr, *sgu = nullptr;
+    // every substep's grid (m, q and u) of the last forward, kept for the
+    // backward when it fits grid_budget bytes (then the backward recomputes
+    // only the particle states); all_grids: the last forward kept them
+    void *agmq = nullptr, *agu = nullptr;
+    int64_t all_slots = 0, grid_budget = int64_t(4) << 30;
+    bool all_grids = false;
     // per-particle Kirchhoff stress handed from each G2P to the next P2G
     // (unused with the reference's SVD stopping rule, whose comparisons
     // factor every F as the reference does)
@@ -1033,12 +1039,17 @@ template <class R> static void substep(bm_diff_handle h, PS<R> in, PS<R> out, in
     const Par<R> P = params_at<R>(h, step);
     const R dt = (R)h->dt;
     R* const ks = stress_buffer<R>(h);
+    const int64_t nn = h->nn;
+    // kept grids: this substep's own (m, q, u), left in place for the backward
+    R* const gm = h->all_grids ? static_cast<R*>(h->agmq) + 4 * nn * step : (R*)h->gm;
+    R* const gq = h->all_grids ? gm + nn : (R*)h->gq;
+    R* const gu = h->all_grids ? static_cast<R*>(h->agu) + 3 * nn * step : (R*)h->gu;
     hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, (const R*)h->m,
-                       (const R*)h->V0, h->n, h->box, P, dt, (R*)h->gm, (R*)h->gq, h->err, (const R*)ks);
-    hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, h->box, P, dt, (R*)h->gm,
-                       (R*)h->gq, (R*)h->gu, 1);
+                       (const R*)h->V0, h->n, h->box, P, dt, gm, gq, h->err, (const R*)ks);
+    hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm, gq, gu,
+                       h->all_grids ? 0 : 1);
     hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, out, h->n, h->box, P, dt,
-                       (const R*)h->gu, h->err, ks);
+                       (const R*)gu, h->err, ks);
 }
 
 static int ensure(void** p, size_t bytes) {
@@ -1073,7 +1084,20 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
     }
     HIP_TRY(hipMemcpyAsync(h->ck, h->init, h->rs * QCOMP * h->cap, hipMemcpyDeviceToDevice, h->stream));
     PS<R> cur = slot<R>(h->ck, h->cap, 0);
-    clear_mass_momentum<R>(h);
+    h->all_grids = (int64_t)substeps * 7 * h->nn * (int64_t)h->rs <= h->grid_budget;
+    if (h->all_grids && h->all_slots < substeps) {
+        // an allocation that fails leaves the per-segment grids in use
+        h->all_slots = 0;
+        h->all_grids = ensure(&h->agmq, h->rs * 4 * h->nn * substeps) == BM_OK &&
+                       ensure(&h->agu, h->rs * 3 * h->nn * substeps) == BM_OK;
+        if (h->all_grids) h->all_slots = substeps;
+    }
+    if (h->all_grids) {
+        hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(4 * h->nn * substeps)), dim3(256), 0, h->stream,
+                           static_cast<R*>(h->agmq), 4 * h->nn * substeps);
+    } else {
+        clear_mass_momentum<R>(h);
+    }
     seed_stress<R>(h, cur, 0);
     int pp = 0;  // states between checkpoints alternate between two scratch slots
     for (int s = 0; s < substeps; ++s) {
@@ -1101,11 +1125,54 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
     return BM_OK;
 }
 
+// The backward over the grids the forward kept: each segment's particle
+// states are recomputed from its checkpoint with the kept velocities, then
+// walked in reverse.
+template <class R> static int backward_kept(bm_diff_handle h, double* g) {
+    const int K = h->K;
+    const int T = h->substeps;
+    const int nseg = (T + K - 1) / K;
+    const int64_t nn = h->nn;
+    R* const gmq = static_cast<R*>(h->agmq);
+    R* const gus = static_cast<R*>(h->agu);
+    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
+    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * nn)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn);
+    PS<R> a = slot<R>(h->abar, h->cap, 0), a2 = slot<R>(h->abar2, h->cap, 0);
+    const R dt = (R)h->dt;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * K, len = std::min(K, T - s0);
+        hipMemcpyAsync(slot<R>(h->seg, h->cap, 0).p, slot<R>(h->ck, h->cap, sg).p, h->rs * QCOMP * h->cap,
+                       hipMemcpyDeviceToDevice, h->stream);
+        for (int j = 0; j + 1 < len; ++j)
+            hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, slot<R>(h->seg, h->cap, j),
+                               slot<R>(h->seg, h->cap, j + 1), h->n, h->box, params_at<R>(h, s0 + j), dt,
+                               (const R*)(gus + 3 * nn * (s0 + j)), h->err, (R*)nullptr);
+        for (int j = len - 1; j >= 0; --j) {
+            const PS<R> sj = slot<R>(h->seg, h->cap, j);
+            const Par<R> P = params_at<R>(h, s0 + j);
+            R* const gm = gmq + 4 * nn * (s0 + j);
+            R* const gu = gus + 3 * nn * (s0 + j);
+            hipLaunchKernelGGL(k_g2p_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, a, a2, h->n,
+                               h->box, P, dt, (const R*)gu, (R*)h->gub, h->grad);
+            hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
+                               gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad);
+            hipLaunchKernelGGL(k_p2g_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
+                               (const R*)h->V0, h->n, h->box, P, dt, (const R*)h->gqb, (const R*)h->gmb, a2, h->grad);
+            std::swap(a, a2);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    if (int rc = read_err(h)) return rc;
+    HIP_TRY(hipMemcpy(g, h->grad, sizeof(double) * GN, hipMemcpyDeviceToHost));
+    return BM_OK;
+}
+
 template <class R> static int backward_t(bm_diff_handle h, double* g) {
     const int K = h->K;
     const int T = h->substeps;
     const int nseg = (T + K - 1) / K;
     const int64_t nn = h->nn;
+    if (h->all_grids) return backward_kept<R>(h, g);
     if (h->grid_slots < K) {
         if (int rc = ensure(&h->sgmq, h->rs * 4 * nn * K)) return rc;
         if (int rc = ensure(&h->sgu, h->rs * 3 * nn * K)) return rc;
@@ -1299,7 +1366,7 @@ int bm_diff_destroy(bm_diff_handle h) {
     if (!h) return BM_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : {h->m, h->V0, h->x0, h->tgt, h->init, h->gm, h->gq, h->gu, h->gub, h->gqb, h->gmb, h->ck, h->seg, h->abar,
-                    h->abar2, h->sgmq, h->sgu, h->ks, (void*)h->grad, (void*)h->err})
+                    h->abar2, h->sgmq, h->sgu, h->ks, h->agmq, h->agu, (void*)h->grad, (void*)h->err})
         if (p) hipFree(p);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -1315,6 +1382,15 @@ int bm_diff_set_params(bm_diff_handle h, const bm_diff_params* p) {
         return fail(BM_ERR_INVALID_ARG, "tool half extents must be > 0");
     h->prm = *p;
     h->has_prm = true;
+    h->have_forward = false;
+    return BM_OK;
+}
+
+int bm_diff_set_grid_budget(bm_diff_handle h, int64_t bytes) {
+    if (int rc = check_diff(h)) return rc;
+    if (bytes < 0) return fail(BM_ERR_INVALID_ARG, "grid budget must be >= 0");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->grid_budget = bytes;
     h->have_forward = false;
     return BM_OK;
 }

@@ -268,6 +268,7 @@ _SIGS = {
     "bm_diff_create": (C.c_int, [C.POINTER(DiffDescC), C.POINTER(C.c_void_p)]),
     "bm_diff_destroy": (C.c_int, [C.c_void_p]),
     "bm_diff_set_params": (C.c_int, [C.c_void_p, C.POINTER(DiffParamsC)]),
+    "bm_diff_set_grid_budget": (C.c_int, [C.c_void_p, C.c_int64]),
     "bm_diff_upload": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(HostParticles), C.POINTER(C.c_float)]),
     "bm_diff_forward": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_double)]),
     "bm_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(DiffGradC)]),
@@ -955,6 +956,11 @@ class Rollout:
     def set_params(self, p: DiffParams):
         pc = p.to_c()
         _check(lib().bm_diff_set_params(self.h, C.byref(pc)))
+
+    def set_grid_budget(self, nbytes: int):
+        """Device bytes the forward may keep its substeps' grids in for the
+        backward (default 4 GiB; 0 = recompute every segment's grids)."""
+        _check(lib().bm_diff_set_grid_budget(self.h, int(nbytes)))
 
     def upload(self, P, target=None):
         n = int(np.asarray(P["x"]).reshape(-1, 3).shape[0])

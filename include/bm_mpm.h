@@ -478,6 +478,13 @@ typedef struct {
 int bm_diff_create(const bm_diff_desc* desc, bm_diff_handle* out);
 int bm_diff_destroy(bm_diff_handle h);
 int bm_diff_set_params(bm_diff_handle h, const bm_diff_params* params);
+/* Device memory the forward may spend keeping every substep's grid (m, q and
+ * the updated velocity: 7 values per node and substep) for the backward,
+ * which then recomputes only the particle states of each checkpoint segment
+ * instead of every segment's grids too.  A forward whose grids exceed it
+ * keeps none.  Default 4 GiB; 0 = never.  No counterpart in the reference
+ * (it has no adjoint); takes effect from the next forward. */
+int bm_diff_set_grid_budget(bm_diff_handle h, int64_t bytes);
 /* Initial particle state and the target displacement (3n floats; NULL = 0). */
 int bm_diff_upload(bm_diff_handle h, int64_t n, const bm_particles* p, const float* target_displacement);
 /* Run `substeps` substeps, saving a state every checkpoint_every; the loss. */

@@ -82,12 +82,17 @@ def test_adjoint_matches_central_differences(substeps):
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
+    """K = 1, 16, 48, each with the substeps' grids kept by the forward (the
+    default budget) and recomputed per segment by the backward (budget 0)."""
     sc = _scene()
     out = []
     for K in (1, 16, 48):
-        R = _rollout(sc, K=K)
-        R.forward(sc["dt"], 48)
-        out.append(R.backward())
+        for budget in (None, 0):
+            R = _rollout(sc, K=K)
+            if budget is not None:
+                R.set_grid_budget(budget)
+            R.forward(sc["dt"], 48)
+            out.append(R.backward())
     for g in out[1:]:
         for key in ("young_modulus", "friction_deg"):
             assert g[key] == pytest.approx(out[0][key], rel=1e-9)
