@@ -632,11 +632,15 @@ template <class R> struct Body {
     R lin_damp, ang_damp;
     V3<R> f, t;
 };
-template <class R> static void integrate_body(Body<R>& b, R dt) {
-    b.v = add(b.v, mul(mul(b.f, b.inv_mass), dt));
-    b.w = add(b.w, mul(mv(b.inv_I, b.t), dt));
-    b.v = mul(b.v, R(1) - b.lin_damp * dt);
-    b.w = mul(b.w, R(1) - b.ang_damp * dt);
+// kinematic: the body moves at its own velocities whatever the forces (a
+// dragged tool, as the research harness's tine drag moves its tine).
+template <class R> static void integrate_body(Body<R>& b, R dt, bool kinematic = false) {
+    if (!kinematic) {
+        b.v = add(b.v, mul(mul(b.f, b.inv_mass), dt));
+        b.w = add(b.w, mul(mv(b.inv_I, b.t), dt));
+        b.v = mul(b.v, R(1) - b.lin_damp * dt);
+        b.w = mul(b.w, R(1) - b.ang_damp * dt);
+    }
     b.x = add(b.x, mul(b.v, dt));
     const Q4<R> qv{R(0), b.w.x, b.w.y, b.w.z};
     const Q4<R>& o = b.q;
@@ -691,6 +695,64 @@ template <class R> void store_reaction(const Reaction<R>& r, double* o) {
     o[0] = r.f.x; o[1] = r.f.y; o[2] = r.f.z;
     o[3] = r.t.x; o[4] = r.t.y; o[5] = r.t.z;
     o[6] = (double)r.n;
+}
+
+// The coupled substep in the contract order of mpm_rigid_coupler.h:209-211 and
+// the body half of bench_compare_all.cpp:642-663: force += applied load;
+// P2G; grid update; reaction = coupler(grid, collider of the body, dt);
+// force += reaction.force, torque += reaction.torque; TimeIntegrator::
+// integrate; G2P.  col: the collider template (shape, half extents,
+// friction, mass; pose and velocities come from the body).  body: 31 values
+// as orc_*_integrate_body, in/out.  applied: force(3) torque(3).  rec (may be
+// null): per substep the body after integration and the reaction,
+// x(3) q(4) v(3) w(3) force(3) torque(3) nodes_inside = 20 values.
+// motion: 1 dynamic (TimeIntegrator::integrate), 2 kinematic.
+template <class R> void sim_step_coupled(void* h, int method, const double* col, double* bd, const double* applied,
+                                         double dt, int nsteps, double* rec, int motion) {
+    Sim<R>& S = *(Sim<R>*)h;
+    Collider<R> c = load_col<R>(col);
+    Body<R> b;
+    b.x = v3<R>((R)bd[0], (R)bd[1], (R)bd[2]);
+    b.q = Q4<R>{(R)bd[3], (R)bd[4], (R)bd[5], (R)bd[6]};
+    b.v = v3<R>((R)bd[7], (R)bd[8], (R)bd[9]);
+    b.w = v3<R>((R)bd[10], (R)bd[11], (R)bd[12]);
+    b.inv_mass = (R)bd[13];
+    b.inv_I = load_m3<R>(bd + 14);
+    b.lin_damp = (R)bd[23];
+    b.ang_damp = (R)bd[24];
+    b.f = v3<R>((R)bd[25], (R)bd[26], (R)bd[27]);
+    b.t = v3<R>((R)bd[28], (R)bd[29], (R)bd[30]);
+    const V3<R> af = v3<R>((R)applied[0], (R)applied[1], (R)applied[2]);
+    const V3<R> at = v3<R>((R)applied[3], (R)applied[4], (R)applied[5]);
+    for (int s_ = 0; s_ < nsteps; ++s_) {
+        b.f = add(b.f, af);
+        b.t = add(b.t, at);
+        S.p2g();
+        S.update((R)dt);
+        c.c = b.x;
+        c.q = b.q;
+        c.vl = b.v;
+        c.wa = b.w;
+        const Reaction<R> r = method == 0 ? couple_m0<R>(S, c, (R)dt) : couple_m1<R>(S, c, (R)dt);
+        b.f = add(b.f, r.f);
+        b.t = add(b.t, r.t);
+        integrate_body<R>(b, (R)dt, motion == 2);
+        S.g2p((R)dt);
+        if (rec) {
+            double* o = rec + 20 * (int64_t)s_;
+            o[0] = b.x.x; o[1] = b.x.y; o[2] = b.x.z;
+            o[3] = b.q.w; o[4] = b.q.x; o[5] = b.q.y; o[6] = b.q.z;
+            o[7] = b.v.x; o[8] = b.v.y; o[9] = b.v.z;
+            o[10] = b.w.x; o[11] = b.w.y; o[12] = b.w.z;
+            store_reaction<R>(r, o + 13);
+        }
+    }
+    bd[0] = b.x.x; bd[1] = b.x.y; bd[2] = b.x.z;
+    bd[3] = b.q.w; bd[4] = b.q.x; bd[5] = b.q.y; bd[6] = b.q.z;
+    bd[7] = b.v.x; bd[8] = b.v.y; bd[9] = b.v.z;
+    bd[10] = b.w.x; bd[11] = b.w.y; bd[12] = b.w.z;
+    bd[25] = b.f.x; bd[26] = b.f.y; bd[27] = b.f.z;
+    bd[28] = b.t.x; bd[29] = b.t.y; bd[30] = b.t.z;
 }
 
 template <class R> void* sim_create(double dx, int mass_mode, int q_exp) {
@@ -802,6 +864,11 @@ template <class R> void sim_grid_set_dense(void* h, const int* org, const int* d
         Collider<R> c = load_col<R>(col);                                                                          \
         Reaction<R> r = method == 0 ? couple_m0<R>(*(Sim<R>*)h, c, (R)dt) : couple_m1<R>(*(Sim<R>*)h, c, (R)dt);  \
         store_reaction<R>(r, out);                                                                                 \
+    }                                                                                                              \
+    extern "C" void orc_##SUF##_step_coupled(void* h, int method, const double* col, double* body,                \
+                                             const double* applied, double dt, int nsteps, double* rec,            \
+                                             int motion) {                                                         \
+        sim_step_coupled<R>(h, method, col, body, applied, dt, nsteps, rec, motion);                               \
     }                                                                                                              \
     extern "C" int64_t orc_##SUF##_num_blocks(void* h) { return (int64_t)((Sim<R>*)h)->blocks.size(); }           \
     extern "C" void orc_##SUF##_grid_dense(void* h, const int* org, const int* dims, int64_t* mq, double* mass,    \

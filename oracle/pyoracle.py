@@ -193,6 +193,7 @@ class OracleSim(_Base):
             ("plasticity", None, [C.c_void_p, _dp, _dp, C.c_double, _dp]),
             ("weights", None, [C.c_void_p, _dp, _dp, _ip, _dp]),
             ("integrate_body", None, [_dp, C.c_double]),
+            ("step_coupled", None, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, _dp, C.c_int]),
         ]:
             f = getattr(L, f"orc_{precision}_{nm}")
             f.restype = res
@@ -307,6 +308,23 @@ class OracleSim(_Base):
         a = body.pack()
         self._f("integrate_body")(_p(a), float(dt))
         return Body.unpack(a, body)
+
+    def step_coupled(self, method, col: Collider, body: Body, applied_force, applied_torque, dt, n,
+                     kinematic=False):
+        """n coupled substeps in the contract order (mpm_rigid_coupler.h:209-211,
+        bench_compare_all.cpp:642-663): force += applied; P2G; update;
+        coupler on the body's collider; reaction into the body; integrate;
+        G2P.  col supplies shape, half extents, friction and mass.  Returns
+        the body after the last substep and an (n, 20) record per substep:
+        x(3) q(4) v(3) w(3) reaction force(3) torque(3) nodes_inside.
+        kinematic: the body moves at its own velocities (x += v dt, the
+        quaternion turned by w), the reaction is only reported."""
+        a = body.pack()
+        ap = np.array([*applied_force, *applied_torque], np.float64)
+        rec = np.zeros((int(n), 20))
+        self._f("step_coupled")(self.h, int(method), _p(col.pack()), _p(a), _p(ap), float(dt), int(n), _p(rec),
+                                2 if kinematic else 1)
+        return Body.unpack(a, body), rec
 
 
 class RefSim(_Base):

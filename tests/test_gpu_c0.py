@@ -1,0 +1,168 @@
+"""The box-drop workload at its full size against the FP64 oracle, over 1000
+substeps, in both modes.
+
+Scene (basements_amd.scenes.box_drop): a 0.5 x 0.25 x 0.5 m soil block on a
+64^3-node grid (dx = 1/64 m), 8 jittered particles per cell (131,072
+particles), rho 1600, E 3.5e5, nu 0.3, friction 35 deg, floor friction 0.4,
+gravity -9.81, dt 1e-4; a 0.1 m, 1 kg rigid cube coupled by M0 (friction 0.5),
+with the reference's scenario body settings (no damping, inverse inertia
+left at identity).  Two variants:
+  * drop: the cube starts at rest 0.1 m above the surface -- it needs 0.143 s
+    to land, longer than the 0.1 s run, so the coupler finds no node;
+  * contact: the cube's bottom face starts on the surface at -1.40 m/s (the
+    free-fall speed from 0.1 m), so M0 engages from the first substep.
+The coupled loop runs in the contract order of mpm_rigid_coupler.h:209-211
+(bench_compare_all.cpp:642-663): body load, P2G, grid update, M0 on the
+body's collider, reaction into the body, TimeIntegrator::integrate, G2P.
+
+The yardstick is the reference's own arithmetic: the FP32 oracle (bit-exact
+to the reference build) against the FP64 oracle on the same input.  Measured
+over these 1000 substeps it reaches about 5e-5 (x), 1e-3 (v), 3e-4 (F) and
+2e-3 (C) of each field's largest magnitude, the cube's position 2e-5 m, and
+M0's nodes_inside differs from the FP64 run at some substeps by a few nodes
+or one face layer (a face crossing a node plane one substep earlier or
+later).  Bars, device against the FP64 oracle:
+  * grid mass: the fixed-point total equals the particles' quanta after
+    every substep (bit-exact);
+  * particle x, v, F, C and the body's position and velocity after 1000
+    substeps: within 2.5x the FP32 oracle's own distance, plus 1e-6;
+  * nodes_inside at every 10th substep: equal, except at no more samples
+    than 3x the FP32 oracle's mismatching samples (plus 3), each off by at
+    most one face layer of the cube;
+  * M0 reaction force at every 10th substep where nodes_inside agrees:
+    within 2.5x the FP32 oracle's largest error relative to the series'
+    largest magnitude, plus 1e-6.
+"""
+from __future__ import annotations
+
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+from _util import gpu_solver, params_from, rel_err
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(1200)]
+
+N_SUB = 1000
+CHUNK = 100
+EVERY = 10
+DT = 1e-4
+
+
+def _scene():
+    from basements_amd.scenes import box_drop
+
+    return box_drop(seed=0)
+
+
+def _setup(variant):
+    sc = _scene()
+    t = sc.tool
+    c = list(t["center"])
+    v0 = (0.0, 0.0, 0.0)
+    if variant == "contact":
+        top = float(sc.particles["x"][:, 1].max())
+        c[1] = 0.25 + t["half"][1]  # bottom face on the bed top (y = 16 dx = 0.25 m)
+        assert top <= 0.25
+        v0 = (0.0, -1.40, 0.0)
+    return sc, tuple(c), v0
+
+
+def _oracle_run(variant, prec, out):
+    from oracle.pyoracle import Body, Collider, OracleSim, Params
+
+    sc, c, v0 = _setup(variant)
+    prm = Params(E=sc.E, nu=sc.nu, rho=sc.rho, alpha=sc.alpha, mu_floor=sc.mu_floor, gravity=sc.gravity)
+    o = OracleSim(sc.dx, prec, params=prm)
+    o.set_particles({k: np.asarray(v, np.float64) for k, v in sc.particles.items()})
+    col = Collider(center=c, half_extents=sc.tool["half"], friction=sc.tool["friction"], mass=1.0, shape=0)
+    body = Body(x=c, v=v0, inv_mass=1.0, lin_damp=0.0, ang_damp=0.0)
+    recs = []
+    for k in range(0, N_SUB, CHUNK):
+        body, rec = o.step_coupled(0, col, body, (0.0, -9.81, 0.0), (0.0, 0.0, 0.0), DT, CHUNK)
+        recs.append(rec)
+        # the oracle takes minutes: report each chunk past pytest's capture
+        print(f"oracle {variant} {prec}: {k + CHUNK} of {N_SUB} substeps", file=sys.__stderr__, flush=True)
+    out[(variant, prec)] = (o.get_particles(), np.concatenate(recs))
+
+
+@pytest.fixture(scope="module")
+def oracle_runs():
+    """Both variants in FP32 and FP64, four oracle runs in parallel threads
+    (the oracle's calls release the interpreter lock)."""
+    out = {}
+    ths = [threading.Thread(target=_oracle_run, args=(v, p, out)) for v in ("drop", "contact") for p in ("f32", "f64")]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert len(out) == 4, "an oracle run failed"
+    return out
+
+
+def _gpu_run(variant, mode):
+    from basements_amd import BODY_DYNAMIC, M0, SHAPE_BOX, Body
+
+    sc, c, v0 = _setup(variant)
+    P = sc.particles
+    prm = params_from(E=sc.E, nu=sc.nu, rho=sc.rho, alpha=sc.alpha, mu_floor=sc.mu_floor, g=sc.gravity)
+    S = gpu_solver(P, sc.dx, sc.origin, sc.dims, prm, mode=mode)
+    S.set_body(Body(position=c, linear_velocity=v0, inv_mass=1.0, linear_damping=0.0, angular_damping=0.0,
+                    applied_force=(0.0, -9.81, 0.0), half_extents=sc.tool["half"], friction=sc.tool["friction"],
+                    mass=1.0, shape=SHAPE_BOX, integrate=BODY_DYNAMIC))
+    q_exp = S.mass_quantum_exp()
+    want_q = int(np.rint(P["m"].astype(np.float64) * 2.0 ** (-q_exp)).astype(np.int64).sum())
+    rec = []
+    for k in range(N_SUB):
+        S.step_coupled(M0, DT, 1)
+        got_q = S.grid_mass_total()
+        assert got_q == want_q, f"substep {k}: grid mass {got_q} quanta, particles hold {want_q}"
+        if (k + 1) % EVERY == 0:
+            body, (f, t, n) = S.get_body()
+            rec.append((k, np.array(body.position), np.array(body.linear_velocity), f, n))
+    return S.download(), S.get_body()[0], rec
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["reference", "fast"])
+@pytest.mark.parametrize("variant", ["drop", "contact"])
+def test_box_drop_1000_substeps_at_fp32_noise_floor(oracle_runs, variant, mode):
+    P32, r32 = oracle_runs[(variant, "f32")]
+    P64, r64 = oracle_runs[(variant, "f64")]
+    Pg, body, rec = _gpu_run(variant, mode)
+    report = {}
+    for key in ("x", "v", "F", "C"):
+        floor = rel_err(P32[key], P64[key])
+        got = rel_err(Pg[key], P64[key])
+        report[key] = (got, floor)
+        assert got <= 2.5 * floor + 1e-6, (variant, mode, key, got, floor)
+    last64 = r64[-1]
+    for what, g, lo, hi in (("position", body.position, 0, 3), ("velocity", body.linear_velocity, 7, 10)):
+        floor = float(np.abs(r32[-1, lo:hi] - last64[lo:hi]).max())
+        got = float(np.abs(np.asarray(g) - last64[lo:hi]).max())
+        report["body " + what] = (got, floor)
+        assert got <= 2.5 * floor + 1e-6, (variant, mode, what, got, floor)
+    steps = np.array([k for k, *_ in rec])
+    ni_g = np.array([n for *_, n in rec])
+    ni_64 = r64[steps, 19].astype(int)
+    ni_32 = r32[steps, 19].astype(int)
+    layer = int(np.ceil(2 * 0.05 * 64) + 1) ** 2  # nodes on one face of the cube
+    mism_g = ni_g != ni_64
+    mism_32 = int((ni_32 != ni_64).sum())
+    report["nodes_inside mismatching samples"] = (int(mism_g.sum()), mism_32)
+    assert mism_g.sum() <= 3 * mism_32 + 3, (variant, mode, int(mism_g.sum()), mism_32)
+    assert np.abs(ni_g - ni_64).max(initial=0) <= layer
+    if variant == "contact":
+        assert ni_64.max() > 0 and ni_g.max() > 0, "the cube never touched the soil: the test would pin nothing"
+    else:
+        assert ni_64.max() == 0 and ni_g.max() == 0
+    fy_g = np.array([f[1] for *_, f, _ in rec])
+    sc64 = max(float(np.abs(r64[:, 14]).max()), 1e-30)
+    agree = ~mism_g & (ni_32 == ni_64)
+    if agree.any():
+        floor = float(np.abs(r32[steps, 14] - r64[steps, 14])[agree].max()) / sc64
+        got = float(np.abs(fy_g - r64[steps, 14])[agree].max()) / sc64
+        report["reaction fy"] = (got, floor)
+        assert got <= 2.5 * floor + 1e-6, (variant, mode, got, floor)
+    print(f"box_drop {variant} mode {mode}: (device vs FP64, FP32 oracle vs FP64) {report}")

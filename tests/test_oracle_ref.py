@@ -114,3 +114,53 @@ def test_rigid_integrator_bit_exact():
         assert np.array_equal(bo.pack(), br.pack())
         b = Body(x=bo.x, quat=bo.quat, v=bo.v, w=bo.w, inv_mass=b.inv_mass, inv_I=b.inv_I, lin_damp=b.lin_damp,
                  ang_damp=b.ang_damp, force=(1.0, 2.0, -3.0), torque=(0.1, 0.2, 0.3))
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_coupled_loop_bit_exact(method):
+    """The oracle's coupled loop (orc_*_step_coupled) against the reference's
+    own phases, coupler and integrator composed in the contract order of
+    mpm_rigid_coupler.h:209-211 / bench_compare_all.cpp:642-663, with the
+    body's force accumulated in FP32 as RigidBody holds it."""
+    o, r = _scene(4, n=500)
+    f32 = lambda a: np.asarray(a, np.float32)  # noqa: E731
+    col = Collider(half_extents=(0.05, 0.05, 0.05), friction=0.5, mass=1.0, shape=0)
+    body = Body(x=(0.3, 0.36, 0.3), v=(0.05, -1.4, 0.0), w=(0.3, 0.0, -0.2), inv_mass=1.0, lin_damp=0.0,
+                ang_damp=0.0)
+    got, rec = o.step_coupled(method, col, body, (0.0, -9.81, 0.0), (0.0, 0.0, 0.0), 1e-4, 30)
+    b = body
+    touched = 0
+    for k in range(30):
+        f = f32(b.force) + f32((0.0, -9.81, 0.0))
+        t = f32(b.torque) + f32((0.0, 0.0, 0.0))
+        r.p2g()
+        r.update(1e-4)
+        c = Collider(center=b.x, quat=b.quat, half_extents=col.half_extents, lin_vel=b.v, ang_vel=b.w,
+                     friction=col.friction, mass=col.mass, shape=col.shape)
+        rf, rt, n = r.couple(method, c, 1e-4)
+        touched += n
+        b = Body(x=b.x, quat=b.quat, v=b.v, w=b.w, inv_mass=b.inv_mass, inv_I=b.inv_I, lin_damp=0.0, ang_damp=0.0,
+                 force=tuple(f + f32(rf)), torque=tuple(t + f32(rt)))
+        b = r.integrate_body(b, 1e-4)
+        r.g2p(1e-4)
+        want = np.concatenate([b.x, b.quat, b.v, b.w, rf, rt, [n]])
+        assert np.array_equal(rec[k], want), k
+    assert touched > 0, "the body never touched the soil"
+    po, pr = o.get_particles(), r.get_particles()
+    for k in "xvFC":
+        assert np.array_equal(po[k], pr[k]), k
+
+
+def test_coupled_loop_kinematic_is_dynamic_without_inverse_mass():
+    """A kinematic body moves at its own velocities: bit-identical to the
+    dynamic integration with zero inverse mass and inertia and no damping."""
+    runs = []
+    for kin in (True, False):
+        o, _ = _scene(5, n=300)
+        col = Collider(half_extents=(0.06, 0.04, 0.05), friction=0.4, mass=2.0, shape=1)
+        body = Body(x=(0.3, 0.35, 0.3), quat=(0.9, 0.1, 0.3, 0.2), v=(0.3, -0.2, 0.1), w=(0.0, 0.0, -3.0),
+                    inv_mass=1.0 if kin else 0.0, inv_I=np.eye(3) if kin else np.zeros((3, 3)), lin_damp=0.0,
+                    ang_damp=0.0)
+        runs.append(o.step_coupled(1, col, body, (0.0, -20.0, 0.0), (0.0, 0.0, 0.0), 1e-4, 20, kinematic=kin)[1])
+    assert np.array_equal(runs[0], runs[1])
+    assert runs[0][:, 19].max() > 0
