@@ -151,8 +151,11 @@ def tool_scene(seed, n_target, grid_edge=None, name="tool", depth=None):
     top = hi[1] * dx
     size = max(3, W // 6) * dx  # the tool spans a sixth of the bed
     sink = 2 * dx if depth is None else float(depth)
-    cx = (lo[0] + 0.25 * W) * dx
-    cz = (lo[2] + 0.5 * W) * dx
+    # a small off-grid shift keeps every tool face off the node planes at
+    # (i + 1/2) dx: a face on a node plane leaves that plane's nodes inside or
+    # outside by rounding alone
+    cx = (lo[0] + 0.25 * W + 0.0123) * dx
+    cz = (lo[2] + 0.5 * W + 0.0137) * dx
     if kind == "box":
         he = (0.5 * size, 0.5 * size, 0.5 * size)
         tool = {"kind": "box", "half": he, "center": (cx, top + 0.5 * size - sink, cz), "speed": speed,
@@ -212,6 +215,27 @@ def tool_body(tool):
     return Body(position=tool["center"], linear_velocity=(tool["speed"], 0.0, 0.0), inv_mass=1.0 / tool["mass"],
                 linear_damping=0.0, angular_damping=0.0, half_extents=tool["half"], friction=tool["friction"],
                 mass=tool["mass"], shape=SHAPE_BOX, integrate=BODY_KINEMATIC)
+
+
+def dynamic_body(tool, g=9.81):
+    """The scene's rigid body as the reference's scenarios build it: dynamic,
+    undamped, gravity as an applied force, inertia assigned directly so the
+    inverse inertia stays the identity (rigid_body.h:122,
+    scenario_rover_wheel.h:95-133, scenario_box_drop.h:60-67).  A wheel rolls:
+    local Y (axle) along world Z, v = the tool speed forward, w = -v/R."""
+    from basements_amd import SHAPE_BOX, SHAPE_CYLINDER, Body
+
+    if tool["kind"] == "cylinder":
+        m = tool["load"] / g
+        R = tool["radius"]
+        return Body(position=tool["center"], orientation=(math.cos(math.pi / 4), math.sin(math.pi / 4), 0.0, 0.0),
+                    linear_velocity=(tool["speed"], 0.0, 0.0), angular_velocity=(0.0, 0.0, -tool["speed"] / R),
+                    inv_mass=1.0 / m, linear_damping=0.0, angular_damping=0.0, applied_force=(0.0, -g * m, 0.0),
+                    half_extents=(R, tool["half_len"], R), friction=tool["friction"], mass=m, shape=SHAPE_CYLINDER)
+    m = tool.get("mass", 1.0)
+    return Body(position=tool["center"], inv_mass=1.0 / m, linear_damping=0.0, angular_damping=0.0,
+                applied_force=(0.0, -g * m, 0.0), half_extents=tool["half"], friction=tool["friction"], mass=m,
+                shape=SHAPE_BOX)
 
 
 def tine_scene(seed=0, cells=(32, 4, 32), velocity=None):

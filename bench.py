@@ -114,24 +114,10 @@ def copy_calibration():
 
 
 def make_body(tool, g=9.81):
-    """The scene's rigid body as the reference's scenarios build it: dynamic,
-    undamped, gravity as an applied force, inertia assigned directly so the
-    inverse inertia stays the identity (rigid_body.h:122,
-    scenario_rover_wheel.h:95-133, scenario_box_drop.h:60-67).  The wheel
-    rolls: local Y (axle) along world Z, v = 0.2 m/s forward, w = -v/R."""
-    from basements_amd import SHAPE_BOX, SHAPE_CYLINDER, Body
+    """The scene's dynamic body (basements_amd.scenes.dynamic_body)."""
+    from basements_amd.scenes import dynamic_body
 
-    if tool["kind"] == "cylinder":
-        m = tool["load"] / g
-        R = tool["radius"]
-        return Body(position=tool["center"], orientation=(math.cos(math.pi / 4), math.sin(math.pi / 4), 0.0, 0.0),
-                    linear_velocity=(tool["speed"], 0.0, 0.0), angular_velocity=(0.0, 0.0, -tool["speed"] / R),
-                    inv_mass=1.0 / m, linear_damping=0.0, angular_damping=0.0, applied_force=(0.0, -g * m, 0.0),
-                    half_extents=(R, tool["half_len"], R), friction=tool["friction"], mass=m, shape=SHAPE_CYLINDER)
-    m = tool.get("mass", 1.0)
-    return Body(position=tool["center"], inv_mass=1.0 / m, linear_damping=0.0, angular_damping=0.0,
-                applied_force=(0.0, -g * m, 0.0), half_extents=tool["half"], friction=tool["friction"], mass=m,
-                shape=SHAPE_BOX)
+    return dynamic_body(tool, g)
 
 
 def cpu_baseline(scene, nsteps):

@@ -62,3 +62,58 @@ def rel_err(got, want):
     got = np.asarray(got, np.float64)
     want = np.asarray(want, np.float64)
     return float(np.abs(got - want).max() / max(np.abs(want).max(), 1e-300))
+
+
+def _r32(t):
+    """Values as the device holds them (FP32), handed to the oracle in double."""
+    return tuple(float(np.float32(v)) for v in np.ravel(t))
+
+
+def oracle_coupled_run(P, dx, prm: Params, body, method, dt, n, precision):
+    """The oracle's coupled loop (contract order, OracleSim.step_coupled) from
+    the same particles and device body description (basements_amd.Body, its
+    values rounded to FP32 as the device holds them).  Returns (particles,
+    per-substep record)."""
+    from oracle.pyoracle import Body as OBody
+
+    if body.integrate not in (1, 2):
+        raise ValueError("the oracle's coupled loop integrates dynamic or kinematic bodies")
+    col = OCollider(half_extents=_r32(body.half_extents), friction=_r32([body.friction])[0],
+                    mass=_r32([body.mass])[0], shape=int(body.shape))
+    ob = OBody(x=_r32(body.position), quat=_r32(body.orientation), v=_r32(body.linear_velocity),
+               w=_r32(body.angular_velocity), inv_mass=_r32([body.inv_mass])[0],
+               inv_I=np.array(_r32(body.inv_inertia)).reshape(3, 3), lin_damp=_r32([body.linear_damping])[0],
+               ang_damp=_r32([body.angular_damping])[0], force=_r32(body.force), torque=_r32(body.torque))
+    o = OracleSim(dx, precision=precision, params=prm)
+    o.set_particles({k: np.asarray(v, np.float64) for k, v in P.items()})
+    _, rec = o.step_coupled(method, col, ob, _r32(body.applied_force), _r32(body.applied_torque), dt, n,
+                            kinematic=body.integrate == 2)
+    return o.get_particles(), rec
+
+
+def oracle_pairs(jobs, workers=16):
+    """Run oracle_coupled_run for every job (a dict of its keyword arguments
+    without `precision`) in FP32 and FP64, in parallel threads (the oracle's
+    calls release the interpreter lock).  Returns {(job index, precision): result}."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = {}
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = {(i, p): ex.submit(oracle_coupled_run, precision=p, **job) for i, job in enumerate(jobs)
+                for p in ("f32", "f64")}
+        for k, f in futs.items():
+            out[k] = f.result()
+    return out
+
+
+def assert_at_noise_floor(got, P32, P64, what, fields="xvFC", factor=2.5, slack=1e-6):
+    """Device vs the FP64 oracle within `factor` x the FP32 oracle's own
+    distance from it (plus `slack`), per field relative to its largest
+    magnitude.  Returns {field: (device error, FP32 floor)}."""
+    rep = {}
+    for k in fields:
+        floor = rel_err(P32[k], P64[k])
+        err = rel_err(got[k], P64[k])
+        rep[k] = (err, floor)
+        assert err <= factor * floor + slack, (what, k, err, floor)
+    return rep
