@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -54,16 +55,20 @@ def _run(cmd):
 
 def build_library(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
     tmpdir = os.path.join(LIBDIR, "obj")
     os.makedirs(tmpdir, exist_ok=True)
+    cmds, objs = [], []
     for src, extra in UNITS:
         obj = os.path.join(tmpdir, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *COMMON, *extra, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([HIPCC, *COMMON, *extra, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src),
+                     "-o", obj])
         objs.append(obj)
+        if verbose:
+            print(" ".join(cmds[-1]))
+    # translation units compile independently: a few at a time
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "4")), os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -747,19 +747,22 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
 // velocity's adjoint from the M0 nodes.
 template <class R>
 __global__ void __launch_bounds__(256) k_update_adj(Box b, Par<R> P, R dt, R* gm, R* gq, R* gub,
-                                                    R* gqb, R* gmb, double* grad) {
+                                                    R* gqb, R* gmb, double* grad, int clear_mq) {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nn = (int64_t)b.nx * b.ny * b.nz;
     v3<R> vtb = mk(R(0), R(0), R(0));
     if (id < nn) {
         // the substep's last reader of (m, q) and of the velocity adjoint:
         // each thread reads its own node and leaves it cleared for the next
-        // P2G / G2P adjoint that accumulates into it
+        // P2G / G2P adjoint that accumulates into it; (m, q) that the forward
+        // kept (clear_mq = 0) stay for a later backward
         const R m = gm[id];
         const v3<R> q = mk(gq[3 * id], gq[3 * id + 1], gq[3 * id + 2]);
         const v3<R> u2b_in = mk(gub[3 * id], gub[3 * id + 1], gub[3 * id + 2]);
-        gm[id] = R(0);
-        gq[3 * id] = gq[3 * id + 1] = gq[3 * id + 2] = R(0);
+        if (clear_mq) {
+            gm[id] = R(0);
+            gq[3 * id] = gq[3 * id + 1] = gq[3 * id + 2] = R(0);
+        }
         gub[3 * id] = gub[3 * id + 1] = gub[3 * id + 2] = R(0);
         gqb[3 * id] = gqb[3 * id + 1] = gqb[3 * id + 2] = R(0);
         gmb[id] = R(0);
@@ -899,9 +902,52 @@ __global__ void __launch_bounds__(256) k_p2g_adj(PS<R> s, const R* m, const R* V
     wave_add(grad + GLAM, glam);
 }
 
-template <class R> __global__ void k_zero(R* p, int64_t n) {
+// The two per-particle SVD maps of the rollout and their vector-Jacobian
+// products on their own, for checking the adjoint in isolation: map 0 the
+// Drucker-Prager return map F -> U h(sigma) V^T, map 1 the first
+// Piola-Kirchhoff stress P(F) = U diag(tau_k / sigma_k) V^T.
+template <class R>
+__global__ void __launch_bounds__(64) k_map_vjp(int map, int64_t n, const double* F, const double* Gb, Par<R> P,
+                                                double* G, double* Fb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = R(0);
+    if (i >= n) return;
+    m3<R> Fm, Gbar;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        Fm.a[c] = (R)F[9 * i + c];
+        Gbar.a[c] = (R)Gb[9 * i + c];
+    }
+    m3<R> U, V;
+    v3<R> s;
+    svd_d(Fm, U, s, V, 0);
+    v3<R> h;
+    R J[9];
+    if (map == 0) {
+        const DPh<R> d = dp_h(s, P.alpha, P.koh);
+        h = d.h;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) J[c] = d.J[c];
+    } else {
+        const Hh<R> d = hencky_h(s, P.mu, P.lam);
+        h = d.h;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) J[c] = d.J[c];
+    }
+    const m3<R> Gm = usv(U, h, V);
+    m3<R> M;
+    const m3<R> Fbar = usv_adjoint(U, s, V, h, J, Gbar, M);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        G[9 * i + c] = (double)Gm.a[c];
+        Fb[9 * i + c] = (double)Fbar.a[c];
+    }
+}
+
+// Grid-stride: the kept grids of a long rollout exceed the 2^32 threads a
+// launch may have.
+template <class R> __global__ void k_zero(R* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = R(0);
 }
 
 }  // namespace dif
@@ -913,6 +959,8 @@ using namespace bm::dif;
 namespace {
 
 inline int blocks_for(int64_t n) { return (int)((n + 255) / 256); }
+// Workgroups of a grid-stride fill (k_zero) of n values.
+inline int fill_blocks(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 1 << 16); }
 // Particle kernels run 64 lanes per workgroup: a rollout of a few 10^4
 // particles then still reaches every CU (each lane's work -- SVD, stress,
 // 27-node scatter -- is long, so the launch is latency bound).
@@ -1029,8 +1077,8 @@ template <class R> static void seed_stress(bm_diff_handle h, PS<R> st, int step)
 // them zero for the next one (k_update or k_update_adj clears each node after
 // its last read).
 template <class R> static void clear_mass_momentum(bm_diff_handle h) {
-    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
-    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
+    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
+    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
 }
 
 // One forward substep from state `in` to state `out` (grid velocity left in
@@ -1059,6 +1107,13 @@ static int ensure(void** p, size_t bytes) {
     return BM_OK;
 }
 
+static void release_kept_grids(bm_diff_handle h) {
+    if (h->agmq) hipFree(h->agmq);
+    if (h->agu) hipFree(h->agu);
+    h->agmq = h->agu = nullptr;
+    h->all_slots = 0;
+}
+
 static int read_err(bm_diff_handle h) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     int e = 0;
@@ -1084,16 +1139,29 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
     }
     HIP_TRY(hipMemcpyAsync(h->ck, h->init, h->rs * QCOMP * h->cap, hipMemcpyDeviceToDevice, h->stream));
     PS<R> cur = slot<R>(h->ck, h->cap, 0);
-    h->all_grids = (int64_t)substeps * 7 * h->nn * (int64_t)h->rs <= h->grid_budget;
+    const int64_t kept_bytes = (int64_t)substeps * 7 * h->nn * (int64_t)h->rs;
+    h->all_grids = kept_bytes <= h->grid_budget;
     if (h->all_grids && h->all_slots < substeps) {
-        // an allocation that fails leaves the per-segment grids in use
-        h->all_slots = 0;
-        h->all_grids = ensure(&h->agmq, h->rs * 4 * h->nn * substeps) == BM_OK &&
-                       ensure(&h->agu, h->rs * 3 * h->nn * substeps) == BM_OK;
-        if (h->all_grids) h->all_slots = substeps;
+        // keep the grids only where they fit the device's free memory with
+        // room to spare; an allocation that fails anyway leaves the
+        // per-segment grids in use
+        size_t free_b = 0, total_b = 0;
+        h->all_grids = hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+                       kept_bytes + (int64_t(1) << 30) <= (int64_t)free_b;
+        if (h->all_slots > 0 || !h->all_grids) release_kept_grids(h);
+        if (h->all_grids) {
+            h->all_grids = ensure(&h->agmq, h->rs * 4 * h->nn * substeps) == BM_OK &&
+                           ensure(&h->agu, h->rs * 3 * h->nn * substeps) == BM_OK;
+            if (h->all_grids) {
+                h->all_slots = substeps;
+            } else {
+                (void)hipGetLastError();  // the failed allocation's error is handled here
+                release_kept_grids(h);
+            }
+        }
     }
     if (h->all_grids) {
-        hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(4 * h->nn * substeps)), dim3(256), 0, h->stream,
+        hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(4 * h->nn * substeps)), dim3(256), 0, h->stream,
                            static_cast<R*>(h->agmq), 4 * h->nn * substeps);
     } else {
         clear_mass_momentum<R>(h);
@@ -1125,6 +1193,15 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
     return BM_OK;
 }
 
+// The adjoint of the loss with respect to the final state, into the first
+// adjoint slot.  Every backward starts from it (the reverse walk overwrites
+// the slots), so backward can be called again after one forward.
+template <class R> static void seed_loss_adjoint(bm_diff_handle h) {
+    const int64_t last = (h->substeps + h->K - 1) / h->K;
+    hipLaunchKernelGGL(k_loss<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, slot<R>(h->ck, h->cap, last),
+                       (const R*)h->x0, (const R*)h->tgt, h->n, slot<R>(h->abar, h->cap, 0), h->grad);
+}
+
 // The backward over the grids the forward kept: each segment's particle
 // states are recomputed from its checkpoint with the kept velocities, then
 // walked in reverse.
@@ -1136,7 +1213,8 @@ template <class R> static int backward_kept(bm_diff_handle h, double* g) {
     R* const gmq = static_cast<R*>(h->agmq);
     R* const gus = static_cast<R*>(h->agu);
     HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
-    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * nn)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn);
+    seed_loss_adjoint<R>(h);
+    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * nn)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn);
     PS<R> a = slot<R>(h->abar, h->cap, 0), a2 = slot<R>(h->abar2, h->cap, 0);
     const R dt = (R)h->dt;
     for (int sg = nseg - 1; sg >= 0; --sg) {
@@ -1155,7 +1233,7 @@ template <class R> static int backward_kept(bm_diff_handle h, double* g) {
             hipLaunchKernelGGL(k_g2p_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, a, a2, h->n,
                                h->box, P, dt, (const R*)gu, (R*)h->gub, h->grad);
             hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
-                               gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad);
+                               gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad, 0);
             hipLaunchKernelGGL(k_p2g_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
                                (const R*)h->V0, h->n, h->box, P, dt, (const R*)h->gqb, (const R*)h->gmb, a2, h->grad);
             std::swap(a, a2);
@@ -1181,11 +1259,12 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
     R* const gmq = static_cast<R*>(h->sgmq);
     R* const gus = static_cast<R*>(h->sgu);
     HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
+    seed_loss_adjoint<R>(h);
     // the slots' (m, q) and the velocity adjoint start at zero; the update's
     // adjoint clears each after its last read, so they stay zero between uses
     for (int j = 0; j < K; ++j)
-        hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(4 * nn)), dim3(256), 0, h->stream, gmq + 4 * nn * j, 4 * nn);
-    hipLaunchKernelGGL(k_zero<R>, dim3(blocks_for(3 * nn)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn);
+        hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(4 * nn)), dim3(256), 0, h->stream, gmq + 4 * nn * j, 4 * nn);
+    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * nn)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn);
     PS<R> a = slot<R>(h->abar, h->cap, 0), a2 = slot<R>(h->abar2, h->cap, 0);
     const R dt = (R)h->dt;
     for (int sg = nseg - 1; sg >= 0; --sg) {
@@ -1219,7 +1298,7 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
             hipLaunchKernelGGL(k_g2p_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, a, a2, h->n,
                                h->box, P, dt, (const R*)gu, (R*)h->gub, h->grad);
             hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
-                               gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad);
+                               gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad, 1);
             hipLaunchKernelGGL(k_p2g_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
                                (const R*)h->V0, h->n, h->box, P, dt, (const R*)h->gqb, (const R*)h->gmb, a2, h->grad);
             std::swap(a, a2);
@@ -1443,6 +1522,46 @@ int bm_diff_backward(bm_diff_handle h, bm_diff_grad* out) {
     out->mu = gmu;
     out->lambda = glam;
     return BM_OK;
+}
+
+int bm_diff_map_vjp(int32_t map, int64_t n, const double* F, const double* Gbar, const bm_diff_params* params,
+                    int32_t precision, double* G, double* Fbar) {
+    if (map != 0 && map != 1) return fail(BM_ERR_INVALID_ARG, "map must be 0 (return map) or 1 (stress)");
+    if (precision != 32 && precision != 64) return fail(BM_ERR_INVALID_ARG, "precision must be 32 or 64");
+    if (n < 0 || (n > 0 && (!F || !Gbar || !params || !G || !Fbar))) return fail(BM_ERR_INVALID_ARG, "null array");
+    if (n == 0) return BM_OK;
+    bm_diff tmp;
+    tmp.prm = *params;
+    tmp.desc.dx = 1.0f;
+    const size_t bytes = sizeof(double) * 9 * (size_t)n;
+    double *dF = nullptr, *dGb = nullptr, *dG = nullptr, *dFb = nullptr;
+    int rc = BM_OK;
+    if (hipMalloc(&dF, bytes) != hipSuccess || hipMalloc(&dGb, bytes) != hipSuccess ||
+        hipMalloc(&dG, bytes) != hipSuccess || hipMalloc(&dFb, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(BM_ERR_OOM, "map_vjp buffers");
+    }
+    if (rc == BM_OK && (hipMemcpy(dF, F, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(dGb, Gbar, bytes, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(BM_ERR_HIP, "map_vjp upload");
+    if (rc == BM_OK) {
+        const dim3 grid((unsigned)((n + 63) / 64));
+        if (precision == 64)
+            hipLaunchKernelGGL(k_map_vjp<double>, grid, dim3(64), 0, nullptr, map, n, (const double*)dF,
+                               (const double*)dGb, params_at<double>(&tmp, 0), dG, dFb);
+        else
+            hipLaunchKernelGGL(k_map_vjp<float>, grid, dim3(64), 0, nullptr, map, n, (const double*)dF,
+                               (const double*)dGb, params_at<float>(&tmp, 0), dG, dFb);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(G, dG, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(Fbar, dFb, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(BM_ERR_HIP, "map_vjp kernel or download");
+    }
+    hipFree(dF);
+    hipFree(dGb);
+    hipFree(dG);
+    hipFree(dFb);
+    return rc;
 }
 
 int bm_diff_download(bm_diff_handle h, bm_particles* out) {

@@ -273,6 +273,8 @@ _SIGS = {
     "bm_diff_forward": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_double)]),
     "bm_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(DiffGradC)]),
     "bm_diff_download": (C.c_int, [C.c_void_p, C.POINTER(HostParticles)]),
+    "bm_diff_map_vjp": (C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(DiffParamsC), C.c_int32,
+                                  C.c_void_p, C.c_void_p]),
     "bm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "bm_mpm_download_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -987,3 +989,20 @@ class Rollout:
         hp = _host_particles(out)[0]
         _check(lib().bm_diff_download(self.h, C.byref(hp)))
         return out
+
+
+MAP_RETURN = 0  # bm_diff_map_vjp: the Drucker-Prager return map F -> U h(sigma) V^T
+MAP_STRESS = 1  # the first Piola-Kirchhoff stress P(F) of the Hencky model
+
+
+def map_vjp(which, F, Gbar, params: DiffParams, precision=64):
+    """(G, Fbar) of one of the rollout's per-particle SVD maps (bm_diff_map_vjp):
+    G = map(F) and Fbar = Gbar . dG/dF for arrays of 3x3 matrices."""
+    F = np.ascontiguousarray(np.asarray(F, np.float64).reshape(-1, 9))
+    Gb = np.ascontiguousarray(np.asarray(Gbar, np.float64).reshape(-1, 9))
+    n = F.shape[0]
+    G = np.zeros((n, 9))
+    Fb = np.zeros((n, 9))
+    pc = params.to_c()
+    _check(lib().bm_diff_map_vjp(int(which), n, _ptr(F), _ptr(Gb), C.byref(pc), int(precision), _ptr(G), _ptr(Fb)))
+    return G.reshape(-1, 3, 3), Fb.reshape(-1, 3, 3)

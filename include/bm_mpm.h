@@ -489,10 +489,23 @@ int bm_diff_set_grid_budget(bm_diff_handle h, int64_t bytes);
 int bm_diff_upload(bm_diff_handle h, int64_t n, const bm_particles* p, const float* target_displacement);
 /* Run `substeps` substeps, saving a state every checkpoint_every; the loss. */
 int bm_diff_forward(bm_diff_handle h, float dt, int32_t substeps, double* loss);
-/* Gradient of the last forward's loss. */
+/* Gradient of the last forward's loss; may be called again (the same
+ * gradient) until the next forward or parameter change. */
 int bm_diff_backward(bm_diff_handle h, bm_diff_grad* grad);
 /* Final state of the last forward (x, v, F, C; m and V0 are not written). */
 int bm_diff_download(bm_diff_handle h, bm_particles* p);
+/* The rollout's two per-particle SVD maps and their hand-written
+ * vector-Jacobian products on their own, for checking the adjoint against an
+ * independent differentiation: map 0 = the Drucker-Prager return map
+ * F -> U h(sigma) V^T (mpm_solver.h:114-166), map 1 = the first
+ * Piola-Kirchhoff stress P(F) = U diag(tau_k / sigma_k) V^T of the
+ * Hencky-Kirchhoff model (mpm_solver.h:186-217).  n row-major 3x3 matrices
+ * F and cotangents Gbar in; G = map(F) and Fbar = Gbar . dG/dF out (9n doubles
+ * each).  params supplies E, nu and the friction angle; precision 32 or 64 is
+ * the arithmetic of the kernel.  No counterpart in the reference (its tape
+ * records nothing). */
+int bm_diff_map_vjp(int32_t map, int64_t n, const double* F, const double* Gbar, const bm_diff_params* params,
+                    int32_t precision, double* G, double* Fbar);
 
 #ifdef __cplusplus
 }
