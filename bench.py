@@ -12,7 +12,12 @@ integration, G2P with plasticity -- all on the device, no host round trip.
 Prints ONE JSON line: particle-updates/s (value), the roofline of the
 dominant kernel (HIP events on the handle's stream), and a CPU baseline (the
 reference's own MPMSolver::step built from its sources, one thread, on a
-bounded sample of the same bed).
+bounded sample of the same bed).  Beside them the same line carries the
+batched and served forms: `batched` -- 64 uneven tool scenes (2e4-5e5
+particles) stepped together against the same scenes one at a time on the
+plain path; `served` -- 48 soil-tool requests (5e4-2e6 particles, 200-2000
+substeps) arriving at 200 requests/s through the request loop, with p50 /
+p99 latency, against the same requests one at a time.
 """
 from __future__ import annotations
 
@@ -67,6 +72,9 @@ KERNEL_OF = {
     ("reference", "g2p"): "k_g2p_ref",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+REBIN_PERIOD = 32   # a single handle's default re-binning period (substeps)
+BATCH_STEPS = 64    # the default line's batched measurement: two re-binning cycles of 64 scenes
+SERVE_RATE = 200.0  # the default line's served measurement: a saturating arrival rate (requests/s)
 
 
 def _args():
@@ -94,6 +102,10 @@ def _args():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--batched", type=int, default=1,
+                    help="default workload: also measure 64 batched scenes against the same scenes one at a time")
+    ap.add_argument("--served", type=int, default=1,
+                    help="default workload: also measure 48 served requests at a saturating arrival rate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (for HBM counter scaling)")
     return ap.parse_args()
@@ -132,20 +144,40 @@ def cpu_baseline(scene, nsteps):
     P = {k: np.asarray(v[sel], np.float64) for k, v in scene.particles.items()}
     prm = Params(E=scene.E, nu=scene.nu, rho=scene.rho, alpha=scene.alpha, mu_floor=scene.mu_floor,
                  gravity=scene.gravity)
+    note = None
     if ref_available():
         sim, kind = RefSim(scene.dx, params=prm), "reference"
     else:
         sim, kind = OracleSim(scene.dx, "f32", params=prm), "port"
+        note = ("oracle/_ref (the reference's own sources, built where the reference tree is mounted) is absent "
+                "on this host: the oracle's FP32 restatement, pinned bit-exact to it, is timed instead")
     sim.set_particles(P)
     sim.step(scene.dt, 1)  # warm-up: allocates the sparse blocks
     t0 = time.perf_counter()
     sim.step(scene.dt, nsteps)
     t = time.perf_counter() - t0
     n = int(P["x"].shape[0])
-    return {"value": n * nsteps / t, "unit": "particle-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{n} particles (the quarter of the bed with the lowest x), {nsteps} substeps of "
-                      f"MPMSolver::step after 1 warm-up, dt {scene.dt}, single thread",
-            "seconds": t}
+    out = {"value": n * nsteps / t, "unit": "particle-updates/s", "cores": 1, "kind": kind,
+           "sample": f"{n} particles (the quarter of the bed with the lowest x), {nsteps} substeps of "
+                     f"MPMSolver::step after 1 warm-up, dt {scene.dt}, single thread",
+           "seconds": t, "host_cpu": host_cpu_model()}
+    if note:
+        out["note"] = note
+    return out
+
+
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model or 'unknown'} ({os.cpu_count()} logical CPUs visible)"
 
 
 def measured_traffic(workload, mode, kernel):
@@ -177,7 +209,7 @@ def _percentile(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q)) if len(xs) else None
 
 
-def bench_serve(a):
+def run_serve(seed, n_requests, rates, cap_p, cap_b, serial=True):
     """Served soil-tool rollouts: a Poisson stream of requests (particle
     counts log-uniform in [5e4, 2e6], tool depth U(0.02, 0.1) m, speed
     U(0.05, 0.5) m/s, substeps uniform in {200..2000}) through the request
@@ -189,10 +221,10 @@ def bench_serve(a):
     from basements_amd import MODE_FAST, M0, M1, REQ_DONE, Server, Solver
     from basements_amd.scenes import poisson_arrivals, serve_requests, tool_body
 
-    reqs = serve_requests(a.seed, a.requests)
+    reqs = serve_requests(seed, n_requests)
     dt = 1e-4
     total_pu = float(sum(s.n * k for s, k in reqs))
-    cap_p, cap_b, cap_s = a.serve_particles, a.serve_blocks, 64
+    cap_s = 64
 
     def run(rate):
         S = Server(max_particles=cap_p, max_blocks=cap_b, max_scenes=cap_s, dt=dt)
@@ -200,7 +232,7 @@ def bench_serve(a):
         w = [S.submit(_scene_desc(s), s.particles, 16, body=tool_body(s.tool)) for s, _ in reqs[:2]]
         while any(S.info(r).state != REQ_DONE for r in w):
             S.poll()
-        arr = poisson_arrivals(rate, len(reqs), a.seed)
+        arr = poisson_arrivals(rate, len(reqs), seed)
         t0 = S.clock() + 0.01
         ids = [S.submit(_scene_desc(s), s.particles, k, body=tool_body(s.tool), arrival=t0 + float(t))
                for (s, k), t in zip(reqs, arr)]
@@ -222,12 +254,12 @@ def bench_serve(a):
                 "max_ms": max(lat) * 1e3, "makespan_s": makespan, "value": total_pu / makespan,
                 "most_running": most, "server_time": st}
 
-    per_rate = [run(float(r)) for r in a.rates.split(",")]
+    per_rate = [run(float(r)) for r in rates]
     top = max(per_rate, key=lambda r: r["rate"])
 
     # the same requests one at a time on the plain path
     t_serial = 0.0
-    if a.serial:
+    if serial:
         for s, k in reqs:
             S = Solver(s.dx, s.origin, s.dims, s.n, MODE_FAST)
             S.set_material(s.E, s.nu, s.rho)
@@ -243,19 +275,27 @@ def bench_serve(a):
             S.download()
             t_serial += time.perf_counter() - t0
             S.close()
-    serial = total_pu / t_serial if t_serial > 0 else None
+    serial_v = total_pu / t_serial if t_serial > 0 else None
+    return {"reqs": reqs, "total_pu": total_pu, "per_rate": per_rate, "top": top, "serial_value": serial_v,
+            "serial_s": t_serial if serial_v else None,
+            "config": {"workload": "serve", "requests": len(reqs), "particles": int(sum(s.n for s, _ in reqs)),
+                       "particle_updates": total_pu, "particles_min": min(s.n for s, _ in reqs),
+                       "particles_max": max(s.n for s, _ in reqs), "substeps_min": min(k for _, k in reqs),
+                       "substeps_max": max(k for _, k in reqs), "dt": dt, "server_particles": cap_p,
+                       "server_blocks": cap_b, "server_scenes": cap_s}}
+
+
+def bench_serve(a):
+    r = run_serve(a.seed, a.requests, [float(x) for x in a.rates.split(",")], a.serve_particles, a.serve_blocks,
+                  bool(a.serial))
+    top, serial = r["top"], r["serial_value"]
     out = {"metric": "particle-updates/s (served requests, aggregate)", "value": top["value"],
-           "unit": "particle-updates/s", "n_gpus": 1, "steps": len(reqs), "warmup": 2,
-           "ms_per_step": top["makespan_s"] / len(reqs) * 1e3, "higher_is_better": True, "scaling": "weak",
+           "unit": "particle-updates/s", "n_gpus": 1, "steps": len(r["reqs"]), "warmup": 2,
+           "ms_per_step": top["makespan_s"] / len(r["reqs"]) * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "fp32",
            "data": "synthetic (seeded soil-tool requests, Poisson arrivals, no external data)",
-           "config": {"workload": "serve", "requests": len(reqs), "particles": int(sum(s.n for s, _ in reqs)),
-                      "particle_updates": total_pu, "particles_min": min(s.n for s, _ in reqs),
-                      "particles_max": max(s.n for s, _ in reqs), "substeps_min": min(k for _, k in reqs),
-                      "substeps_max": max(k for _, k in reqs), "dt": dt, "server_particles": cap_p,
-                      "server_blocks": cap_b, "server_scenes": cap_s},
-           "rates": per_rate, "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
-           "serial_value": serial, "serial_s": t_serial if serial else None,
+           "config": r["config"], "rates": r["per_rate"], "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
+           "serial_value": serial, "serial_s": r["serial_s"],
            "speedup_vs_serial": top["value"] / serial if serial else None}
     print(json.dumps(out))
     return 0
@@ -373,43 +413,39 @@ def bench_adjoint(a):
     return 0
 
 
-def bench_batch(a):
+def run_batch(seed, n_scenes, steps, warmup, graph=True, tuning="", serial=True):
     """Batched uneven scenes: aggregate particle-updates/s of all scenes in one
     batch (autotuned, graph-replayed) against running the same scenes one at a
     time, each on its own single-scene handle, for the same substeps."""
-    from basements_amd import M0, M1, MODE_FAST, Batch, SceneDesc, Solver
+    from basements_amd import M0, M1, MODE_FAST, Batch, Solver
     from basements_amd.scenes import batch_scenes, tool_body
 
-    scenes = batch_scenes(a.seed, a.scenes)
+    scenes = batch_scenes(seed, n_scenes)
     dt = 1e-4
     n_total = sum(s.n for s in scenes)
     blocks = sum((-(-s.dims[0] // 4)) * (-(-s.dims[1] // 4)) * (-(-s.dims[2] // 4)) for s in scenes)
 
-    def desc(s):
-        return SceneDesc(dx=s.dx, origin=s.origin, dims=s.dims, E=s.E, nu=s.nu, rho=s.rho, alpha=s.alpha,
-                         mu_floor=s.mu_floor, gravity=s.gravity, coupler=M1 if s.tool["method"] == 1 else M0)
-
     B = Batch(max_particles=n_total + 64, max_blocks=blocks, max_scenes=len(scenes))
-    ids = [B.add_scene(desc(s), s.particles, tool_body(s.tool)) for s in scenes]
-    if a.tuning:
-        unit, sort_every, groups = (int(t) for t in a.tuning.split(","))
+    ids = [B.add_scene(_scene_desc(s), s.particles, tool_body(s.tool)) for s in scenes]
+    if tuning:
+        unit, sort_every, groups = (int(t) for t in tuning.split(","))
         B.set_tuning(unit, sort_every, groups)
     else:
         unit, sort_every, groups = B.autotune(dt, substeps=64)
-    B.set_graph(bool(a.graph))
+    B.set_graph(bool(graph))
     # warm-up long enough to capture the replayed substeps (two re-binning cycles)
-    warmup = max(a.warmup, 3 * sort_every)
+    warmup = max(warmup, 3 * sort_every)
     B.step(dt, warmup)
     B.synchronize()
     t0 = time.perf_counter()
-    B.step(dt, a.steps)
+    B.step(dt, steps)
     B.synchronize()
     t_batch = time.perf_counter() - t0
     del B
 
     # the same scenes one at a time on the plain path
     t_serial = 0.0
-    for s in scenes if a.serial else []:
+    for s in scenes if serial else []:
         S = Solver(s.dx, s.origin, s.dims, s.n, MODE_FAST)
         S.set_material(s.E, s.nu, s.rho)
         S.set_plastic_alpha(s.alpha)
@@ -418,26 +454,33 @@ def bench_batch(a):
         S.upload(s.particles)
         S.set_body(tool_body(s.tool))
         method = M1 if s.tool["method"] == 1 else M0
-        S.step_coupled(method, dt, a.warmup)
+        S.step_coupled(method, dt, warmup)
         S.synchronize()
         t0 = time.perf_counter()
-        S.step_coupled(method, dt, a.steps)
+        S.step_coupled(method, dt, steps)
         S.synchronize()
         t_serial += time.perf_counter() - t0
         S.close()
-    value = n_total * a.steps / t_batch
-    serial = n_total * a.steps / t_serial if t_serial > 0 else None
-    out = {"metric": "particle-updates/s (batched scenes, aggregate)", "value": value,
-           "unit": "particle-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": warmup,
-           "ms_per_step": t_batch / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+    value = n_total * steps / t_batch
+    serial_v = n_total * steps / t_serial if t_serial > 0 else None
+    return {"value": value, "steps": steps, "warmup": warmup, "ms_per_step": t_batch / steps * 1e3,
+            "serial_value": serial_v, "serial_ms_per_step": t_serial / steps * 1e3 if serial_v else None,
+            "speedup_vs_serial": value / serial_v if serial_v else None,
+            "config": {"workload": "batch", "scenes": len(scenes), "particles": n_total, "grid_blocks": blocks,
+                       "particles_min": min(s.n for s in scenes), "particles_max": max(s.n for s in scenes),
+                       "grids": sorted({s.dims[0] for s in scenes}), "dt": dt, "graph": bool(graph),
+                       "tuned_unit_particles": unit, "tuned_sort_every": sort_every,
+                       "tuned_deposit_groups": groups}}
+
+
+def bench_batch(a):
+    r = run_batch(a.seed, a.scenes, a.steps, a.warmup, bool(a.graph), a.tuning, bool(a.serial))
+    out = {"metric": "particle-updates/s (batched scenes, aggregate)", "value": r["value"],
+           "unit": "particle-updates/s", "n_gpus": 1, "steps": r["steps"], "warmup": r["warmup"],
+           "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "fp32", "data": "synthetic (seeded soil beds with dragged tools, no external data)",
-           "config": {"workload": "batch", "scenes": len(scenes), "particles": n_total, "grid_blocks": blocks,
-                      "particles_min": min(s.n for s in scenes), "particles_max": max(s.n for s in scenes),
-                      "grids": sorted({s.dims[0] for s in scenes}), "dt": dt, "graph": bool(a.graph),
-                      "tuned_unit_particles": unit, "tuned_sort_every": sort_every,
-                      "tuned_deposit_groups": groups},
-           "serial_value": serial, "serial_ms_per_step": t_serial / a.steps * 1e3 if serial else None,
-           "speedup_vs_serial": value / serial if serial else None}
+           "config": r["config"], "serial_value": r["serial_value"], "serial_ms_per_step": r["serial_ms_per_step"],
+           "speedup_vs_serial": r["speedup_vs_serial"]}
     print(json.dumps(out))
     return 0
 
@@ -483,11 +526,17 @@ def main():
     # body and its integration, g2p (bm_mpm_step_coupled): no host round trip.
     S.set_body(make_body(tool))
 
-    S.step_coupled(method, dt, a.warmup)
+    # Fast mode re-bins the particles every REBIN_PERIOD substeps.  The
+    # warm-up is rounded up to whole periods, so the timed window starts with
+    # a re-binning and holds ceil(K / period) of them -- at least its share
+    # of the steady state, whatever K is.
+    fast = mode == MODE_FAST
+    period = (int(a.tuning.split(",")[1]) if a.tuning and mode == MODE_FAST else REBIN_PERIOD) if fast else 0
+    warm = -(-a.warmup // period) * period if fast else a.warmup
+    S.step_coupled(method, dt, warm)
     S.synchronize()
     # Timed region: the G2P and P2G kernels carry HIP events recorded by their
     # own dispatches (fast mode), nothing is recorded between launches.
-    fast = mode == MODE_FAST
     S.set_timing(True, kernels_only=fast)
     S.read_timing(reset=True)
     t0 = time.perf_counter()
@@ -551,7 +600,9 @@ def main():
         "data": "synthetic (seeded jittered soil bed, no external data)",
         "config": {"workload": scene.name, "particles": n, "grid": list(scene.dims), "dx": scene.dx,
                    "dt": dt, "coupler": "M1" if method == M1 else "M0", "mode": a.mode,
-                   "active_nodes": n_nodes, "active_blocks": st["active_blocks"]},
+                   "active_nodes": n_nodes, "active_blocks": st["active_blocks"],
+                   "rebin_period": period or None, "rebins_in_window": -(-a.steps // period) if fast else None,
+                   "warmup_rounded_to_period": warm},
         "roofline": {"bound": "hbm", "kernel": per_kernel[dom]["kernel"], "phase": dom, "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None,
@@ -568,7 +619,26 @@ def main():
     }
     if a.cpu_baseline:
         cb = cpu_baseline(scene, a.cpu_steps)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "host_cpu", "note")
+                               if k in cb}
+    S.close()
+    del S
+    if a.batched and a.config == "wheel_bed":
+        r = run_batch(a.seed, 64, BATCH_STEPS, 10)
+        out["batched"] = {"metric": "particle-updates/s (64 batched scenes, aggregate)", "value": r["value"],
+                          "serial_value": r["serial_value"], "speedup_vs_serial": r["speedup_vs_serial"],
+                          "steps": r["steps"], "ms_per_step": r["ms_per_step"],
+                          "serial_ms_per_step": r["serial_ms_per_step"], "config": r["config"]}
+    if a.served and a.config == "wheel_bed":
+        r = run_serve(a.seed, 48, [SERVE_RATE], a.serve_particles, a.serve_blocks)
+        top = r["top"]
+        out["served"] = {"metric": "particle-updates/s (served requests, aggregate over the makespan)",
+                         "value": top["value"], "rate_per_s": top["rate"], "p50_ms": top["p50_ms"],
+                         "p99_ms": top["p99_ms"], "makespan_s": top["makespan_s"],
+                         "serial_value": r["serial_value"],
+                         "speedup_vs_serial": top["value"] / r["serial_value"] if r["serial_value"] else None,
+                         "most_running": top["most_running"], "server_time": top["server_time"],
+                         "config": r["config"]}
     print(json.dumps(out))
     return 0
 
