@@ -36,7 +36,6 @@ COMMON = [
 UNITS = [
     ("mpm_ref_kernels.hip", ["-ffp-contract=off"]),
     ("mpm_fast_kernels.hip", []),
-    ("mpm_fused.hip", []),
     ("mpm_split.hip", []),
     ("bm_mpm_api.hip", ["-ffp-contract=off"]),
     ("bm_batch.hip", []),

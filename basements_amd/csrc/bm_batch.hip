@@ -332,7 +332,8 @@ static void batch_unit_bytes(int64_t& per_particle, int64_t& per_block, int64_t&
                              + sizeof(int) * 3              // FP64 redo list, bin key, bin slot
                              + 2);                          // deposit units and chunk table
     per_block = (int64_t)(2 + sizeof(int)                   // touched flags, block -> scene
-                          + 2 * (sizeof(long long) * 64 + sizeof(float4) * 64 + sizeof(uint32_t) + sizeof(int32_t))
+                          + 2 * (sizeof(long long) * 64 + sizeof(float4) * 64 + sizeof(float) * 3 * 64 +
+                                 sizeof(uint32_t) + sizeof(int32_t))
                           + sizeof(int) * 4 + sizeof(int4)  // bin counts and starts, units
                           + 8);                             // scan scratch
     per_scene = (int64_t)(sizeof(SceneDev) + sizeof(bm_body) + sizeof(ColliderDev) + sizeof(int) * 3 +
@@ -389,11 +390,13 @@ int bm_batch_create(const bm_batch_desc* desc, bm_batch_handle* out) {
     for (int k = 0; ok && k < 2; ++k) {
         GridBuf& g = b->grid[k];
         ok = A((void**)&g.mq, sizeof(long long) * 64 * cb) && A((void**)&g.mv, sizeof(float4) * 64 * cb) &&
-             A((void**)&g.bflag, sizeof(uint32_t) * cb) && A((void**)&g.blist, sizeof(int32_t) * cb) &&
-             A((void**)&g.bcount, sizeof(int32_t));
+             A((void**)&g.mom, sizeof(float) * 3 * 64 * cb) && A((void**)&g.bflag, sizeof(uint32_t) * cb) &&
+             A((void**)&g.blist, sizeof(int32_t) * cb) && A((void**)&g.bcount, sizeof(int32_t));
+        g.mstride = 64 * cb;
         if (ok) {
             ok = hipMemset(g.mq, 0, sizeof(long long) * 64 * cb) == hipSuccess &&
                  hipMemset(g.mv, 0, sizeof(float4) * 64 * cb) == hipSuccess &&
+                 hipMemset(g.mom, 0, sizeof(float) * 3 * 64 * cb) == hipSuccess &&
                  hipMemset(g.bflag, 0, sizeof(uint32_t) * cb) == hipSuccess &&
                  hipMemset(g.bcount, 0, sizeof(int32_t)) == hipSuccess;
         }
@@ -431,6 +434,7 @@ int bm_batch_destroy(bm_batch_handle b) {
     for (GridBuf& g : b->grid) {
         hipFree(g.mq);
         hipFree(g.mv);
+        hipFree(g.mom);
         hipFree(g.bflag);
         hipFree(g.blist);
         hipFree(g.bcount);
@@ -541,6 +545,8 @@ int bm_batch_remove_scene(bm_batch_handle b, int32_t id) {
     for (GridBuf& g : b->grid) {
         HIP_TRY(hipMemsetAsync(g.mq, 0, sizeof(long long) * 64 * b->B, b->stream));
         HIP_TRY(hipMemsetAsync(g.mv, 0, sizeof(float4) * 64 * b->B, b->stream));
+        for (int c = 0; c < 3; ++c)
+            HIP_TRY(hipMemsetAsync(g.mom + c * g.mstride, 0, sizeof(float) * 64 * b->B, b->stream));
         HIP_TRY(hipMemsetAsync(g.bflag, 0, sizeof(uint32_t) * b->B, b->stream));
         HIP_TRY(hipMemsetAsync(g.bcount, 0, sizeof(int32_t), b->stream));
     }
@@ -662,8 +668,8 @@ int bm_batch_set_tuning(bm_batch_handle b, int32_t unit_particles, int32_t sort_
     if (int rc = check_batch(b)) return rc;
     if (deposit_groups > 0) b->deposit_groups = (deposit_groups + 7) / 8 * 8;
     if (unit_particles > 0) {
-        if (unit_particles < 16 || unit_particles > FUSED_UNIT_MAX)
-            return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", FUSED_UNIT_MAX);
+        if (unit_particles < 16 || unit_particles > UNIT_MAX)
+            return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", UNIT_MAX);
         b->unit_p = unit_particles;
     }
     if (sort_every > 0) b->sort_every = sort_every;

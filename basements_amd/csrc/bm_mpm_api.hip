@@ -181,10 +181,15 @@ static int check_handle(bm_mpm_handle h) {
     return BM_OK;
 }
 
-static int alloc_grid(GridBuf& g, int64_t nnodes, int64_t nblocks, bool with_force) {
+static int alloc_grid(GridBuf& g, int64_t nnodes, int64_t nblocks, bool with_force, bool with_mom) {
     HIP_TRY(hipMalloc(&g.mq, sizeof(long long) * nnodes));
     HIP_TRY(hipMalloc(&g.mv, sizeof(float4) * nnodes));
     if (with_force) HIP_TRY(hipMalloc(&g.f, sizeof(float4) * nnodes));
+    if (with_mom) {
+        HIP_TRY(hipMalloc(&g.mom, sizeof(float) * 3 * nnodes));
+        HIP_TRY(hipMemset(g.mom, 0, sizeof(float) * 3 * nnodes));
+        g.mstride = nnodes;
+    }
     HIP_TRY(hipMalloc(&g.bflag, sizeof(uint32_t) * nblocks));
     HIP_TRY(hipMalloc(&g.blist, sizeof(int32_t) * nblocks));
     HIP_TRY(hipMalloc(&g.bcount, sizeof(int32_t)));
@@ -200,6 +205,7 @@ static void free_grid(GridBuf& g) {
     hipFree(g.mq);
     hipFree(g.mv);
     hipFree(g.f);
+    hipFree(g.mom);
     hipFree(g.bflag);
     hipFree(g.blist);
     hipFree(g.bcount);
@@ -273,9 +279,9 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
         return bail(fail(BM_ERR_OOM, "particle arrays (%lld particles)", (long long)h->cap));
     h->fast = desc->mode == BM_MODE_FAST;
     if (desc->sort_every > 0) h->sort_every = desc->sort_every;
-    if ((rc = alloc_grid(h->grid[0], h->nnodes, h->nblocks, !h->fast)) != BM_OK) return bail(BM_ERR_OOM);
+    if ((rc = alloc_grid(h->grid[0], h->nnodes, h->nblocks, !h->fast, h->fast)) != BM_OK) return bail(BM_ERR_OOM);
     if (h->fast) {
-        if ((rc = alloc_grid(h->grid[1], h->nnodes, h->nblocks, false)) != BM_OK) return bail(BM_ERR_OOM);
+        if ((rc = alloc_grid(h->grid[1], h->nnodes, h->nblocks, false, true)) != BM_OK) return bail(BM_ERR_OOM);
         if (h->cap > (int64_t)INT32_MAX) return bail(fail(BM_ERR_INVALID_ARG, "fast mode holds at most 2^31-1 particles"));
         if (h->nblocks > (int64_t)INT32_MAX / 64) return bail(fail(BM_ERR_INVALID_ARG, "grid too large"));
         const int64_t pc = std::max<int64_t>(h->cap, 1);
@@ -532,44 +538,39 @@ static int rebin(bm_mpm_handle h, bool with_vc = true) {
     return BM_OK;
 }
 
-// The fast substep's particle kernels: fused G2P+P2G, or the two-kernel form
-// (tuning variant bit 2).  do_g2p = false deposits the current state only;
-// store_vc = false skips the velocity and C stores of the two-kernel G2P.
+// The fast substep's particle kernels: the per-particle G2P (or the deposit
+// record of the current state when do_g2p = false) and the per-cell deposit.
+// store_vc = false skips the velocity and C stores of the G2P.
 static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const GridBuf& gout, float dt_g2p,
                          float dt_fold, bool store_vc = true) {
-    if (h->variant & 4) {
-        // Time each kernel on its own: the particle pass under the caller's
-        // phase (G2P, or P2G for a deposit of the current state) and the cell
-        // deposit under P2G, each with events recorded by its own dispatch;
-        // the small bookkeeping launches between them are OTHER.
-        struct Ctx {
-            bm_mpm_handle h;
-            int adv_phase;
-        } ctx{h, h->open_phase};
-        auto mark = [](void* p, int stage, hipEvent_t* a, hipEvent_t* b) {
-            auto* c = (Ctx*)p;
-            bm_mpm_handle hh = c->h;
-            if (!hh->timing || c->adv_phase < 0) return;
-            if (stage == SPLIT_REDO) {
-                hh->begin(BM_PHASE_OTHER);
-                return;
-            }
-            hh->end();
-            *a = hh->take_event();
-            *b = hh->take_event();
-            if (*a && *b)
-                hh->pending.push_back(bm_mpm_solver::Timed{stage == SPLIT_ADVANCE ? c->adv_phase : BM_PHASE_P2G, *a, *b});
-            else
-                *a = *b = nullptr;
-        };
-        if (ctx.adv_phase >= 0) h->open_phase = BM_PHASE_OTHER;  // the opening event now starts OTHER
-        launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
-                     dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream, mark,
-                     &ctx, (h->variant & 8) != 0, SplitBatch{}, store_vc);
-    } else
-        launch_fused(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p,
-                     dt_fold, h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->variant & 3,
-                     h->stream);
+    // Time each kernel on its own: the particle pass under the caller's phase
+    // (G2P, or P2G for a deposit of the current state) and the cell deposit
+    // under P2G, each with events recorded by its own dispatch; the small
+    // bookkeeping launches between them are OTHER.
+    struct Ctx {
+        bm_mpm_handle h;
+        int adv_phase;
+    } ctx{h, h->open_phase};
+    auto mark = [](void* p, int stage, hipEvent_t* a, hipEvent_t* b) {
+        auto* c = (Ctx*)p;
+        bm_mpm_handle hh = c->h;
+        if (!hh->timing || c->adv_phase < 0) return;
+        if (stage == SPLIT_REDO) {
+            hh->begin(BM_PHASE_OTHER);
+            return;
+        }
+        hh->end();
+        *a = hh->take_event();
+        *b = hh->take_event();
+        if (*a && *b)
+            hh->pending.push_back(bm_mpm_solver::Timed{stage == SPLIT_ADVANCE ? c->adv_phase : BM_PHASE_P2G, *a, *b});
+        else
+            *a = *b = nullptr;
+    };
+    if (ctx.adv_phase >= 0) h->open_phase = BM_PHASE_OTHER;  // the opening event now starts OTHER
+    launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p, dt_fold,
+                 h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream, mark, &ctx,
+                 (h->variant & 8) != 0, SplitBatch{}, store_vc);
 }
 
 static int p2g_fast(bm_mpm_handle h, float dt) {
@@ -583,7 +584,7 @@ static int p2g_fast(bm_mpm_handle h, float dt) {
         if (int rc = rebin(h)) return rc;
     GridBuf& g = h->grid[h->cur];
     h->begin(BM_PHASE_CLEAR);
-    launch_clear(g, (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream, h->n_redo, (h->variant & 4) != 0);
+    launch_clear(g, (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream, h->n_redo, true);
     h->end();
     h->begin(BM_PHASE_P2G);
     fast_kernels(h, false, g, g, dt, dt);
@@ -604,11 +605,11 @@ static int g2p_fast(bm_mpm_handle h, float dt, bool store_vc) {
     if (!h->next_cleared) {
         h->begin(BM_PHASE_CLEAR);
         launch_clear(h->grid[nxt], (int)std::min<int64_t>(h->nblocks, 1 << 30), h->stream, h->n_redo,
-                     (h->variant & 4) != 0);
+                     true);
         h->end();
     }
     h->next_cleared = false;
-    h->begin((h->variant & 4) ? BM_PHASE_G2P : BM_PHASE_G2P_P2G);
+    h->begin(BM_PHASE_G2P);
     fast_kernels(h, true, h->grid[h->cur], h->grid[nxt], dt, dt, store_vc);
     h->end();
     HIP_TRY(hipGetLastError());
@@ -641,7 +642,7 @@ static int grid_update_impl(bm_mpm_handle h, float dt, bool with_body) {
     if (h->phase != PH_P2G) return fail(BM_ERR_STATE, "grid_update needs a preceding p2g");
     // two-kernel fast form: the update also clears the grid the next G2P
     // deposits into (and the FP64 redo count), saving that launch
-    const bool clear_next = h->fast && (h->variant & 4);
+    const bool clear_next = h->fast;
     const GridBuf* nxt = clear_next ? &h->grid[1 - h->cur] : nullptr;
     int* zw = clear_next ? h->n_redo : nullptr;
     h->begin(BM_PHASE_UPDATE);
@@ -920,9 +921,12 @@ int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallback
 
 int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_every, int32_t variant) {
     if (int rc = check_handle(h)) return rc;
-    if (unit_particles > FUSED_UNIT_MAX || (unit_particles > 0 && unit_particles < 16))
-        return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", FUSED_UNIT_MAX);
+    if (unit_particles > UNIT_MAX || (unit_particles > 0 && unit_particles < 16))
+        return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", UNIT_MAX);
     if (variant > 15) return fail(BM_ERR_INVALID_ARG, "unknown variant %d", variant);
+    if (variant >= 0 && !(variant & 4))
+        return fail(BM_ERR_NOT_SUPPORTED, "variant %d: the single-kernel G2P+P2G form was retired (bit 2 must be set)",
+                    variant);
     if (unit_particles > 0 && unit_particles != h->unit_p) {
         h->unit_p = unit_particles;
         h->binned = false;  // units must be rebuilt

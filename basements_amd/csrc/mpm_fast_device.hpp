@@ -276,10 +276,9 @@ __device__ __forceinline__ void deposit_global(const GridBuf& g, uint8_t* bever,
                                                int lz, int dq, v3<float> mom) {
     const int64_t nid = node_of(d, lx, ly, lz);
     atomicAdd((unsigned long long*)&g.mq[nid], (unsigned long long)(long long)dq);
-    float* nm = (float*)&g.mv[nid];
-    atomicAdd(nm + 0, mom.x);
-    atomicAdd(nm + 1, mom.y);
-    atomicAdd(nm + 2, mom.z);
+    atomicAdd(g.mom + nid, mom.x);
+    atomicAdd(g.mom + g.mstride + nid, mom.y);
+    atomicAdd(g.mom + 2 * g.mstride + nid, mom.z);
     touch_block_f(g, bever, (int)(nid >> 6));
 }
 

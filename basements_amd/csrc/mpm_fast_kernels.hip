@@ -118,87 +118,6 @@ __global__ void k_iota(int* p, int64_t n, int64_t first) {
     if (i < n) p[i] = (int)(first + i);
 }
 
-// Particles whose FP32 SVD lost orthogonality: the whole particle update is
-// redone here with the FP64 Jacobi, reading the grid from global memory and
-// depositing with global atomics.  One lane per listed particle.
-template <bool DO_G2P>
-__global__ void __launch_bounds__(64) k_redo(PView P, GridBuf gin, GridBuf gout, uint8_t* bever, Domain d, Phys ph,
-                                             FastArgs fa, int* err, unsigned long long* counters) {
-    const int cnt = *fa.n_redo;
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
-        const int64_t i = fa.redo[r];
-        float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
-        m3<float> F;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) F.a[c] = P.p[(PF + c) * P.cap + i];
-        v3<float> vel, lns;
-        m3<float> Cm, Us;
-        const float dx = d.dx, inv_dx = d.inv_dx;
-        if (DO_G2P) {
-            Stencil s;
-            make_stencil(x, y, z, inv_dx, s);
-            const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
-            gather(s, dx, 4.0f * inv_dx * inv_dx,
-                   [&](int a, int b, int c) { return gin.mv[node_of(d, lx + a, ly + b, lz + c)]; }, vel, Cm);
-            const float dt = fa.dt_g2p;
-            x += vel.x * dt;
-            y += vel.y * dt;
-            z += vel.z * dt;
-            m3<float> A;
-#pragma unroll
-            for (int c = 0; c < 9; ++c) A.a[c] = ((c % 4 == 0) ? 1.0f : 0.0f) + Cm.a[c] * dt;
-            const m3<float> Ftr = mmul(A, F);
-            m3<float> V;
-            v3<float> sg;
-            svd_guarded(Ftr, Us, sg, V);
-            const Material M{ph.mu, ph.lambda, ph.alpha, ph.kohesion};
-            const v3<float> sn = dp_project(sg, M, &lns);
-            F = recompose(Us, sn, V);
-            if (y < 0.0f) {
-                y = 0.0f;
-                vel.y = 0.0f;
-            }
-            check_finite(err, -1, x, y, z, vel.x, vel.y, vel.z);
-            P.p[(PX + 0) * P.cap + i] = x;
-            P.p[(PX + 1) * P.cap + i] = y;
-            P.p[(PX + 2) * P.cap + i] = z;
-            P.p[(PV + 0) * P.cap + i] = vel.x;
-            P.p[(PV + 1) * P.cap + i] = vel.y;
-            P.p[(PV + 2) * P.cap + i] = vel.z;
-#pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                P.p[(PF + c) * P.cap + i] = F.a[c];
-                P.p[(PC + c) * P.cap + i] = Cm.a[c];
-            }
-        } else {
-            vel = mk(P.p[(PV + 0) * P.cap + i], P.p[(PV + 1) * P.cap + i], P.p[(PV + 2) * P.cap + i]);
-#pragma unroll
-            for (int c = 0; c < 9; ++c) Cm.a[c] = P.p[(PC + c) * P.cap + i];
-            m3<float> V;
-            v3<float> sg;
-            svd_guarded(F, Us, sg, V);
-            lns = mk(logf(fmaxf(sg.x, 1e-6f)), logf(fmaxf(sg.y, 1e-6f)), logf(fmaxf(sg.z, 1e-6f)));
-        }
-        const float mass = P.p[PM * P.cap + i];
-        const float V0 = P.p[PV0 * P.cap + i];
-        v3<float> mvel;
-        m3<float> Aff;
-        p2g_coeffs(mass, V0, vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
-        Stencil s;
-        make_stencil(x, y, z, inv_dx, s);
-        const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
-        if (!stencil_ok(d, lx, ly, lz)) {
-            atomicOr(err, 1);
-            continue;
-        }
-        const double qp = rint((double)mass * fa.qscale_inv);
-        scatter(s, dx, qp, mvel, Aff, [&](int a, int b, int c, float w, int dq, v3<float> mom) {
-            if (w != 0.0f) deposit_global(gout, bever, d, lx + a, ly + b, lz + c, dq, mom);
-        });
-        atomicAdd(&counters[0], 1ull);
-    }
-}
-
 // ================================================================ launchers
 static inline int grid_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
@@ -243,15 +162,6 @@ void launch_unpermute(const PView& P, float* dst, const int* pid, int64_t base, 
 
 void launch_iota(int* p, int64_t n, int64_t first, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, first);
-}
-
-void launch_redo(bool do_g2p, const PView& P, const GridBuf& gin, const GridBuf& gout, uint8_t* bever,
-                 const Domain& d, const Phys& ph, const FastArgs& fa, int* err, unsigned long long* counters,
-                 hipStream_t s) {
-    if (do_g2p)
-        hipLaunchKernelGGL(k_redo<true>, dim3(32), dim3(64), 0, s, P, gin, gout, bever, d, ph, fa, err, counters);
-    else
-        hipLaunchKernelGGL(k_redo<false>, dim3(32), dim3(64), 0, s, P, gin, gout, bever, d, ph, fa, err, counters);
 }
 
 

@@ -28,6 +28,7 @@ __global__ void __launch_bounds__(64) k_clear_blocks(GridBuf g, int* zero_word) 
         g.mq[nid] = 0;
         g.mv[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (g.f) g.f[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (g.mom) g.mom[nid] = g.mom[g.mstride + nid] = g.mom[2 * g.mstride + nid] = 0.f;
         if (threadIdx.x == 0) g.bflag[blk] = 0u;
     }
     if (zero_word && blockIdx.x == 0 && threadIdx.x == 0) *zero_word = 0;
@@ -159,6 +160,7 @@ __global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d0, Phys ph0, d
             const int64_t nid = (int64_t)blk * 64 + threadIdx.x;
             clr.mq[nid] = 0;
             clr.mv[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (clr.mom) clr.mom[nid] = clr.mom[clr.mstride + nid] = clr.mom[2 * clr.mstride + nid] = 0.f;
             if (threadIdx.x == 0) clr.bflag[blk] = 0u;
         }
         if (zero_word && blockIdx.x == 0 && threadIdx.x == 0) *zero_word = 0;
@@ -185,8 +187,10 @@ __global__ void __launch_bounds__(64) k_update(GridBuf g, Domain d0, Phys ph0, d
         }
         const int64_t nid = (int64_t)blk * 64 + threadIdx.x;
         const long long mq = g.mq[nid];
-        float4 q = g.mv[nid];
         if (mq == 0) continue;  // inactive node (no particle weight)
+        // the deposit's momentum: per-component arrays (fast mode) or mv
+        float4 q = g.mom ? make_float4(g.mom[nid], g.mom[g.mstride + nid], g.mom[2 * g.mstride + nid], 0.f)
+                         : g.mv[nid];
         const float m = (float)((double)mq * qscale);
         q.w = m;
         if (m > 1e-6f) {
@@ -617,6 +621,11 @@ __global__ void k_grid_gather(GridBuf g, uint8_t* bever, Domain d, int org0, int
         const int64_t nid = node_of(d, lx, ly, lz);
         mq = g.mq[nid];
         q = g.mv[nid];
+        if (!updated && g.mom) {  // the deposit's momentum (fast mode)
+            q.x = g.mom[nid];
+            q.y = g.mom[g.mstride + nid];
+            q.z = g.mom[2 * g.mstride + nid];
+        }
         if (g.f) f = g.f[nid];
         al = bever ? bever[nid >> 6] : 0;
     }

@@ -27,7 +27,7 @@
 namespace bm {
 
 constexpr int NCOEF = 12;
-constexpr int SUNIT_MAX = 512;   // largest unit (see FUSED_UNIT_MAX)
+constexpr int SUNIT_MAX = 512;   // largest unit (see UNIT_MAX)
 
 __device__ __forceinline__ int stix(int ix, int iy, int iz) { return ((ix * TE + iy) * TE) + ((iz + 4 * (iy & 1)) & 7); }
 __device__ __forceinline__ void stile_coords(int k, int& ix, int& iy, int& iz) {
@@ -495,10 +495,9 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
         const int lx = tx0 + ix, ly = ty0 + iy, lz = tz0 + iz;
         const int64_t nid = node_of(d, lx, ly, lz);
         atomicAdd((unsigned long long*)&gout.mq[nid], (unsigned long long)m);
-        float* nm = (float*)&gout.mv[nid];
-        atomicAdd(nm + 0, q.x);
-        atomicAdd(nm + 1, q.y);
-        atomicAdd(nm + 2, q.z);
+        atomicAdd(gout.mom + nid, q.x);
+        atomicAdd(gout.mom + gout.mstride + nid, q.y);
+        atomicAdd(gout.mom + 2 * gout.mstride + nid, q.z);
         S.tb[(((lx >> 2) - bx + 1) * 3 + ((ly >> 2) - by + 1)) * 3 + ((lz >> 2) - bz + 1)] = 1;
     }
     __syncthreads();

@@ -52,21 +52,17 @@ BYTES_REF = {
 # per substep instead of 80): P2G deposits m + momentum (16 B), the update
 # reads 16 B and writes 12 B, G2P reads 12 B.  Two-kernel form: the particle
 # pass (phase g2p) completes G2P; the cell deposit (phase p2g) completes the
-# next substep's P2G.  The single-kernel form (g2p_p2g) completes both and
-# is credited with the state read once: x F m V0 (56 B) in, x v C F (96 B)
-# out per particle; v in and m + momentum out per node.
+# next substep's P2G.
 BYTES_FAST = {
     "p2g": (104, 16),
     "update": (0, 28),
     "g2p": (144, 12),
-    "g2p_p2g": (152, 28),
 }
 # Kernel that each phase's roofline describes (names as rocprofv3 lists them).
 KERNEL_OF = {
     ("fast", "g2p"): "k_advance<true>",
     ("fast", "p2g"): "k_p2g_cell<1>",
     ("fast", "update"): "k_update",
-    ("fast", "g2p_p2g"): "k_fused",
     ("reference", "p2g"): "k_p2g_ref",
     ("reference", "update"): "k_update",
     ("reference", "g2p"): "k_g2p_ref",

@@ -256,13 +256,12 @@ int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallback
 /* BM_MODE_FAST launch parameters (what the autotuner searches):
  * unit_particles: particles per workgroup unit (16..512, default 512);
  * sort_every: substeps between particle re-binnings (>= 1, default 32);
- * variant: bit 2 = two-kernel form (per-particle G2P writing deposit
- * coefficients, then a per-block P2G accumulating each cell's particles in
- * registers; the default, variant 4); without bit 2 the fused G2P+P2G
- * kernel runs, with bit 0 = plain read-modify-write LDS deposit (else
- * collision-free LDS atomics) and bit 1 = G2P node velocities staged in LDS;
- * bit 3 (with bit 2) = the per-cell deposit in two passes over each cell's
- * particles (fewer registers, particle data read twice).
+ * variant: bit 2 = the two-kernel form (per-particle G2P writing deposit
+ * records, then a per-block P2G accumulating each cell's particles in
+ * registers; always set -- the single-kernel fused form was retired and a
+ * variant without bit 2 returns BM_ERR_NOT_SUPPORTED); bit 3 = the per-cell
+ * deposit in two passes over each cell's particles (fewer registers,
+ * particle data read twice).  Default 4.
  * Values <= 0 / < 0 keep the current setting.  No effect in
  * BM_MODE_REFERENCE. */
 int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_every, int32_t variant);
@@ -277,7 +276,7 @@ enum bm_phase {
     BM_PHASE_UPDATE = 2,  /* grid update (gravity, floor, walls) */
     BM_PHASE_COUPLE = 3,  /* M0 / M1 coupler and its reaction reduction */
     BM_PHASE_G2P = 4,     /* grid-to-particle gather + plasticity */
-    BM_PHASE_G2P_P2G = 5, /* fused G2P(n) + P2G(n+1) (BM_MODE_FAST) */
+    BM_PHASE_G2P_P2G = 5, /* the retired fused G2P(n) + P2G(n+1) kernel (no longer recorded) */
     BM_PHASE_SORT = 6,    /* particle re-binning (BM_MODE_FAST) */
     BM_PHASE_OTHER = 7,
     BM_NUM_PHASES = 8
