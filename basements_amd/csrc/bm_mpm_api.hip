@@ -17,6 +17,8 @@
 #include "host_staging.hpp"
 #include "mpm_internal.hpp"
 
+#include <atomic>
+
 using namespace bm;
 
 // ------------------------------------------------------------------ errors
@@ -225,6 +227,52 @@ static int sync_and_check(bm_mpm_handle h) {
     return BM_OK;
 }
 
+// ------------------------------------------------------------------ allocation-free coupler
+// Scratch of bm_mpm_couple_noalloc: the coupler's per-workgroup partial
+// reactions (COUPLE_WGS x 7 doubles) and their fixed-order sum (7 + 1
+// doubles), 7,232 bytes per GPU, held until the process ends.  Every
+// workgroup writes all 7 of its partials and the sum is written whole, so
+// nothing needs clearing between calls; calls that overlap on one GPU share
+// the array and race.
+constexpr int NOALLOC_PARTIALS = COUPLE_WGS * 7;
+constexpr int NOALLOC_DOUBLES = NOALLOC_PARTIALS + 8;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+// The device address of `scratch` on each GPU (the host-side name is only a
+// stand-in): looked up once per device, with that device current.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::atomic<double*> g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+static int noalloc_scratch(double** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(BM_ERR_NOT_SUPPORTED, "device %d beyond the %d the allocation-free coupler tracks", dev,
+                    NOALLOC_MAX_DEVICES);
+    double* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        void* addr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !addr)
+            return fail(BM_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        p = static_cast<double*>(addr);
+        g_scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+    *out = p;
+    return BM_OK;
+}
+
+// The reaction in the caller's device memory: the FP64 sum rounded to FP32
+// as bm_mpm_couple rounds it on the host (zero reaction when the sum is null).
+__global__ void k_reaction_out(const double* out7, bm_reaction* r) {
+    if (threadIdx.x != 0) return;
+    for (int a = 0; a < 3; ++a) {
+        r->force[a] = out7 ? (float)out7[a] : 0.0f;
+        r->torque[a] = out7 ? (float)out7[3 + a] : 0.0f;
+    }
+    r->nodes_inside = out7 ? (int32_t)llround(out7[6]) : 0;
+}
+
 extern "C" {
 
 int bm_abi_version(void) { return BM_MPM_ABI_VERSION; }
@@ -239,6 +287,11 @@ int bm_mpm_create(const bm_mpm_desc* desc, bm_mpm_handle* out) {
         if (desc->grid_dims[a] <= 0) return fail(BM_ERR_INVALID_ARG, "grid_dims must be > 0");
     if (desc->mode != BM_MODE_REFERENCE && desc->mode != BM_MODE_FAST)
         return fail(BM_ERR_INVALID_ARG, "unknown mode %d", desc->mode);
+
+    // set-up of the allocation-free coupler's scratch on this device
+    double* scratch_ptr = nullptr;
+    if (int rc = noalloc_scratch(&scratch_ptr)) return rc;
+    (void)scratch_ptr;
 
     auto* h = new bm_mpm_solver();
     h->desc = *desc;
@@ -721,6 +774,32 @@ int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, floa
         out->torque[a] = (float)r[3 + a];
     }
     out->nodes_inside = (int32_t)std::llround(r[6]);
+    return BM_OK;
+}
+
+int bm_mpm_couple_noalloc(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* reaction) {
+    if (int rc = check_handle(h)) return rc;
+    if (!body || !reaction) return fail(BM_ERR_INVALID_ARG, "null argument");
+    if (method < BM_COUPLER_M0_DIRICHLET || method > BM_COUPLER_M45_BSPLINE)
+        return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
+    // the partials fit the scratch for every input: one launch always runs
+    // exactly COUPLE_WGS workgroups (the scan range only changes their work)
+    static_assert(NOALLOC_PARTIALS >= COUPLE_WGS * 7, "partials exceed the scratch");
+    double* sp = nullptr;
+    if (int rc = noalloc_scratch(&sp)) return rc;
+    if (!(dt > 0.0f)) {  // zero reaction, as bm_mpm_couple (mpm_rigid_coupler.cpp:62)
+        hipLaunchKernelGGL(k_reaction_out, dim3(1), dim3(64), 0, h->stream, (const double*)nullptr, reaction);
+        HIP_TRY(hipGetLastError());
+        return BM_OK;
+    }
+    if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "couple needs an updated grid");
+    ColliderDev c = make_collider(h, body);
+    h->begin(BM_PHASE_COUPLE);
+    launch_couple(method, c, h->grid[h->cur], h->bever, h->dom, dt, sp, sp + NOALLOC_PARTIALS, h->stream);
+    hipLaunchKernelGGL(k_reaction_out, dim3(1), dim3(64), 0, h->stream, (const double*)(sp + NOALLOC_PARTIALS),
+                       reaction);
+    h->end();
+    HIP_TRY(hipGetLastError());
     return BM_OK;
 }
 

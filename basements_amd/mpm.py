@@ -273,6 +273,7 @@ _SIGS = {
     "bm_diff_forward": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_double)]),
     "bm_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(DiffGradC)]),
     "bm_diff_download": (C.c_int, [C.c_void_p, C.POINTER(HostParticles)]),
+    "bm_mpm_couple_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ColliderC), C.c_float, C.c_void_p]),
     "bm_diff_map_vjp": (C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(DiffParamsC), C.c_int32,
                                   C.c_void_p, C.c_void_p]),
     "bm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -438,6 +439,13 @@ class Solver:
         c = col.to_c()
         _check(lib().bm_mpm_couple(self.h, int(method), C.byref(c), float(dt), C.byref(r)))
         return np.array(r.force[:], np.float64), np.array(r.torque[:], np.float64), int(r.nodes_inside)
+
+    def couple_noalloc(self, method, col: Collider, dt, reaction_device_ptr):
+        """bm_mpm_couple_noalloc: the reaction (28 bytes, bm_reaction) goes to
+        the given device address on the handle's stream; nothing waits."""
+        c = col.to_c()
+        _check(lib().bm_mpm_couple_noalloc(self.h, int(method), C.byref(c), float(dt),
+                                           C.c_void_p(int(reaction_device_ptr))))
 
     # device body -------------------------------------------------------
     def set_body(self, body: Body):

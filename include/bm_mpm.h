@@ -180,6 +180,17 @@ int bm_mpm_g2p(bm_mpm_handle h, float dt);
  * reaction, as in the reference (mpm_rigid_coupler.cpp:62, _pic.h:92). */
 int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* out);
 
+/* bm_mpm_couple without allocation or host synchronisation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops): the
+ * coupler's partial reactions live in a fixed 7,232-byte device array of the
+ * library (one per GPU, held until the process ends), and the reaction is
+ * written to `reaction` in the caller's DEVICE memory, on the handle's stream;
+ * the call returns without waiting.  The grid is updated in place as by
+ * bm_mpm_couple.  Calls that overlap on one GPU share that array and race:
+ * order them on one stream.  No counterpart in the reference (its couplers,
+ * mpm_rigid_coupler.h:212-214, return the reaction by value). */
+int bm_mpm_couple_noalloc(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* reaction);
+
 /* ---------------------------------------------------------------- device body */
 /* The rigid body of the coupled loop, kept in device memory so a coupled
  * substep needs no host round trip.  Body fields follow RigidBody
