@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "../../include/bm_mpm.h"
+#include "particle_layout.hpp"
 
 namespace bm {
 
@@ -46,9 +47,8 @@ struct Domain {
     int64_t b0;          // first block of this box in the grid arrays (0 for a lone scene)
 };
 
-// Particle SoA view: component c of particle i at p[c * cap + i].
-// Components: x 0-2, v 3-5, F 6-14, C 15-23, m 24, V0 25.
-enum { PX = 0, PV = 3, PF = 6, PC = 15, PM = 24, PV0 = 25, PCOMP = 26 };
+// Particle SoA view: component c of particle i at p[c * cap + i]
+// (component indices in particle_layout.hpp).
 
 struct PView {
     float* p;
