@@ -19,6 +19,11 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libbm_mpm.so")
 HOSTTEST_SRC = os.path.join(ROOT, "tests", "cpp", "host_api_test.cpp")
 HOSTTEST_BIN = os.path.join(LIBDIR, "host_api_test")
+PLUGIN_SRC = os.path.join(PKG, "plugin", "bm_mpm_soil_plugin.c")
+PLUGIN_LIB = os.path.join(LIBDIR, "libbm_mpm_soil_plugin.so")
+# the reference's headers: the plugin form compiles against its plugin_capi.h
+REF_INCLUDE = os.environ.get("BASEMENTS_REF_INCLUDE", "/root/reference/include")
+PLUGIN_HDR = os.path.join("basements", "engine", "plugins", "plugin_capi.h")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -88,9 +93,30 @@ def build_host_test(verbose: bool = False) -> str | None:
     return HOSTTEST_BIN
 
 
+def plugin_cmd(out: str, ref_include: str = REF_INCLUDE, lang: str = "c") -> list:
+    """Compile line of the reference-plugin form (plugin/bm_mpm_soil_plugin.c)
+    against the reference's plugin header, linking libbm_mpm.so."""
+    std = ["-x", "c", "-std=c99"] if lang == "c" else ["-x", "c++", "-std=c++17"]
+    return ["gcc" if lang == "c" else "g++", *std, "-O2", "-Wall", "-Wextra", "-Werror", "-pedantic", "-shared",
+            "-fPIC", "-isystem", ref_include, "-I", os.path.join(ROOT, "include"), PLUGIN_SRC, "-x", "none", "-L", LIBDIR,
+            "-lbm_mpm", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN", "-o", out]
+
+
+def build_plugin(verbose: bool = False) -> str | None:
+    """libbm_mpm_soil_plugin.so, where the reference's plugin header is present."""
+    if not os.path.exists(os.path.join(REF_INCLUDE, PLUGIN_HDR)):
+        return None
+    cmd = plugin_cmd(PLUGIN_LIB)
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    return PLUGIN_LIB
+
+
 def build(verbose: bool = False):
     lib = build_library(verbose)
     build_host_test(verbose)
+    build_plugin(verbose)
     return lib
 
 

@@ -352,9 +352,26 @@ class Solver:
         self.mode = mode
         self.cap = int(max_particles)
 
+    @classmethod
+    def attach(cls, handle, dx, origin, dims, max_particles, mode=MODE_REFERENCE):
+        """Wrap a handle created elsewhere (a plugin's, another library's)
+        without taking it over: close() leaves it to its owner."""
+        if not handle:
+            raise BMError(1, "attach: null handle")
+        self = cls.__new__(cls)
+        self.h = C.c_void_p(int(handle) if not isinstance(handle, C.c_void_p) else handle.value)
+        self.dx = float(dx)
+        self.origin = tuple(int(v) for v in origin)
+        self.dims = tuple((int(v) + 3) // 4 * 4 for v in dims)
+        self.mode = mode
+        self.cap = int(max_particles)
+        self._owned = False
+        return self
+
     def close(self):
         if getattr(self, "h", None):
-            lib().bm_mpm_destroy(self.h)
+            if getattr(self, "_owned", True):
+                lib().bm_mpm_destroy(self.h)
             self.h = None
 
     def __del__(self):
