@@ -340,6 +340,103 @@ static void batch_unit_bytes(int64_t& per_particle, int64_t& per_block, int64_t&
                           sizeof(double) * (7 * COUPLE_WGS + 8) + sizeof(int4));
 }
 
+// Admission, first half: validation and placement at the arena ends
+// (nothing is written yet).
+static int place_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, int q_exp, const bm_body* body,
+                       SceneHost& S) {
+    if (!(sd->dx > 0.0f) || !std::isfinite(sd->dx)) return fail(BM_ERR_INVALID_ARG, "dx must be > 0");
+    for (int a = 0; a < 3; ++a)
+        if (sd->grid_dims[a] <= 0) return fail(BM_ERR_INVALID_ARG, "grid_dims must be > 0");
+    if (body && (sd->coupler < BM_COUPLER_M0_DIRICHLET || sd->coupler > BM_COUPLER_M45_BSPLINE))
+        return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", sd->coupler);
+    if (body && body->shape != BM_SHAPE_BOX && body->shape != BM_SHAPE_CYLINDER)
+        return fail(BM_ERR_INVALID_ARG, "unknown shape %d", body->shape);
+    S = SceneHost{};
+    S.desc = *sd;
+    Domain& d = S.d;
+    d.ox = sd->grid_origin[0];
+    d.oy = sd->grid_origin[1];
+    d.oz = sd->grid_origin[2];
+    d.nx = (sd->grid_dims[0] + 3) & ~3;
+    d.ny = (sd->grid_dims[1] + 3) & ~3;
+    d.nz = (sd->grid_dims[2] + 3) & ~3;
+    d.nbx = d.nx / 4;
+    d.nby = d.ny / 4;
+    d.nbz = d.nz / 4;
+    d.dx = sd->dx;
+    d.inv_dx = 1.0f / sd->dx;
+    d.b0 = b->B;
+    S.nb = (int64_t)d.nbx * d.nby * d.nbz;
+    if ((int)b->scenes.size() >= b->capS) return fail(BM_ERR_OOM, "batch scene capacity %d reached", b->capS);
+    if (b->P + n > b->capP) return fail(BM_ERR_OOM, "batch particle capacity %lld exceeded", (long long)b->capP);
+    if (b->B + S.nb > b->capB) return fail(BM_ERR_OOM, "batch block capacity %lld exceeded", (long long)b->capB);
+    S.p0 = b->P;
+    S.n = n;
+    S.q_exp = q_exp;
+    S.has_body = body != nullptr;
+    return BM_OK;
+}
+
+// Admission, second half (the particles are on their way into the arena on
+// the batch stream): body, reaction slot, tables.
+static int commit_scene(bm_batch_handle b, SceneHost& S, const bm_body* body, int32_t* scene_id) {
+    S.id = b->next_id++;
+    const int s = (int)b->scenes.size();
+    if (body) HIP_TRY(hipMemcpyAsync(b->d_bodies + s, body, sizeof(bm_body), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemsetAsync(b->d_out7 + (int64_t)s * 8, 0, sizeof(double) * 8, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));  // the caller's buffers and the staging are free again
+    b->scenes.push_back(S);
+    b->P += S.n;
+    b->B += S.nb;
+    b->invalidate();
+    if (int rc = upload_tables(b)) return rc;
+    *scene_id = S.id;
+    return BM_OK;
+}
+
+namespace bm {
+
+hipStream_t batch_stream(bm_batch_handle b) { return b->stream; }
+
+int batch_quantum_exp(const float* m, int64_t n) { return n > 0 ? quantum_exp_of(m, n) : 0; }
+
+int batch_add_scene_staged(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, int q_exp, const float* dev_soa,
+                           hipEvent_t ready, const bm_body* body, int32_t* scene_id) {
+    if (!b || !sd || !scene_id || n < 0 || (n > 0 && !dev_soa)) return fail(BM_ERR_INVALID_ARG, "null argument");
+    SceneHost S{};
+    if (int rc = place_scene(b, sd, n, q_exp, body, S)) return rc;
+    if (n > 0) {
+        if (ready) HIP_TRY(hipStreamWaitEvent(b->stream, ready, 0));
+        HIP_TRY(hipMemcpy2DAsync(b->pbuf + S.p0, sizeof(float) * b->capP, dev_soa, sizeof(float) * n,
+                                 sizeof(float) * n, PCOMP, hipMemcpyDeviceToDevice, b->stream));
+        hipLaunchKernelGGL(k_iota_from, dim3(grid256(n)), dim3(256), 0, b->stream, b->pid + S.p0, n, S.p0);
+        HIP_TRY(hipGetLastError());
+    }
+    return commit_scene(b, S, body, scene_id);
+}
+
+int batch_take_scene(bm_batch_handle b, int32_t id, float* dev_soa, int* dev_order, int64_t* n_out,
+                     int64_t* base_out) {
+    if (!b || !n_out || !base_out) return fail(BM_ERR_INVALID_ARG, "null argument");
+    const int s = b->find(id);
+    if (s < 0) return fail(BM_ERR_INVALID_ARG, "no scene %d", id);
+    if (int rc = batch_check_err(b, s)) return rc;
+    const SceneHost& S = b->scenes[s];
+    *n_out = S.n;
+    *base_out = S.p0;
+    if (S.n == 0) return BM_OK;
+    if (!dev_soa || !dev_order) return fail(BM_ERR_INVALID_ARG, "null staging");
+    // as bm_batch_download_scene, with the copies staying on the device
+    launch_unpermute(b->pview(), b->pbuf2, b->pid, S.p0, S.n, b->stream);
+    HIP_TRY(hipMemcpyAsync(dev_order, b->pid + S.p0, sizeof(int) * S.n, hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpy2DAsync(dev_soa, sizeof(float) * S.n, b->pbuf2 + S.p0, sizeof(float) * b->capP,
+                             sizeof(float) * S.n, PCOMP, hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipGetLastError());
+    return BM_OK;
+}
+
+}  // namespace bm
+
 extern "C" {
 
 int bm_batch_footprint(int64_t particles, int64_t blocks, int32_t scenes, int64_t* bytes) {
@@ -455,37 +552,8 @@ int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, co
     if (!sd || !scene_id || n < 0 || (n > 0 && !p)) return fail(BM_ERR_INVALID_ARG, "null argument");
     if (n > 0 && (!p->x || !p->v || !p->F || !p->C || !p->m || !p->V0))
         return fail(BM_ERR_INVALID_ARG, "null particle field");
-    if (!(sd->dx > 0.0f) || !std::isfinite(sd->dx)) return fail(BM_ERR_INVALID_ARG, "dx must be > 0");
-    for (int a = 0; a < 3; ++a)
-        if (sd->grid_dims[a] <= 0) return fail(BM_ERR_INVALID_ARG, "grid_dims must be > 0");
-    if (body && (sd->coupler < BM_COUPLER_M0_DIRICHLET || sd->coupler > BM_COUPLER_M45_BSPLINE))
-        return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", sd->coupler);
-    if (body && body->shape != BM_SHAPE_BOX && body->shape != BM_SHAPE_CYLINDER)
-        return fail(BM_ERR_INVALID_ARG, "unknown shape %d", body->shape);
     SceneHost S{};
-    S.desc = *sd;
-    Domain& d = S.d;
-    d.ox = sd->grid_origin[0];
-    d.oy = sd->grid_origin[1];
-    d.oz = sd->grid_origin[2];
-    d.nx = (sd->grid_dims[0] + 3) & ~3;
-    d.ny = (sd->grid_dims[1] + 3) & ~3;
-    d.nz = (sd->grid_dims[2] + 3) & ~3;
-    d.nbx = d.nx / 4;
-    d.nby = d.ny / 4;
-    d.nbz = d.nz / 4;
-    d.dx = sd->dx;
-    d.inv_dx = 1.0f / sd->dx;
-    d.b0 = b->B;
-    S.nb = (int64_t)d.nbx * d.nby * d.nbz;
-    if ((int)b->scenes.size() >= b->capS) return fail(BM_ERR_OOM, "batch scene capacity %d reached", b->capS);
-    if (b->P + n > b->capP) return fail(BM_ERR_OOM, "batch particle capacity %lld exceeded", (long long)b->capP);
-    if (b->B + S.nb > b->capB) return fail(BM_ERR_OOM, "batch block capacity %lld exceeded", (long long)b->capB);
-    S.p0 = b->P;
-    S.n = n;
-    S.q_exp = n > 0 ? quantum_exp_of(p->m, n) : 0;
-    S.has_body = body != nullptr;
-    S.id = b->next_id++;
+    if (int rc = place_scene(b, sd, n, n > 0 ? quantum_exp_of(p->m, n) : 0, body, S)) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (n > 0) {
         // component-major pinned staging, then one pitched copy into the arena
@@ -498,17 +566,7 @@ int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, co
         hipLaunchKernelGGL(k_iota_from, dim3(grid256(n)), dim3(256), 0, b->stream, b->pid + S.p0, n, S.p0);
         HIP_TRY(hipGetLastError());
     }
-    const int s = (int)b->scenes.size();
-    if (body) HIP_TRY(hipMemcpyAsync(b->d_bodies + s, body, sizeof(bm_body), hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(hipMemsetAsync(b->d_out7 + (int64_t)s * 8, 0, sizeof(double) * 8, b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));  // the caller's buffers and the staging are free again
-    b->scenes.push_back(S);
-    b->P += n;
-    b->B += S.nb;
-    b->invalidate();
-    if (int rc = upload_tables(b)) return rc;
-    *scene_id = S.id;
-    return BM_OK;
+    return commit_scene(b, S, body, scene_id);
 }
 
 int bm_batch_remove_scene(bm_batch_handle b, int32_t id) {

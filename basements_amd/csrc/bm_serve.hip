@@ -6,28 +6,44 @@
 // serving (its solver holds one scene and runs on one thread).
 //
 // Scheduling, one bm_server_poll:
-//   1. admit: walk the queue in arrival order; a request whose arrival has
+//   1. wait for the chunk launched by the previous poll; meanwhile, requests
+//      that have arrived but are still queued are transposed into pinned
+//      memory and uploaded into the server's device staging area on a copy
+//      stream, so that their admission is a device-to-device copy.  Then
+//      retire: requests with no substeps left leave the batch -- their
+//      particle state is first copied, in insertion order, into the staging
+//      area -- and a request whose scene raised a device error is retired as
+//      failed.
+//   2. admit: walk the queue in arrival order; a request whose arrival has
 //      come and that fits is added to the batch.  A later request may pass an
 //      older one that does not fit, at most max_bypass times; then the queue
 //      waits for the oldest.
-//   2. step: the batch advances min(max_chunk, fewest remaining substeps of
-//      the running requests) substeps, so that no request overshoots.
-//   3. retire: requests with no substeps left are copied back to their
-//      output buffers (and body), removed from the batch, and reported done;
-//      a request whose scene raised a device error is retired as failed.
-// With nothing running, a poll sleeps until the next arrival.
+//   3. step: the batch is launched for min(max_chunk, fewest remaining
+//      substeps of the running requests) substeps, so that no request
+//      overshoots; the poll does not wait for it.
+//   4. while that chunk runs on the device: the retired requests' state
+//      comes down to pinned host memory on the copy stream and is transposed
+//      into their output buffers; then they are reported done.
+// With nothing running or in flight, a poll sleeps until the next arrival.
+// With overlap off (bm_server_set_overlap), every step waits for the device
+// in the poll that launched it and admissions and retirements go through the
+// batch's synchronous host paths.
 //
 // Host-only code over the batch entry points.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
+#include <limits>
 #include <deque>
+#include <map>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
+#include "host_transpose.hpp"
 #include "mpm_internal.hpp"
 
 using namespace bm;
@@ -44,7 +60,61 @@ struct Request {
     int32_t left = 0;    // substeps still to run
     int64_t bytes = 0;   // budget charge while running
     int64_t blocks = 0;
+    // input staged ahead of admission (overlap on): staging range, upload
+    // event, mass quantum; or a non-finite input found while staging
+    int64_t in_off = -1, in_len = 0;
+    hipEvent_t in_ready = nullptr;
+    int q_exp = 0;
+    bool in_nonfinite = false;
 };
+
+// A retired request whose state is on its way to its output buffer.
+struct Retiring {
+    int64_t id;
+    int64_t off, len, n, base;
+    hipEvent_t landed;  // the download to pinned memory has finished
+};
+
+// First-fit ranges of the staging area, in 256-byte granules.
+struct RangeAlloc {
+    std::map<int64_t, int64_t> free_;  // offset -> length
+    static int64_t round(int64_t b) { return (b + 255) & ~(int64_t)255; }
+    void init(int64_t size) {
+        free_.clear();
+        if (size > 0) free_[0] = size;
+    }
+    int64_t take(int64_t len) {
+        len = round(len);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < len) continue;
+            const int64_t off = it->first, rest = it->second - len;
+            free_.erase(it);
+            if (rest > 0) free_[off + len] = rest;
+            return off;
+        }
+        return -1;
+    }
+    void give(int64_t off, int64_t len) {
+        len = round(len);
+        auto it = free_.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+// Staging bytes of n particles: the component-major state, then the order.
+int64_t soa_bytes(int64_t n) { return RangeAlloc::round((int64_t)sizeof(float) * PCOMP * n); }
+int64_t stage_len(int64_t n) { return soa_bytes(n) + RangeAlloc::round((int64_t)sizeof(int) * n); }
 
 int64_t blocks_of(const bm_scene_desc& d) {
     int64_t nb = 1;
@@ -71,6 +141,17 @@ struct bm_server {
     double t_admit = 0.0, t_step = 0.0, t_retire = 0.0, t_idle = 0.0;
     int64_t chunks = 0, substeps = 0;
 
+    // overlap of staging with stepping
+    bool overlap = true;
+    hipStream_t cs = nullptr;          // copy stream (default priority)
+    hipEvent_t chunk_done = nullptr;   // recorded after each launched chunk
+    bool in_flight = false;            // a launched chunk not yet waited for
+    char* dstage = nullptr;            // device staging area
+    char* hstage = nullptr;            // pinned host staging, same offsets
+    int64_t stage_bytes = 0;
+    RangeAlloc ranges;
+    std::vector<Retiring> retiring;
+
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
@@ -89,7 +170,19 @@ static void enqueue_sorted(bm_server_handle s, int64_t id) {
     s->queue.insert(it, id);
 }
 
-// Step 1: admissions.
+// Release a request's staged input (waits for its upload first).
+static void drop_staged(bm_server_handle s, Request& R) {
+    if (R.in_ready) {
+        hipEventSynchronize(R.in_ready);
+        hipEventDestroy(R.in_ready);
+        R.in_ready = nullptr;
+    }
+    if (R.in_off >= 0) s->ranges.give(R.in_off, R.in_len);
+    R.in_off = -1;
+    R.in_len = 0;
+}
+
+// Admissions.
 static int admit(bm_server_handle s) {
     const double t = s->now();
     for (size_t qi = 0; qi < s->queue.size();) {
@@ -103,8 +196,17 @@ static int admit(bm_server_handle s) {
         if (!is_oldest && s->bypassed >= s->desc.max_bypass) break;
         int rc = BM_ERR_OOM;
         if (s->in_use + R.bytes <= s->budget) {
-            rc = bm_batch_add_scene(s->batch, &R.req.scene, R.req.n, R.req.input, R.has_body ? &R.body : nullptr,
-                                    &R.scene);
+            if (R.in_nonfinite) {
+                rc = fail(BM_ERR_NONFINITE, "particle state holds a non-finite value");
+            } else if (R.in_off >= 0) {  // staged: the state is on the device already
+                rc = batch_add_scene_staged(s->batch, &R.req.scene, R.req.n, R.q_exp,
+                                            reinterpret_cast<const float*>(s->dstage + R.in_off), R.in_ready,
+                                            R.has_body ? &R.body : nullptr, &R.scene);
+                if (rc != BM_ERR_OOM) drop_staged(s, R);  // used (the batch stream has synced) or failed
+            } else {
+                rc = bm_batch_add_scene(s->batch, &R.req.scene, R.req.n, R.req.input,
+                                        R.has_body ? &R.body : nullptr, &R.scene);
+            }
             if (rc == BM_ERR_HIP) return rc;
         }
         if (rc == BM_ERR_OOM) {  // no room now: wait (the request fits an empty server)
@@ -130,7 +232,9 @@ static int admit(bm_server_handle s) {
     return BM_OK;
 }
 
-// Step 3: retirements (and device failures).
+// Retirements (and device failures).  With overlap on, a finished request's
+// state goes to the staging area and starts down to pinned memory on the
+// copy stream; finish_retirements writes its output later.
 static int retire(bm_server_handle s) {
     for (size_t i = 0; i < s->running.size();) {
         Request& R = s->reqs[s->running[i]];
@@ -140,23 +244,133 @@ static int retire(bm_server_handle s) {
             ++i;
             continue;
         }
+        bool staged = false;
+        Retiring T{R.id, -1, 0, 0, 0, nullptr};
         if (code == BM_OK) {
-            if (R.req.output)
+            if (R.req.output && s->overlap && s->stage_bytes > 0 && R.req.n > 0) {
+                T.len = stage_len(R.req.n);
+                T.off = s->ranges.take(T.len);
+            }
+            if (T.off >= 0) {
+                if (int rc = batch_take_scene(s->batch, R.scene, reinterpret_cast<float*>(s->dstage + T.off),
+                                              reinterpret_cast<int*>(s->dstage + T.off + soa_bytes(R.req.n)), &T.n,
+                                              &T.base)) {
+                    s->ranges.give(T.off, T.len);
+                    return rc;
+                }
+                staged = true;
+            } else if (R.req.output) {
                 if (int rc = bm_batch_download_scene(s->batch, R.scene, R.req.output)) return rc;
+            }
             if (R.has_body)
                 if (int rc = bm_batch_get_body(s->batch, R.scene, &R.info.body, &R.info.reaction)) return rc;
-            R.info.state = BM_REQ_DONE;
         } else {
             R.info.state = BM_REQ_FAILED;
             R.info.status = code;
         }
-        if (int rc = bm_batch_remove_scene(s->batch, R.scene)) return rc;
+        if (int rc = bm_batch_remove_scene(s->batch, R.scene)) return rc;  // syncs the batch stream
         R.info.finished = s->now();
         s->in_use -= R.bytes;
         R.scene = -1;
-        s->done.push_back(R.id);
         s->running.erase(s->running.begin() + (std::ptrdiff_t)i);
+        if (staged) {
+            HIP_TRY(hipMemcpyAsync(s->hstage + T.off, s->dstage + T.off, (size_t)T.len, hipMemcpyDeviceToHost, s->cs));
+            HIP_TRY(hipEventCreateWithFlags(&T.landed, hipEventDisableTiming));
+            HIP_TRY(hipEventRecord(T.landed, s->cs));
+            s->retiring.push_back(T);
+        } else {
+            if (code == BM_OK) R.info.state = BM_REQ_DONE;
+            s->done.push_back(R.id);
+        }
     }
+    return BM_OK;
+}
+
+// Outputs of the staged retirements: wait for each download, transpose into
+// the caller's buffers, report done.
+static int finish_retirements(bm_server_handle s) {
+    for (Retiring& T : s->retiring) {
+        Request& R = s->reqs[T.id];
+        HIP_TRY(hipEventSynchronize(T.landed));
+        hipEventDestroy(T.landed);
+        const float* st = reinterpret_cast<const float*>(s->hstage + T.off);
+        const int* order = reinterpret_cast<const int*>(s->hstage + T.off + soa_bytes(T.n));
+        const bool ok = order_in_range(order, T.n, T.base);
+        if (ok) particles_from_soa(st, T.n, nullptr, 0, R.req.output);
+        s->ranges.give(T.off, T.len);
+        if (ok) {
+            R.info.state = BM_REQ_DONE;
+        } else {
+            R.info.state = BM_REQ_FAILED;
+            R.info.status = fail(BM_ERR_HIP, "particle order corrupted");
+        }
+        s->done.push_back(T.id);
+    }
+    s->retiring.clear();
+    return BM_OK;
+}
+
+// Stage the first queued request that has arrived and is not staged yet:
+// transpose its input into pinned memory and upload it into the staging area
+// on the copy stream.  *did: one was staged.  *next: arrival time of the
+// first queued request still to come (infinity when none, or when the
+// staging area is full).
+static int stage_one(bm_server_handle s, bool* did, double* next) {
+    *did = false;
+    *next = std::numeric_limits<double>::infinity();
+    if (!s->overlap || s->stage_bytes == 0) return BM_OK;
+    const double t = s->now();
+    for (int64_t id : s->queue) {
+        Request& R = s->reqs[id];
+        if (R.info.arrival > t) {
+            *next = R.info.arrival;
+            return BM_OK;
+        }
+        if (R.in_off >= 0 || R.in_nonfinite) continue;
+        const bm_particles* p = R.req.input;
+        if (!p->x || !p->v || !p->F || !p->C || !p->m || !p->V0) continue;  // left to the batch's own checks
+        const int64_t len = soa_bytes(R.req.n);
+        const int64_t off = s->ranges.take(len);
+        if (off < 0) return BM_OK;  // full: the rest are admitted the synchronous way
+        float* st = reinterpret_cast<float*>(s->hstage + off);
+        *did = true;
+        if (!soa_from_particles(p->x, p->v, p->F, p->C, p->m, p->V0, R.req.n, st)) {
+            s->ranges.give(off, len);
+            R.in_nonfinite = true;
+            return BM_OK;
+        }
+        R.q_exp = batch_quantum_exp(p->m, R.req.n);
+        HIP_TRY(hipMemcpyAsync(s->dstage + off, st, (size_t)len, hipMemcpyHostToDevice, s->cs));
+        HIP_TRY(hipEventCreateWithFlags(&R.in_ready, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(R.in_ready, s->cs));
+        R.in_off = off;
+        R.in_len = len;
+        return BM_OK;
+    }
+    return BM_OK;
+}
+
+// Wait for the chunk in flight, staging arrivals meanwhile: requests that
+// have arrived are staged one at a time; with none waiting, the host checks
+// the chunk every 50 us until the next arrival, or blocks on it when no
+// arrival is due.
+static int wait_chunk(bm_server_handle s) {
+    if (!s->in_flight) return BM_OK;
+    const double t = s->now();
+    for (;;) {
+        const hipError_t q = hipEventQuery(s->chunk_done);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) HIP_TRY(q);
+        bool did = false;
+        double next = 0.0;
+        if (int rc = stage_one(s, &did, &next)) return rc;
+        if (did) continue;
+        if (!std::isfinite(next)) break;
+        std::this_thread::sleep_for(std::chrono::duration<double>(std::min(std::max(next - s->now(), 0.0), 50e-6)));
+    }
+    HIP_TRY(hipEventSynchronize(s->chunk_done));
+    s->in_flight = false;
+    s->t_step += s->now() - t;
     return BM_OK;
 }
 
@@ -179,6 +393,27 @@ int bm_server_create(const bm_server_desc* desc, bm_server_handle* out) {
     int64_t arenas = 0;
     bm_batch_footprint(desc->batch.max_particles, desc->batch.max_blocks, desc->batch.max_scenes, &arenas);
     s->budget = desc->hbm_budget > 0 ? std::min(desc->hbm_budget, arenas) : arenas;
+    // staging: room for the whole particle capacity once (state + order), in
+    // device memory and in pinned host memory; the copy stream keeps the
+    // default priority.  Without it, the synchronous paths serve.
+    if (hipStreamCreateWithFlags(&s->cs, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->chunk_done, hipEventDisableTiming) != hipSuccess) {
+        bm_server_destroy(s);
+        return fail(BM_ERR_HIP, "server stream or event");
+    }
+    const int64_t want = stage_len(desc->batch.max_particles) + ((int64_t)1 << 20);
+    void* d = nullptr;
+    void* h = nullptr;
+    if (hipMalloc(&d, (size_t)want) == hipSuccess && hipHostMalloc(&h, (size_t)want, hipHostMallocDefault) == hipSuccess) {
+        s->dstage = static_cast<char*>(d);
+        s->hstage = static_cast<char*>(h);
+        s->stage_bytes = want;
+        s->ranges.init(want);
+    } else {
+        (void)hipGetLastError();
+        if (d) hipFree(d);
+        s->overlap = false;
+    }
     s->t0 = std::chrono::steady_clock::now();
     *out = s;
     return BM_OK;
@@ -186,8 +421,26 @@ int bm_server_create(const bm_server_desc* desc, bm_server_handle* out) {
 
 int bm_server_destroy(bm_server_handle s) {
     if (!s) return BM_OK;
+    if (s->cs) hipStreamSynchronize(s->cs);
+    for (auto& kv : s->reqs)
+        if (kv.second.in_ready) hipEventDestroy(kv.second.in_ready);
+    for (Retiring& T : s->retiring) hipEventDestroy(T.landed);
     bm_batch_destroy(s->batch);
+    if (s->chunk_done) hipEventDestroy(s->chunk_done);
+    if (s->cs) hipStreamDestroy(s->cs);
+    if (s->dstage) hipFree(s->dstage);
+    if (s->hstage) hipHostFree(s->hstage);
     delete s;
+    return BM_OK;
+}
+
+int bm_server_set_overlap(bm_server_handle s, int32_t on) {
+    if (int rc = check_server(s)) return rc;
+    if (on && s->stage_bytes == 0) return fail(BM_ERR_OOM, "no staging area (its allocation failed)");
+    s->overlap = on != 0;
+    if (int rc = wait_chunk(s)) return rc;  // the next poll starts from a settled device either way
+    if (!on)
+        for (int64_t id : s->queue) drop_staged(s, s->reqs[id]);
     return BM_OK;
 }
 
@@ -238,9 +491,17 @@ int bm_server_submit(bm_server_handle s, const bm_request* req, int64_t* id) {
 int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int32_t* n_done) {
     if (int rc = check_server(s)) return rc;
     if (capacity < 0 || (capacity > 0 && !done_ids)) return fail(BM_ERR_INVALID_ARG, "bad done-id buffer");
+    // 1. the chunk in flight ends; finished requests leave the batch
+    if (int rc = wait_chunk(s)) return rc;
+    if (!s->running.empty()) {
+        const double t = s->now();
+        if (int rc = retire(s)) return rc;
+        s->t_retire += s->now() - t;
+    }
+    // 2. admissions
     double t = s->now();
     if (int rc = admit(s)) return rc;
-    if (s->running.empty() && !s->queue.empty() && s->done.empty()) {
+    if (s->running.empty() && !s->queue.empty() && s->done.empty() && s->retiring.empty()) {
         // idle: wait for the next arrival, then admit it
         const double wait = s->reqs[s->queue.front()].info.arrival - s->now();
         const double t1 = s->now();
@@ -249,20 +510,29 @@ int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int3
         if (int rc = admit(s)) return rc;
     }
     s->t_admit += s->now() - t;
+    // 3. the next chunk
     if (!s->running.empty()) {
         int32_t k = s->desc.max_chunk;
         for (int64_t id : s->running) k = std::min(k, s->reqs[id].left);
         t = s->now();
         if (int rc = bm_batch_step(s->batch, s->desc.dt, k)) return rc;
-        int32_t code = 0;  // waits for the device (per-scene errors are read at retirement)
-        if (int rc = bm_batch_scene_error(s->batch, s->reqs[s->running.front()].scene, &code)) return rc;
-        s->t_step += s->now() - t;
+        HIP_TRY(hipEventRecord(s->chunk_done, batch_stream(s->batch)));
+        s->in_flight = true;
         ++s->chunks;
         s->substeps += k;
         for (int64_t id : s->running) {
             s->reqs[id].left -= k;
             s->reqs[id].info.substeps_done += k;
         }
+        s->t_step += s->now() - t;
+    }
+    // 4. while it runs: outputs of the retired requests (arrivals are staged
+    // while the next poll waits for the chunk)
+    t = s->now();
+    if (int rc = finish_retirements(s)) return rc;
+    (s->in_flight ? s->t_step : s->t_retire) += s->now() - t;
+    if (!s->overlap && s->in_flight) {  // synchronous form: the chunk and its retirements end in this poll
+        if (int rc = wait_chunk(s)) return rc;
         t = s->now();
         if (int rc = retire(s)) return rc;
         s->t_retire += s->now() - t;
@@ -291,7 +561,10 @@ int bm_server_cancel(bm_server_handle s, int64_t id) {
     if (it == s->reqs.end()) return fail(BM_ERR_INVALID_ARG, "no request %lld", (long long)id);
     Request& R = it->second;
     if (R.info.state == BM_REQ_QUEUED) {
+        drop_staged(s, R);
         s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
+    } else if (R.info.state == BM_REQ_RUNNING && R.scene < 0) {
+        return fail(BM_ERR_STATE, "request %lld has already finished (its output is being written)", (long long)id);
     } else if (R.info.state == BM_REQ_RUNNING) {
         if (int rc = bm_batch_remove_scene(s->batch, R.scene)) return rc;
         s->in_use -= R.bytes;

@@ -302,4 +302,18 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   void* ctx = nullptr, bool two_pass = false, const SplitBatch& batch = SplitBatch{},
                   bool store_vc = true);
 
+// Batch entry points of the request loop (bm_batch.hip): admission of a scene
+// whose component-major particle state (PCOMP rows of n floats) is already on
+// the device, or on its way there behind `ready`; and the copy of a scene's
+// state, in insertion order, into device staging on the batch stream, with
+// nothing waiting (the scene stays in the batch).
+hipStream_t batch_stream(bm_batch_handle b);
+int batch_quantum_exp(const float* m, int64_t n);
+int batch_add_scene_staged(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, int q_exp, const float* dev_soa,
+                           hipEvent_t ready, const bm_body* body, int32_t* scene_id);
+// base_out: the scene's first arena slot; the staged order entries are
+// insertion index + base.
+int batch_take_scene(bm_batch_handle b, int32_t id, float* dev_soa, int* dev_order, int64_t* n_out,
+                     int64_t* base_out);
+
 }  // namespace bm

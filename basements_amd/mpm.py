@@ -263,6 +263,7 @@ _SIGS = {
     "bm_server_info": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestInfoC)]),
     "bm_server_cancel": (C.c_int, [C.c_void_p, C.c_int64]),
     "bm_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "bm_server_set_overlap": (C.c_int, [C.c_void_p, C.c_int32]),
     "bm_server_load": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64)]),
     "bm_diff_create": (C.c_int, [C.POINTER(DiffDescC), C.POINTER(C.c_void_p)]),
@@ -823,7 +824,7 @@ class Server:
     The Python side keeps every request's input and output arrays alive."""
 
     def __init__(self, max_particles, max_blocks, max_scenes, dt, hbm_budget=0, max_chunk=0, max_bypass=0,
-                 stream=None):
+                 stream=None, overlap=True):
         d = ServerDescC()
         d.batch.max_particles, d.batch.max_blocks, d.batch.max_scenes = int(max_particles), int(max_blocks), \
             int(max_scenes)
@@ -833,6 +834,12 @@ class Server:
         _check(lib().bm_server_create(C.byref(d), C.byref(h)))
         self.h = h
         self._keep = {}  # id -> (input structs and arrays, output arrays, has_body)
+        if not overlap:
+            self.set_overlap(False)
+
+    def set_overlap(self, on):
+        """Overlap host staging with stepping (bm_server_set_overlap; on by default)."""
+        _check(lib().bm_server_set_overlap(self.h, int(bool(on))))
 
     def close(self):
         if getattr(self, "h", None):

@@ -426,18 +426,32 @@ int bm_server_clock(bm_server_handle s, double* seconds);
 int bm_server_batch(bm_server_handle s, bm_batch_handle* batch);
 /* Queue a request.  BM_ERR_OOM when it could not fit even an empty server. */
 int bm_server_submit(bm_server_handle s, const bm_request* req, int64_t* request_id);
-/* One scheduling round: admit, advance one chunk, retire.  Sleeps until the
- * next arrival when nothing is running and nothing is left to report.
- * Writes up to `capacity` ids of requests finished (done or failed) since
- * the last report; the rest are reported by later calls. */
+/* One scheduling round: the chunk launched by the previous round ends and
+ * the requests it finished leave the batch; admissions; the next chunk is
+ * launched and not waited for; while it runs, the finished requests' outputs
+ * are written and arrived requests are staged onto the device for their
+ * admission (overlap on, the default).  Sleeps until the next arrival when
+ * nothing is running and nothing is left to report.  Writes up to
+ * `capacity` ids of requests finished (done or failed) since the last
+ * report; the rest are reported by later calls.  A request's output is
+ * written when its id is reported; `finished` in its info is the time it
+ * left the batch.  substeps_done counts launched substeps. */
 int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int32_t* n_done);
+/* Overlap of staging with stepping (on by default): a copy stream and a
+ * staging area sized for the batch's particle capacity (108 bytes per
+ * particle, in device memory and in pinned host memory, outside hbm_budget).
+ * Off: each poll waits for its chunk, and admissions and retirements use the
+ * batch's synchronous paths.  BM_ERR_OOM when the staging area could not be
+ * allocated at creation. */
+int bm_server_set_overlap(bm_server_handle s, int32_t on);
 int bm_server_info(bm_server_handle s, int64_t request_id, bm_request_info* info);
 /* Drop a queued or running request (its output is not written). */
 int bm_server_cancel(bm_server_handle s, int64_t request_id);
 int bm_server_load(bm_server_handle s, int32_t* queued, int32_t* running, int64_t* bytes_in_use, int64_t* budget);
 /* Where the time went: seconds[4] = admitting, stepping (launch until the
- * device is idle), retiring, sleeping for arrivals; counts[2] = chunks and
- * substeps run.  Either may be NULL. */
+ * device is idle, including the host staging done meanwhile), retiring,
+ * sleeping for arrivals; counts[2] = chunks and substeps run.  Either may be
+ * NULL. */
 int bm_server_stats(bm_server_handle s, double* seconds, int64_t* counts);
 
 /* ---------------------------------------------------------------- adjoint */

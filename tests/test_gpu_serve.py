@@ -156,3 +156,34 @@ def test_request_with_non_finite_state_fails_alone():
     inf = S.info(failing)
     assert inf.state == REQ_FAILED and inf.status == 4  # BM_ERR_NONFINITE
     _check_alone(S, good, scenes[0], 8, "request beside a non-finite one")
+
+
+@pytest.mark.parametrize("overlap", [True, False], ids=["overlap", "synchronous"])
+def test_queued_requests_staged_while_the_device_runs(overlap):
+    """A budget that holds one request at a time keeps the others queued
+    while a chunk runs: with overlap on they are staged onto the device then
+    (a non-finite one is caught there) and admitted from the staging area;
+    retirements go out through it.  Both forms give each request's scene
+    alone, and the non-finite request fails alone with BM_ERR_NONFINITE."""
+    from basements_amd import REQ_FAILED, batch_footprint
+
+    scenes = _scenes()
+    one = max(batch_footprint(sc[0]["x"].shape[0], np.prod([(d + 3) // 4 for d in sc[2].dims]), 1)
+              for sc in scenes)
+    S = _server(budget=int(one * 1.01), max_chunk=4)
+    S.set_overlap(overlap)
+    bad = {k: np.array(v, copy=True) for k, v in scenes[1][0].items()}
+    bad["F"][3, 1, 1] = np.inf
+    counts = [9, 6, 7, 11]
+    ids = [S.submit(scenes[0][2], scenes[0][0], counts[0], body=scenes[0][3]),
+           S.submit(scenes[1][2], bad, counts[1], body=scenes[1][3]),
+           S.submit(scenes[1][2], scenes[1][0], counts[2], body=scenes[1][3]),
+           S.submit(scenes[2][2], scenes[2][0], counts[3])]
+    most = _drain(S, ids)
+    assert most == 1
+    inf = S.info(ids[1])
+    assert inf.state == REQ_FAILED and inf.status == 4, inf
+    for i in (0, 2, 3):
+        sc = scenes[{0: 0, 2: 1, 3: 2}[i]]
+        _check_alone(S, ids[i], sc, counts[i], f"{'staged' if overlap else 'synchronous'} request {i}")
+    assert S.load()[:3] == (0, 0, 0)
