@@ -262,10 +262,20 @@ __device__ __forceinline__ m3<float> recompose(const m3<float>& U, v3<float> s, 
 // integer mass is nonzero.
 // Quanta of one node: max(1, rint(w q_p)) for w != 0.  q_p < 2^23 and
 // w <= 1, so the double product is exact and the result fits 32 bits.
-__device__ __forceinline__ int mass_quanta(float w, double qp) {
-    if (w == 0.0f) return 0;
-    const int q = (int)rint((double)w * qp);
-    return q < 1 ? 1 : q;
+// Fixed-point mass of one stencil node: w * q_p rounded to the nearest
+// integer in FP32 -- floor(fma(w, q_p, 1/2)), the sum rounded once (q_p <
+// 2^23 is exact in FP32; w * q_p + 1/2 < 2^22) -- and at least one quantum
+// on every node with w > 0 (w >= 0, so its bit pattern is nonzero exactly
+// then), none where w = 0.  The stencil centre takes the particle's
+// remainder, so each particle deposits exactly q_p.  Unbiased: rounding up
+// every node would move mass from each stencil's centre to its rim, which
+// cancels in the interior but not at a free surface.  Formed as
+// median(w q_p + 1/2, w 2^100, 1): for w = 0 that is 1/2 (no quantum); for
+// w > 0 (a stencil's nonzero weights are >= 2^-60) w 2^100 exceeds both
+// others, so the median is max(w q_p + 1/2, 1).  Four instructions (fma,
+// multiply, med3, convert).
+__device__ __forceinline__ int mass_quanta(float w, float qp) {
+    return (int)__builtin_amdgcn_fmed3f(__builtin_fmaf(w, qp, 0.5f), w * 0x1p100f, 1.0f);
 }
 
 }  // namespace bm

@@ -239,7 +239,7 @@ struct NodeAxes {
 // P2G scatter of one particle's mass quanta and folded momentum
 // w (m v + (m C + dt K) d) into 27 nodes; SINK(a, b, c, dq, mom) deposits.
 template <class SinkFn>
-__device__ __forceinline__ void scatter(const Stencil& s, float dx, double qp, const v3<float>& mvel,
+__device__ __forceinline__ void scatter(const Stencil& s, float dx, float qp, const v3<float>& mvel,
                                         const m3<float>& Aff, SinkFn sink) {
     // the stencil centre (1,1,1) is deposited last and takes the remainder,
     // so each other node's quanta are used as soon as they are formed

@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) k_p2g_ref(PView P, GridBuf g, uint8_t* be
     for (int c = 0; c < 9; ++c) K.a[c] = tau.a[c] * ff;
 
     // integer quanta of this particle; the stencil centre takes the remainder
-    const double qp = rint((double)mass * qscale_inv);
+    const float qp = (float)rint((double)mass * qscale_inv);
     long long rest = (long long)qp;
 #pragma unroll
     for (int a = 0; a < 27; ++a) {

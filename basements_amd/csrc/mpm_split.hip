@@ -46,7 +46,7 @@ __device__ __forceinline__ void store_rec(float4* rec, int64_t i, float x, float
 }
 struct DepositRec {
     float x, y, z;
-    double qp;
+    float qp;
     v3<float> mvel;
     m3<float> A;
 };
@@ -56,7 +56,7 @@ __device__ __forceinline__ DepositRec load_rec(const float4* __restrict__ rec, i
     d.x = r0.x;
     d.y = r0.y;
     d.z = r0.z;
-    d.qp = (double)r0.w;
+    d.qp = r0.w;
     d.mvel = mk(r1.x, r1.y, r1.z);
     d.A = m3<float>{{r1.w, r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w}};
     return d;
@@ -273,9 +273,13 @@ struct CellSmem {
 
 // Deposits of slabs [A0, A1) of the stencils of one cell's particles,
 // accumulated in registers (node n = ((a - A0) * 3 + b) * 3 + c).  The
-// centre (1, 1, 1) takes the particle's remaining quanta, so when slab 1 is in
-// range the quanta of the other slabs are formed too.  Same arithmetic as
-// scatter() for the nodes it covers.
+// centre (1, 1, 1) takes each particle's remaining quanta.  With the whole
+// stencil in range the centre's quanta are formed once per cell, after the
+// loop: the cell's total quanta less the other 26 nodes' sums (every particle
+// deposits exactly q_p, so the per-particle remainders add up to that).  With
+// a part of the stencil, the quanta of the other slabs are formed per particle
+// when slab 1 is in range.  Same arithmetic as scatter() for the nodes it
+// covers.
 template <int A0, int A1>
 __device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, const int4& U, const unsigned short* ord,
                                             int s0, int mine, float dx, float inv_dx, v3<float>* acc,
@@ -286,6 +290,9 @@ __device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, cons
         accm[k] = 0u;
     }
     constexpr bool CENTRE = A0 <= 1 && 1 < A1;
+    constexpr bool FULL = A0 == 0 && A1 == 3;
+    constexpr int NC = ((1 - A0) * 3 + 1) * 3 + 1;  // the centre's register slot (when CENTRE)
+    unsigned qsum = 0u;                              // the cell's quanta (FULL)
     // the next particle's record is fetched while this one is deposited
     DepositRec nxt;
     if (mine > 0) nxt = load_rec(rec, (int64_t)U.y + ord[s0]);
@@ -296,9 +303,9 @@ __device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, cons
         make_stencil(r.x, r.y, r.z, inv_dx, s);
         const v3<float>& mvel = r.mvel;
         const m3<float>& Aff = r.A;
-        const double qp = r.qp;
+        const float qp = r.qp;
         int rest = 0;
-        if (CENTRE) {
+        if (CENTRE && !FULL) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 if (a >= A0 && a < A1) continue;
@@ -308,39 +315,42 @@ __device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, cons
                     for (int c = 0; c < 3; ++c) rest += mass_quanta(s.w[0][a] * s.w[1][b] * s.w[2][c], qp);
             }
         }
-        v3<float> mom_c = mk(0.f, 0.f, 0.f);
+        if (FULL) qsum += (unsigned)qp;
+        // node offsets ((b + a) - f - 1/2) dx formed as ((a - fr) - 1/2) dx, fr = f - b: the same
+        // values (fr is exact, and so are a - fr and the subtraction of 1/2, all below 2 in magnitude)
+        const float fr0 = s.f[0] - (float)s.b[0], fr1 = s.f[1] - (float)s.b[1], fr2 = s.f[2] - (float)s.b[2];
 #pragma unroll
         for (int a = A0; a < A1; ++a) {
-            const float ddx = ((float)(s.b[0] + a) - s.f[0] - 0.5f) * dx;
+            const float ddx = (((float)a - fr0) - 0.5f) * dx;
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
-                const float ddy = ((float)(s.b[1] + b) - s.f[1] - 0.5f) * dx;
+                const float ddy = (((float)b - fr1) - 0.5f) * dx;
                 const float px = mvel.x + Aff.a[0] * ddx + Aff.a[1] * ddy;
                 const float py = mvel.y + Aff.a[3] * ddx + Aff.a[4] * ddy;
                 const float pz = mvel.z + Aff.a[6] * ddx + Aff.a[7] * ddy;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float ddz = ((float)(s.b[2] + c) - s.f[2] - 0.5f) * dx;
+                    const float ddz = (((float)c - fr2) - 0.5f) * dx;
                     const float w = s.w[0][a] * s.w[1][b] * s.w[2][c];
                     const v3<float> mom =
                         mk(w * (px + Aff.a[2] * ddz), w * (py + Aff.a[5] * ddz), w * (pz + Aff.a[8] * ddz));
                     const int n = ((a - A0) * 3 + b) * 3 + c;
-                    if (a == 1 && b == 1 && c == 1) {
-                        mom_c = mom;
-                        continue;
-                    }
-                    const int dq = mass_quanta(w, qp);
-                    rest += dq;
                     acc[n] = acc[n] + mom;
+                    if (a == 1 && b == 1 && c == 1) continue;  // the centre's quanta: below
+                    const int dq = mass_quanta(w, qp);
+                    if (!FULL) rest += dq;
                     accm[n] += (unsigned)dq;
                 }
             }
         }
-        if (CENTRE) {
-            const int n = ((1 - A0) * 3 + 1) * 3 + 1;
-            acc[n] = acc[n] + mom_c;
-            accm[n] += (unsigned)((int)qp - rest);
-        }
+        if (CENTRE && !FULL) accm[NC] += (unsigned)((int)qp - rest);
+    }
+    if (FULL) {  // modulo 2^32: the true sum is below 2^32 (at most 512 particles of < 2^23 quanta)
+        unsigned others = 0u;
+#pragma unroll
+        for (int n = 0; n < 27; ++n)
+            if (n != NC) others += accm[n];
+        accm[NC] = qsum - others;
     }
 }
 
@@ -460,7 +470,7 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
         make_stencil(r.x, r.y, r.z, inv_dx, s);
         const v3<float>& mvel = r.mvel;
         const m3<float>& Aff = r.A;
-        const double qp = r.qp;
+        const float qp = r.qp;
         const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
         if (!stencil_ok(d, lx, ly, lz)) {
             flag_error(err, sv.sc ? U.w : -1, 1);

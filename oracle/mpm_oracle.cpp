@@ -323,13 +323,12 @@ template <class R> struct Sim {
                 for (int k = 0; k < 3; ++k) {
                     const int idx = (i * 3 + j) * 3 + k;
                     const float wf = (float)(w[0][i] * w[1][j] * w[2][k]);
-                    int64_t q = 0;
-                    if (wf != 0.0f) {
-                        q = (int64_t)std::nearbyint((double)wf * (double)qp);
-                        if (q < 1) q = 1;
-                    }
+                    // w q_p to the nearest integer in FP32: floor(fma(w, q_p, 1/2)), rounded once
+                    // (q_p < 2^23 is exact); at least 1 where w > 0, 0 where w = 0
+                    int64_t q = (int64_t)std::fma(wf, (float)qp, 0.5f);
+                    if (wf > 0.0f && q < 1) q = 1;
                     d[idx] = q;
-                    if (idx != 13) acc += q;
+                    if (idx != 13) acc += d[idx];
                 }
         d[13] = qp - acc;
     }

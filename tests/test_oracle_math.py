@@ -160,6 +160,49 @@ def test_fixed_point_mass_is_exact():
     assert np.allclose(G["mass"], np.ldexp(G["mass_q"].astype(np.float64), e), rtol=1e-7)
 
 
+def test_fixed_point_node_quanta_round_the_weights_to_nearest():
+    """Per node, the fixed-point deposit of one particle is w q_p rounded to
+    the nearest integer, at least one quantum on every node with w > 0 and
+    none where w = 0 (a particle on a node plane), and the centre takes the
+    remainder.  Rounding up (or down) every node would bias the rim of each
+    stencil against its centre, and fails the half-quantum bound; dropping
+    the minimum would leave a touched node empty."""
+    rng = np.random.default_rng(11)
+    e = -30
+    # few quanta, so that the FP32 weights' own rounding (~1e-7 of w) stays far
+    # below half a quantum and the rounding rule is what the bound sees
+    qp = 5_003
+    m = float(np.ldexp(float(qp), e))
+    xs = list(rng.uniform(0.3, 0.6, size=(6, 3)))
+    xs.append(np.array([10.5, 12.25, 11.0]) * DX)  # f = x/dx - 0.5 integral on x: w = 0 on one plane
+    xs.append(np.array([10.5, 12.5, 11.5]) * DX)   # ... on all three axes
+    xs.append(np.array([10.5 + 1e-2, 12.5 + 2e-2, 11.5 + 3e-3]) * DX)  # tiny weights: the one-quantum minimum
+    tiny = 0
+    for x in xs:
+        s = OracleSim(DX, "f32", MASS_FIXED, e)
+        P = make_particles([x], [[0.0, 0.0, 0.0]], mass=m)
+        s.set_particles(P)
+        s.p2g()
+        f = P["x"][0].astype(np.float64) / DX - 0.5  # the stored FP32 position
+        b = np.floor(f).astype(int)
+        fr = f - b
+        w1 = np.stack([0.5 * (1 - fr) ** 2, 0.75 - (fr - 0.5) ** 2, 0.5 * fr ** 2])  # (3 offsets, 3 axes)
+        G = s.grid_dense(tuple(b), (3, 3, 3))
+        q = G["mass_q"]
+        w = w1[:, 0][:, None, None] * w1[:, 1][None, :, None] * w1[:, 2][None, None, :]
+        assert int(q.sum()) == qp
+        others = np.ones((3, 3, 3), bool)
+        others[1, 1, 1] = False
+        assert np.array_equal(q[others] > 0, w[others] > 0), x
+        wq = w[others] * qp
+        big = wq >= 0.5
+        assert (np.abs(q[others][big] - wq[big]) <= 0.5 + 1e-3).all(), (x, q[others] - wq)
+        assert (q[others][~big & (w[others] > 0)] == 1).all()
+        tiny += int((~big & (w[others] > 0)).sum())
+        assert q[1, 1, 1] == qp - q[others].sum()
+    assert tiny > 0  # the minimum was exercised
+
+
 def test_g2p_reproduces_affine_field():
     s = sim(gravity=(0.0, 0.0, 0.0), alpha=10.0)
     a = np.array([0.3, -0.2, 0.1])
