@@ -74,7 +74,10 @@ def build_library(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    # roctx ranges (trace.hpp) resolve against the ROCm profiler SDK's roctx library
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-L", rocm_lib,
+          "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{rocm_lib}"])
     os.replace(tmp, LIB)
     shutil.rmtree(tmpdir, ignore_errors=True)
     return LIB

@@ -22,6 +22,7 @@
 
 #include "host_staging.hpp"
 #include "mpm_internal.hpp"
+#include "trace.hpp"
 
 using namespace bm;
 
@@ -402,6 +403,7 @@ int batch_quantum_exp(const float* m, int64_t n) { return n > 0 ? quantum_exp_of
 
 int batch_add_scene_staged(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, int q_exp, const float* dev_soa,
                            hipEvent_t ready, const bm_body* body, int32_t* scene_id) {
+    BM_TRACE("batch_add_scene_staged");
     if (!b || !sd || !scene_id || n < 0 || (n > 0 && !dev_soa)) return fail(BM_ERR_INVALID_ARG, "null argument");
     SceneHost S{};
     if (int rc = place_scene(b, sd, n, q_exp, body, S)) return rc;
@@ -417,6 +419,7 @@ int batch_add_scene_staged(bm_batch_handle b, const bm_scene_desc* sd, int64_t n
 
 int batch_take_scene(bm_batch_handle b, int32_t id, float* dev_soa, int* dev_order, int64_t* n_out,
                      int64_t* base_out) {
+    BM_TRACE("batch_take_scene");
     if (!b || !n_out || !base_out) return fail(BM_ERR_INVALID_ARG, "null argument");
     const int s = b->find(id);
     if (s < 0) return fail(BM_ERR_INVALID_ARG, "no scene %d", id);
@@ -548,6 +551,7 @@ int bm_batch_synchronize(bm_batch_handle b) {
 
 int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, const bm_particles* p,
                        const bm_body* body, int32_t* scene_id) {
+    BM_TRACE("bm_batch_add_scene");
     if (int rc = check_batch(b)) return rc;
     if (!sd || !scene_id || n < 0 || (n > 0 && !p)) return fail(BM_ERR_INVALID_ARG, "null argument");
     if (n > 0 && (!p->x || !p->v || !p->F || !p->C || !p->m || !p->V0))
@@ -570,6 +574,7 @@ int bm_batch_add_scene(bm_batch_handle b, const bm_scene_desc* sd, int64_t n, co
 }
 
 int bm_batch_remove_scene(bm_batch_handle b, int32_t id) {
+    BM_TRACE("bm_batch_remove_scene");
     if (int rc = check_batch(b)) return rc;
     const int s = b->find(id);
     if (s < 0) return fail(BM_ERR_INVALID_ARG, "no scene %d", id);
@@ -637,6 +642,7 @@ int bm_batch_remove_scene(bm_batch_handle b, int32_t id) {
 }
 
 int bm_batch_step(bm_batch_handle b, float dt, int32_t k) {
+    BM_TRACE("bm_batch_step");
     if (int rc = check_batch(b)) return rc;
     if (k < 0) return fail(BM_ERR_INVALID_ARG, "k < 0");
     if (b->scenes.empty() || k == 0) return BM_OK;
@@ -660,6 +666,7 @@ int bm_batch_step(bm_batch_handle b, float dt, int32_t k) {
 }
 
 int bm_batch_download_scene(bm_batch_handle b, int32_t id, bm_particles* p) {
+    BM_TRACE("bm_batch_download_scene");
     if (int rc = check_batch(b)) return rc;
     if (!p) return fail(BM_ERR_INVALID_ARG, "null output");
     const int s = b->find(id);

@@ -43,6 +43,7 @@
 
 #include "mpm_device.hpp"
 #include "mpm_internal.hpp"
+#include "trace.hpp"
 
 namespace bm {
 namespace dif {
@@ -1493,6 +1494,7 @@ int bm_diff_upload(bm_diff_handle h, int64_t n, const bm_particles* p, const flo
 }
 
 int bm_diff_forward(bm_diff_handle h, float dt, int32_t substeps, double* loss) {
+    BM_TRACE("bm_diff_forward");
     if (int rc = check_diff(h)) return rc;
     if (!h->has_prm) return fail(BM_ERR_STATE, "parameters not set");
     if (h->n <= 0) return fail(BM_ERR_STATE, "no particles");
@@ -1502,6 +1504,7 @@ int bm_diff_forward(bm_diff_handle h, float dt, int32_t substeps, double* loss) 
 }
 
 int bm_diff_backward(bm_diff_handle h, bm_diff_grad* out) {
+    BM_TRACE("bm_diff_backward");
     if (int rc = check_diff(h)) return rc;
     if (!out) return fail(BM_ERR_INVALID_ARG, "null output");
     if (!h->have_forward) return fail(BM_ERR_STATE, "backward needs a forward pass with the current parameters");

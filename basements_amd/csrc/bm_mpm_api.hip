@@ -16,6 +16,7 @@
 
 #include "host_staging.hpp"
 #include "mpm_internal.hpp"
+#include "trace.hpp"
 
 #include <atomic>
 
@@ -499,6 +500,7 @@ int bm_mpm_add_particles(bm_mpm_handle h, int64_t n, const float* pos, const flo
 }
 
 int bm_mpm_upload_particles(bm_mpm_handle h, int64_t n, const bm_particles* p) {
+    BM_TRACE("bm_mpm_upload_particles");
     if (int rc = check_handle(h)) return rc;
     if (n < 0 || !p) return fail(BM_ERR_INVALID_ARG, "bad particle arrays");
     if (n > h->cap) return fail(BM_ERR_OOM, "capacity %lld exceeded", (long long)h->cap);
@@ -515,6 +517,7 @@ int bm_mpm_upload_particles(bm_mpm_handle h, int64_t n, const bm_particles* p) {
 }
 
 int bm_mpm_download_particles(bm_mpm_handle h, bm_particles* p) {
+    BM_TRACE("bm_mpm_download_particles");
     if (int rc = check_handle(h)) return rc;
     if (!p) return fail(BM_ERR_INVALID_ARG, "null output");
     if (int rc = sync_and_check(h)) return rc;
@@ -579,6 +582,7 @@ int bm_mpm_commit_device_particles(bm_mpm_handle h, int64_t n) {
 // Fast mode: re-bin the particles by grid block (swaps the particle buffers).
 // with_vc = false when a G2P follows, which rewrites velocity and C.
 static int rebin(bm_mpm_handle h, bool with_vc = true) {
+    BM_TRACE("rebin");
     h->begin(BM_PHASE_SORT);
     const int rc = launch_rebin(h->pview(), h->pbuf2, h->pid, h->pid2, h->dom, (int)h->nblocks, h->unit_p, h->bs,
                                 h->stream, &h->n_units, SceneView{}, 0, with_vc);
@@ -674,6 +678,7 @@ static int g2p_fast(bm_mpm_handle h, float dt, bool store_vc) {
 }
 
 int bm_mpm_p2g(bm_mpm_handle h, float dt) {
+    BM_TRACE("bm_mpm_p2g");
     (void)dt;  // p2g_single ignores dt (mpm_solver.h:168)
     if (int rc = check_handle(h)) return rc;
     if (h->fast) return p2g_fast(h, dt);
@@ -714,6 +719,7 @@ static int grid_update_impl(bm_mpm_handle h, float dt, bool with_body) {
 }
 
 int bm_mpm_grid_update(bm_mpm_handle h, float dt) {
+    BM_TRACE("bm_mpm_grid_update");
     if (int rc = check_handle(h)) return rc;
     return grid_update_impl(h, dt, false);
 }
@@ -730,7 +736,10 @@ static int g2p_impl(bm_mpm_handle h, float dt, bool last) {
     return BM_OK;
 }
 
-int bm_mpm_g2p(bm_mpm_handle h, float dt) { return g2p_impl(h, dt, true); }
+int bm_mpm_g2p(bm_mpm_handle h, float dt) {
+    BM_TRACE("bm_mpm_g2p");
+    return g2p_impl(h, dt, true);
+}
 
 static int step_impl(bm_mpm_handle h, float dt, bool last) {
     if (int rc = bm_mpm_p2g(h, dt)) return rc;
@@ -738,9 +747,13 @@ static int step_impl(bm_mpm_handle h, float dt, bool last) {
     return g2p_impl(h, dt, last);
 }
 
-int bm_mpm_step(bm_mpm_handle h, float dt) { return step_impl(h, dt, true); }
+int bm_mpm_step(bm_mpm_handle h, float dt) {
+    BM_TRACE("bm_mpm_step");
+    return step_impl(h, dt, true);
+}
 
 int bm_mpm_step_n(bm_mpm_handle h, float dt, int32_t k) {
+    BM_TRACE("bm_mpm_step_n");
     if (k < 0) return fail(BM_ERR_INVALID_ARG, "k < 0");
     for (int32_t i = 0; i < k; ++i)
         if (int rc = step_impl(h, dt, i == k - 1)) return rc;
@@ -754,6 +767,7 @@ static ColliderDev make_collider(bm_mpm_handle h, const bm_collider* b) {
 }
 
 int bm_mpm_couple(bm_mpm_handle h, int32_t method, const bm_collider* body, float dt, bm_reaction* out) {
+    BM_TRACE("bm_mpm_couple");
     if (int rc = check_handle(h)) return rc;
     if (!body || !out) return fail(BM_ERR_INVALID_ARG, "null argument");
     if (method < BM_COUPLER_M0_DIRICHLET || method > BM_COUPLER_M45_BSPLINE)
@@ -839,6 +853,7 @@ int bm_mpm_get_body(bm_mpm_handle h, bm_body* body, bm_reaction* reaction) {
 }
 
 int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* out) {
+    BM_TRACE("bm_mpm_couple_body");
     if (int rc = check_handle(h)) return rc;
     if (method < BM_COUPLER_M0_DIRICHLET || method > BM_COUPLER_M45_BSPLINE)
         return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
@@ -863,6 +878,7 @@ int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* o
 }
 
 int bm_mpm_step_coupled(bm_mpm_handle h, int32_t method, float dt, int32_t k) {
+    BM_TRACE("bm_mpm_step_coupled");
     if (int rc = check_handle(h)) return rc;
     if (k < 0) return fail(BM_ERR_INVALID_ARG, "k < 0");
     if (method < BM_COUPLER_M0_DIRICHLET || method > BM_COUPLER_M45_BSPLINE)

@@ -35,8 +35,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <limits>
+#include <mutex>
+#include <string>
 #include <deque>
 #include <map>
 #include <thread>
@@ -45,6 +48,7 @@
 
 #include "host_transpose.hpp"
 #include "mpm_internal.hpp"
+#include "trace.hpp"
 
 using namespace bm;
 
@@ -146,6 +150,19 @@ struct bm_server {
     hipStream_t cs = nullptr;          // copy stream (default priority)
     hipEvent_t chunk_done = nullptr;   // recorded after each launched chunk
     bool in_flight = false;            // a launched chunk not yet waited for
+    // The chunk's kernels are enqueued by a launcher thread: the runtime
+    // holds an enqueuing thread back while a few dozen dispatches are
+    // outstanding, i.e. for most of a chunk, and the poll thread stages
+    // meanwhile.  Guarded by mu: job_k (> 0: a chunk being enqueued), the
+    // result of the last one, stop.
+    int device = 0;
+    std::thread launcher;
+    std::mutex mu;
+    std::condition_variable cv;
+    int32_t job_k = 0;
+    int job_rc = 0;
+    std::string job_err;
+    bool stop = false;
     char* dstage = nullptr;            // device staging area
     char* hstage = nullptr;            // pinned host staging, same offsets
     int64_t stage_bytes = 0;
@@ -184,6 +201,7 @@ static void drop_staged(bm_server_handle s, Request& R) {
 
 // Admissions.
 static int admit(bm_server_handle s) {
+    BM_TRACE("server admit");
     const double t = s->now();
     for (size_t qi = 0; qi < s->queue.size();) {
         Request& R = s->reqs[s->queue[qi]];
@@ -236,6 +254,7 @@ static int admit(bm_server_handle s) {
 // state goes to the staging area and starts down to pinned memory on the
 // copy stream; finish_retirements writes its output later.
 static int retire(bm_server_handle s) {
+    BM_TRACE("server retire");
     for (size_t i = 0; i < s->running.size();) {
         Request& R = s->reqs[s->running[i]];
         int32_t code = BM_OK;
@@ -289,6 +308,7 @@ static int retire(bm_server_handle s) {
 // Outputs of the staged retirements: wait for each download, transpose into
 // the caller's buffers, report done.
 static int finish_retirements(bm_server_handle s) {
+    BM_TRACE("server finish retirements");
     for (Retiring& T : s->retiring) {
         Request& R = s->reqs[T.id];
         HIP_TRY(hipEventSynchronize(T.landed));
@@ -316,6 +336,7 @@ static int finish_retirements(bm_server_handle s) {
 // first queued request still to come (infinity when none, or when the
 // staging area is full).
 static int stage_one(bm_server_handle s, bool* did, double* next) {
+    BM_TRACE("server stage arrival");
     *did = false;
     *next = std::numeric_limits<double>::infinity();
     if (!s->overlap || s->stage_bytes == 0) return BM_OK;
@@ -350,17 +371,81 @@ static int stage_one(bm_server_handle s, bool* did, double* next) {
     return BM_OK;
 }
 
+// The launcher thread: enqueue each chunk handed over (bm_batch_step, then
+// the chunk's event on the batch stream) on the server's device.
+static void launcher_main(bm_server_handle s) {
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv.wait(lk, [&] { return s->stop || s->job_k > 0; });
+        if (s->stop) return;
+        const int32_t k = s->job_k;
+        lk.unlock();
+        int rc;
+        {
+            BM_TRACE("server enqueue chunk");
+            rc = bm_batch_step(s->batch, s->desc.dt, k);
+            if (rc == BM_OK && hipEventRecord(s->chunk_done, batch_stream(s->batch)) != hipSuccess)
+                rc = fail(BM_ERR_HIP, "hipEventRecord failed after a chunk");
+        }
+        const std::string err = rc == BM_OK ? std::string() : std::string(bm_last_error());
+        lk.lock();
+        s->job_rc = rc;
+        s->job_err = err;
+        s->job_k = 0;
+        s->cv.notify_all();
+    }
+}
+
+// Launch a chunk of k substeps: handed to the launcher thread with overlap
+// on, enqueued here otherwise.
+static int launch_chunk(bm_server_handle s, int32_t k) {
+    if (!s->overlap || !s->launcher.joinable()) {
+        if (int rc = bm_batch_step(s->batch, s->desc.dt, k)) return rc;
+        HIP_TRY(hipEventRecord(s->chunk_done, batch_stream(s->batch)));
+        return BM_OK;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->job_k = k;
+    s->job_rc = BM_OK;
+    s->cv.notify_all();
+    return BM_OK;
+}
+
+// True when no chunk is being enqueued; with block, waits for that.  A
+// failure of the enqueuing thread is reported here, with its message.
+static bool enqueued(bm_server_handle s, bool block, int* rc) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (block) s->cv.wait(lk, [&] { return s->job_k == 0; });
+    if (s->job_k != 0) return false;
+    *rc = s->job_rc;
+    if (s->job_rc != BM_OK) {
+        *rc = fail(s->job_rc, "%s", s->job_err.c_str());
+        s->job_rc = BM_OK;
+    }
+    return true;
+}
+
 // Wait for the chunk in flight, staging arrivals meanwhile: requests that
 // have arrived are staged one at a time; with none waiting, the host checks
 // the chunk every 50 us until the next arrival, or blocks on it when no
 // arrival is due.
 static int wait_chunk(bm_server_handle s) {
+    BM_TRACE("server wait chunk");
     if (!s->in_flight) return BM_OK;
     const double t = s->now();
     for (;;) {
-        const hipError_t q = hipEventQuery(s->chunk_done);
-        if (q == hipSuccess) break;
-        if (q != hipErrorNotReady) HIP_TRY(q);
+        int erc = BM_OK;
+        const bool ready = enqueued(s, false, &erc);
+        if (erc != BM_OK) {
+            s->in_flight = false;
+            return erc;
+        }
+        if (ready) {
+            const hipError_t q = hipEventQuery(s->chunk_done);
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady) HIP_TRY(q);
+        }
         bool did = false;
         double next = 0.0;
         if (int rc = stage_one(s, &did, &next)) return rc;
@@ -368,8 +453,11 @@ static int wait_chunk(bm_server_handle s) {
         if (!std::isfinite(next)) break;
         std::this_thread::sleep_for(std::chrono::duration<double>(std::min(std::max(next - s->now(), 0.0), 50e-6)));
     }
-    HIP_TRY(hipEventSynchronize(s->chunk_done));
+    int erc = BM_OK;
+    enqueued(s, true, &erc);
     s->in_flight = false;
+    if (erc != BM_OK) return erc;
+    HIP_TRY(hipEventSynchronize(s->chunk_done));
     s->t_step += s->now() - t;
     return BM_OK;
 }
@@ -414,6 +502,8 @@ int bm_server_create(const bm_server_desc* desc, bm_server_handle* out) {
         if (d) hipFree(d);
         s->overlap = false;
     }
+    (void)hipGetDevice(&s->device);
+    s->launcher = std::thread(launcher_main, s);
     s->t0 = std::chrono::steady_clock::now();
     *out = s;
     return BM_OK;
@@ -421,6 +511,15 @@ int bm_server_create(const bm_server_desc* desc, bm_server_handle* out) {
 
 int bm_server_destroy(bm_server_handle s) {
     if (!s) return BM_OK;
+    if (s->launcher.joinable()) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->job_k == 0; });
+            s->stop = true;
+            s->cv.notify_all();
+        }
+        s->launcher.join();
+    }
     if (s->cs) hipStreamSynchronize(s->cs);
     for (auto& kv : s->reqs)
         if (kv.second.in_ready) hipEventDestroy(kv.second.in_ready);
@@ -454,6 +553,7 @@ int bm_server_clock(bm_server_handle s, double* seconds) {
 int bm_server_batch(bm_server_handle s, bm_batch_handle* batch) {
     if (int rc = check_server(s)) return rc;
     if (!batch) return fail(BM_ERR_INVALID_ARG, "null output");
+    if (int rc = wait_chunk(s)) return rc;  // the batch is the caller's until the next poll
     *batch = s->batch;
     return BM_OK;
 }
@@ -489,6 +589,7 @@ int bm_server_submit(bm_server_handle s, const bm_request* req, int64_t* id) {
 }
 
 int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int32_t* n_done) {
+    BM_TRACE("bm_server_poll");
     if (int rc = check_server(s)) return rc;
     if (capacity < 0 || (capacity > 0 && !done_ids)) return fail(BM_ERR_INVALID_ARG, "bad done-id buffer");
     // 1. the chunk in flight ends; finished requests leave the batch
@@ -515,8 +616,7 @@ int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int3
         int32_t k = s->desc.max_chunk;
         for (int64_t id : s->running) k = std::min(k, s->reqs[id].left);
         t = s->now();
-        if (int rc = bm_batch_step(s->batch, s->desc.dt, k)) return rc;
-        HIP_TRY(hipEventRecord(s->chunk_done, batch_stream(s->batch)));
+        if (int rc = launch_chunk(s, k)) return rc;
         s->in_flight = true;
         ++s->chunks;
         s->substeps += k;
@@ -557,6 +657,7 @@ int bm_server_info(bm_server_handle s, int64_t id, bm_request_info* info) {
 
 int bm_server_cancel(bm_server_handle s, int64_t id) {
     if (int rc = check_server(s)) return rc;
+    if (int rc = wait_chunk(s)) return rc;  // no chunk is being enqueued while the batch changes
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return fail(BM_ERR_INVALID_ARG, "no request %lld", (long long)id);
     Request& R = it->second;

@@ -422,15 +422,17 @@ int bm_server_create(const bm_server_desc* desc, bm_server_handle* out);
 int bm_server_destroy(bm_server_handle s);
 /* Seconds since the server was created (the clock of arrivals and timings). */
 int bm_server_clock(bm_server_handle s, double* seconds);
-/* The server's batch, for tuning (bm_batch_set_tuning / _autotune / _set_graph). */
+/* The server's batch, for tuning (bm_batch_set_tuning / _autotune / _set_graph).
+ * Waits for the chunk in flight; the batch is the caller's until the next poll. */
 int bm_server_batch(bm_server_handle s, bm_batch_handle* batch);
 /* Queue a request.  BM_ERR_OOM when it could not fit even an empty server. */
 int bm_server_submit(bm_server_handle s, const bm_request* req, int64_t* request_id);
 /* One scheduling round: the chunk launched by the previous round ends and
  * the requests it finished leave the batch; admissions; the next chunk is
- * launched and not waited for; while it runs, the finished requests' outputs
- * are written and arrived requests are staged onto the device for their
- * admission (overlap on, the default).  Sleeps until the next arrival when
+ * launched and not waited for -- with overlap on (the default) a launcher
+ * thread of the server enqueues its kernels -- and while it runs, the
+ * finished requests' outputs are written and arrived requests are staged
+ * onto the device for their admission.  Sleeps until the next arrival when
  * nothing is running and nothing is left to report.  Writes up to
  * `capacity` ids of requests finished (done or failed) since the last
  * report; the rest are reported by later calls.  A request's output is
@@ -445,7 +447,8 @@ int bm_server_poll(bm_server_handle s, int64_t* done_ids, int32_t capacity, int3
  * allocated at creation. */
 int bm_server_set_overlap(bm_server_handle s, int32_t on);
 int bm_server_info(bm_server_handle s, int64_t request_id, bm_request_info* info);
-/* Drop a queued or running request (its output is not written). */
+/* Drop a queued or running request (its output is not written).  Waits for
+ * the chunk in flight. */
 int bm_server_cancel(bm_server_handle s, int64_t request_id);
 int bm_server_load(bm_server_handle s, int32_t* queued, int32_t* running, int64_t* bytes_in_use, int64_t* budget);
 /* Where the time went: seconds[4] = admitting, stepping (launch until the
