@@ -70,6 +70,7 @@ KERNEL_OF = {
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 REBIN_PERIOD = 32   # a single handle's default re-binning period (substeps)
 BATCH_STEPS = 64    # the default line's batched measurement: two re-binning cycles of 64 scenes
+ADJ_BYTES_PER_PARTICLE_SUBSTEP = 600  # adjoint rollout: forward 200 B + backward 400 B per particle and substep
 SERVE_RATE = 200.0  # the default line's served measurement: a saturating arrival rate (requests/s)
 
 
@@ -403,13 +404,22 @@ def bench_adjoint(a):
                  "adjoint_fp32": g32v, "rel_err_fp32_vs_fp64": rel32, "loss_fp64": loss64, "loss_fp32": loss32,
                  "fd_step": "1e-4 relative (tool speed for velocity components)"}
     value = n * T / (t_f + t_b)
+    # HBM model: the forward substep's 200 B per particle (state read and
+    # written once), the backward twice that (the recomputed forward state plus
+    # the adjoint state read and written); the bound at this size is occupancy
+    waves = -(-n // 64)
+    algo = ADJ_BYTES_PER_PARTICLE_SUBSTEP * n * T
+    roof = {"bound": "latency", "achieved": algo / (t_f + t_b) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": algo / (t_f + t_b) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+            "algo_bytes_per_particle_substep": ADJ_BYTES_PER_PARTICLE_SUBSTEP, "waves_per_kernel": waves,
+            "simds": 1024, "waves_per_simd": waves / 1024}
     out = {"metric": "particle-substeps/s (forward + adjoint, FP32, checkpoint every 16)", "value": value,
            "unit": "particle-substeps/s", "n_gpus": 1, "steps": reps, "warmup": 1,
            "ms_per_step": (t_f + t_b) / T * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "fp32", "data": "synthetic (seeded soil bed and target field, no external data)",
            "config": {"workload": "adjoint", "particles": n, "substeps": T, "dt": dt, "checkpoint_every": 16,
                       "grid_dims": list(sc["dims"]), "tool_velocity": list(sc["params"].tool_velocity)},
-           "forward_ms": t_f * 1e3, "backward_ms": t_b * 1e3, "check": check}
+           "forward_ms": t_f * 1e3, "backward_ms": t_b * 1e3, "roofline": roof, "check": check}
     print(json.dumps(out))
     return 0
 
