@@ -595,6 +595,14 @@ static int rebin(bm_mpm_handle h, bool with_vc = true) {
     return BM_OK;
 }
 
+// The G2P form of a single scene (bm_mpm_set_tuning bits 4 and 5, else by
+// the number of units).
+static bool g2p_tiled(bm_mpm_handle h) {
+    if (h->variant & 16) return false;
+    if (h->variant & 32) return true;
+    return h->n_units >= G2P_TILE_MIN_UNITS;
+}
+
 // The fast substep's particle kernels: the per-particle G2P (or the deposit
 // record of the current state when do_g2p = false) and the per-cell deposit.
 // store_vc = false skips the velocity and C stores of the G2P.
@@ -627,7 +635,7 @@ static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const
     if (ctx.adv_phase >= 0) h->open_phase = BM_PHASE_OTHER;  // the opening event now starts OTHER
     launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p, dt_fold,
                  h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream, mark, &ctx,
-                 (h->variant & 8) != 0, SplitBatch{}, store_vc);
+                 (h->variant & 8) != 0, SplitBatch{}, store_vc, g2p_tiled(h));
 }
 
 static int p2g_fast(bm_mpm_handle h, float dt) {
@@ -1018,7 +1026,8 @@ int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_ever
     if (int rc = check_handle(h)) return rc;
     if (unit_particles > UNIT_MAX || (unit_particles > 0 && unit_particles < 16))
         return fail(BM_ERR_INVALID_ARG, "unit_particles must be in [16, %d]", UNIT_MAX);
-    if (variant > 15) return fail(BM_ERR_INVALID_ARG, "unknown variant %d", variant);
+    if (variant > 63 || (variant >= 0 && (variant & 48) == 48))
+        return fail(BM_ERR_INVALID_ARG, "unknown variant %d", variant);
     if (variant >= 0 && !(variant & 4))
         return fail(BM_ERR_NOT_SUPPORTED, "variant %d: the single-kernel G2P+P2G form was retired (bit 2 must be set)",
                     variant);

@@ -300,7 +300,15 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*) = nullptr,
                   void* ctx = nullptr, bool two_pass = false, const SplitBatch& batch = SplitBatch{},
-                  bool store_vc = true);
+                  bool store_vc = true, bool g2p_tile = true);
+
+// Units below which a single scene's G2P runs per particle over the flat
+// particle range: with few units the workgroups of the staged-tile G2P do not
+// hide its staging latency.  Measured per substep on dragged-tool scenes: the
+// per-particle form faster by 3 us from 2e4 to 5e5 particles, by 1 us at 1e6
+// (3,240 blocks), even at 2e6 (5,808 blocks); the staged form 15 us faster on
+// the 4.1e6-particle wheel bed (11,264 blocks).
+constexpr int G2P_TILE_MIN_UNITS = 6144;
 
 // Batch entry points of the request loop (bm_batch.hip): admission of a scene
 // whose component-major particle state (PCOMP rows of n floats) is already on

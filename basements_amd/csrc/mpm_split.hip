@@ -1,13 +1,14 @@
 // mpm_split.hip -- the two-kernel form of the fast substep (tuning variant
-// bit 2): a per-particle G2P kernel at high occupancy that also writes each
-// particle's deposit coefficients, then a per-block P2G kernel that ranks the
-// particles by their *final* stencil cell and deposits with ordered LDS
+// bit 2): a G2P kernel over the re-binning units (each unit's grid nodes
+// staged in LDS, one lane per particle) that also writes each particle's
+// deposit coefficients, then a per-block P2G kernel that ranks the particles
+// by their *final* stencil cell and deposits with ordered LDS
 // read-modify-writes.
 //
-// Compared with the fused kernel it moves 128 more bytes per particle (the
-// deposit record written and read back) but neither kernel holds the other's
-// registers: the G2P kernel runs without LDS, and the P2G kernel without the
-// SVD.
+// Compared with a fused G2P+P2G kernel it moves 128 more bytes per particle
+// (the deposit record written and read back) but neither kernel holds the
+// other's registers: the G2P kernel needs 8 KiB of LDS and no deposit
+// accumulators, and the P2G kernel no SVD.
 //
 // Deposit record: 16 floats per particle as four float4 quads
 //   (x, y, z, q_p)  (m v, A00)  (A01, A02, A10, A11)  (A12, A20, A21, A22)
@@ -74,7 +75,103 @@ __device__ __forceinline__ int xcd_slot(int w, int G) {
     return x * base + min(x, rem) + q;
 }
 
-// G2P + plasticity + write-back + deposit coefficients; one lane per particle.
+// The state a G2P reads of particle i: position, F, mass and volume.
+struct G2PIn {
+    float x, y, z, m, V0;
+    m3<float> F;
+};
+__device__ __forceinline__ G2PIn g2p_load(const PView& P, int64_t i) {
+    G2PIn q;
+    q.x = P.p[(PX + 0) * P.cap + i];
+    q.y = P.p[(PX + 1) * P.cap + i];
+    q.z = P.p[(PX + 2) * P.cap + i];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) q.F.a[c] = P.p[(PF + c) * P.cap + i];
+    q.m = P.p[PM * P.cap + i];
+    q.V0 = P.p[PV0 * P.cap + i];
+    return q;
+}
+
+// G2P + plasticity + write-back + deposit coefficients of particle i from its
+// loaded state, with GATHER(s, lx, ly, lz, bad, vel, C) the G2P gather of its
+// stencil (base lx, ly, lz in box nodes; `bad` when the stencil leaves the
+// box, in which case the gather must still read in bounds).
+template <class GatherFn>
+__device__ __forceinline__ void g2p_particle(int64_t i, const G2PIn& in, const PView& P, const Domain& d,
+                                             const Phys& ph, const FastArgs& fa, double qscale_inv, float4* rec,
+                                             int* err, int scene, GatherFn gather) {
+    const float inv_dx = d.inv_dx;
+    float x = in.x, y = in.y, z = in.z;
+    m3<float> F = in.F;
+    const float m = in.m, V0 = in.V0;
+    v3<float> vel, lns;
+    m3<float> Cm, Us;
+    Stencil s;
+    make_stencil(x, y, z, inv_dx, s);
+    const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+    // A stencil outside the box is gathered in bounds and rejected after the
+    // gather, so no branch separates the loads above from the gather's and
+    // all of them are in flight together.
+    const bool bad = !stencil_ok(d, lx, ly, lz);
+    gather(s, lx, ly, lz, bad, vel, Cm);
+    if (bad) {
+        flag_error(err, scene, 1);
+        store_rec(rec, i, x, y, z, 0.0f, mk(0.f, 0.f, 0.f), m3<float>{{0, 0, 0, 0, 0, 0, 0, 0, 0}});
+        return;
+    }
+    const float dt = fa.dt_g2p;
+    x += vel.x * dt;
+    y += vel.y * dt;
+    z += vel.z * dt;
+    m3<float> A;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A.a[c] = ((c % 4 == 0) ? 1.0f : 0.0f) + Cm.a[c] * dt;
+    const m3<float> Ftr = mmul(A, F);
+    m3<float> V;
+    v3<float> sg;
+    if (!svd_fast(Ftr, Us, sg, V)) {
+        fa.redo[atomicAdd(fa.n_redo, 1)] = (int)i;  // redone from scratch in FP64
+        return;
+    }
+    const Material M{ph.mu, ph.lambda, ph.alpha, ph.kohesion};
+    const v3<float> sn = dp_fast(sg, M, lns);
+    F = recompose(Us, sn, V);
+    if (y < 0.0f) {
+        y = 0.0f;
+        vel.y = 0.0f;
+    }
+    check_finite(err, scene, x, y, z, vel.x, vel.y, vel.z);
+    P.p[(PX + 0) * P.cap + i] = x;
+    P.p[(PX + 1) * P.cap + i] = y;
+    P.p[(PX + 2) * P.cap + i] = z;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) P.p[(PF + c) * P.cap + i] = F.a[c];
+    if (fa.store_vc) {
+        P.p[(PV + 0) * P.cap + i] = vel.x;
+        P.p[(PV + 1) * P.cap + i] = vel.y;
+        P.p[(PV + 2) * P.cap + i] = vel.z;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) P.p[(PC + c) * P.cap + i] = Cm.a[c];
+    }
+    v3<float> mvel;
+    m3<float> Aff;
+    p2g_coeffs(m, V0, vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
+    store_rec(rec, i, x, y, z, quanta_of(m, qscale_inv), mvel, Aff);
+}
+
+// The G2P gather straight from the block-linear grid (27 global loads).
+struct GlobalGather {
+    const float4* __restrict__ g;
+    Domain d;
+    __device__ __forceinline__ void operator()(const Stencil& s, int lx, int ly, int lz, bool bad, v3<float>& vel,
+                                               m3<float>& Cm) const {
+        const NodeAxes na(d, bad ? 0 : lx, bad ? 0 : ly, bad ? 0 : lz);
+        gather_sep(s, d.dx, 4.0f * d.inv_dx * d.inv_dx, [&](int a, int b, int c) { return g[na(a, b, c)]; }, vel, Cm);
+    }
+};
+
+// One lane per particle over the whole particle range: the G2P (DO_G2P), or
+// the deposit record of the current state (velocity and C as stored).
 template <bool DO_G2P>
 __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float4* rec,
                                                  int* err, SceneView sv) {
@@ -96,76 +193,89 @@ __global__ void __launch_bounds__(256) k_advance(PView P, GridBuf gin, Domain d,
     } else if (i >= P.n) {
         return;
     }
-    const float dx = d.dx, inv_dx = d.inv_dx;
-    float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
+    if (DO_G2P) {
+        // every load of the particle is issued before the gather's (the state
+        // stores may alias P.p, so later loads would wait for the whole G2P)
+        g2p_particle(i, g2p_load(P, i), P, d, ph, fa, fa.qscale_inv, rec, err, scene, GlobalGather{gin.mv, d});
+        return;
+    }
+    const float inv_dx = d.inv_dx;
+    const float m = P.p[PM * P.cap + i], V0 = P.p[PV0 * P.cap + i];
+    const float x = P.p[(PX + 0) * P.cap + i], y = P.p[(PX + 1) * P.cap + i], z = P.p[(PX + 2) * P.cap + i];
     m3<float> F;
 #pragma unroll
     for (int c = 0; c < 9; ++c) F.a[c] = P.p[(PF + c) * P.cap + i];
-    v3<float> vel, lns;
-    m3<float> Cm, Us;
-    if (DO_G2P) {
-        Stencil s;
-        make_stencil(x, y, z, inv_dx, s);
-        const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
-        if (!stencil_ok(d, lx, ly, lz)) {
-            flag_error(err, scene, 1);
-            store_rec(rec, i, x, y, z, 0.0f, mk(0.f, 0.f, 0.f), m3<float>{{0, 0, 0, 0, 0, 0, 0, 0, 0}});
-            return;
-        }
-        const NodeAxes na(d, lx, ly, lz);
-        const float4* __restrict__ g = gin.mv;
-        gather_sep(s, dx, 4.0f * inv_dx * inv_dx, [&](int a, int b, int c) { return g[na(a, b, c)]; }, vel, Cm);
-        const float dt = fa.dt_g2p;
-        x += vel.x * dt;
-        y += vel.y * dt;
-        z += vel.z * dt;
-        m3<float> A;
+    const v3<float> vel = mk(P.p[(PV + 0) * P.cap + i], P.p[(PV + 1) * P.cap + i], P.p[(PV + 2) * P.cap + i]);
+    m3<float> Cm, Us, V;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) A.a[c] = ((c % 4 == 0) ? 1.0f : 0.0f) + Cm.a[c] * dt;
-        const m3<float> Ftr = mmul(A, F);
-        m3<float> V;
-        v3<float> sg;
-        if (!svd_fast(Ftr, Us, sg, V)) {
-            fa.redo[atomicAdd(fa.n_redo, 1)] = (int)i;  // redone from scratch in FP64
-            return;
-        }
-        const Material M{ph.mu, ph.lambda, ph.alpha, ph.kohesion};
-        const v3<float> sn = dp_fast(sg, M, lns);
-        F = recompose(Us, sn, V);
-        if (y < 0.0f) {
-            y = 0.0f;
-            vel.y = 0.0f;
-        }
-        check_finite(err, scene, x, y, z, vel.x, vel.y, vel.z);
-        P.p[(PX + 0) * P.cap + i] = x;
-        P.p[(PX + 1) * P.cap + i] = y;
-        P.p[(PX + 2) * P.cap + i] = z;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) P.p[(PF + c) * P.cap + i] = F.a[c];
-        if (fa.store_vc) {
-            P.p[(PV + 0) * P.cap + i] = vel.x;
-            P.p[(PV + 1) * P.cap + i] = vel.y;
-            P.p[(PV + 2) * P.cap + i] = vel.z;
-#pragma unroll
-            for (int c = 0; c < 9; ++c) P.p[(PC + c) * P.cap + i] = Cm.a[c];
-        }
-    } else {
-        vel = mk(P.p[(PV + 0) * P.cap + i], P.p[(PV + 1) * P.cap + i], P.p[(PV + 2) * P.cap + i]);
-#pragma unroll
-        for (int c = 0; c < 9; ++c) Cm.a[c] = P.p[(PC + c) * P.cap + i];
-        m3<float> V;
-        v3<float> sg;
-        if (!svd_fast(F, Us, sg, V)) {
-            fa.redo[atomicAdd(fa.n_redo, 1)] = (int)i;
-            return;
-        }
-        lns = mk(flog(fmaxf(sg.x, 1e-6f)), flog(fmaxf(sg.y, 1e-6f)), flog(fmaxf(sg.z, 1e-6f)));
+    for (int c = 0; c < 9; ++c) Cm.a[c] = P.p[(PC + c) * P.cap + i];
+    v3<float> sg;
+    if (!svd_fast(F, Us, sg, V)) {
+        fa.redo[atomicAdd(fa.n_redo, 1)] = (int)i;
+        return;
     }
+    const v3<float> lns = mk(flog(fmaxf(sg.x, 1e-6f)), flog(fmaxf(sg.y, 1e-6f)), flog(fmaxf(sg.z, 1e-6f)));
     v3<float> mvel;
     m3<float> Aff;
-    const float m = P.p[PM * P.cap + i];
-    p2g_coeffs(m, P.p[PV0 * P.cap + i], vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
+    p2g_coeffs(m, V0, vel, Cm, Us, lns, ph, inv_dx, fa.dt_fold, mvel, Aff);
     store_rec(rec, i, x, y, z, quanta_of(m, fa.qscale_inv), mvel, Aff);
+}
+
+// The G2P over the re-binning units: a unit's particles share one 4^3-node
+// block (the block of their stencil base at the last re-binning), so their
+// stencils lie in the 8^3 nodes around it (the block, 2 stencil layers and 1
+// drift layer on each side).  The workgroup stages those nodes' velocities
+// in LDS once and every particle gathers its 27 nodes from there at constant
+// offsets from one address; a particle that drifted further gathers from the
+// grid.  One staged tile replaces ~27 x 12 B of L1 traffic per particle with
+// 8^3 x 16 B per unit.  Units are dealt XCD-contiguously (neighbouring units
+// stage overlapping nodes, which then come from one L2).  With the unit count
+// on the device only (a batch) a fixed grid walks the units.
+__global__ void __launch_bounds__(256) k_advance_tile(PView P, GridBuf gin, Domain d, Phys ph, FastArgs fa, float4* rec,
+                                                      int* err, SceneView sv, const int4* __restrict__ units) {
+    __shared__ float4 tile[TNODES];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && fa.list_reset) *fa.list_reset = 0;
+    const int nu = fa.n_units_dev ? *fa.n_units_dev : fa.n_units;
+    for (int u = xcd_slot((int)blockIdx.x, (int)gridDim.x); u < nu; u += gridDim.x) {
+        const int4 U = units[u];
+        Domain dd = d;
+        Phys pp = ph;
+        double qsi = fa.qscale_inv;
+        int scene = -1;
+        if (sv.sc) {  // a batch: the unit's scene
+            const SceneDev& S = sv.sc[U.w];
+            dd = S.d;
+            pp = S.ph;
+            qsi = S.qscale_inv;
+            scene = U.w;
+        }
+        int bx, by, bz;
+        block_coords(dd, U.x, bx, by, bz);
+        const int tx0 = bx * 4 - 1, ty0 = by * 4 - 1, tz0 = bz * 4 - 1;
+        __syncthreads();  // the previous unit's gathers are done with the tile
+        for (int k = threadIdx.x; k < TNODES; k += blockDim.x) {
+            const int lx = tx0 + (k >> 6), ly = ty0 + ((k >> 3) & 7), lz = tz0 + (k & 7);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lx >= 0 && ly >= 0 && lz >= 0 && lx < dd.nx && ly < dd.ny && lz < dd.nz) v = gin.mv[node_of(dd, lx, ly, lz)];
+            tile[k] = v;
+        }
+        __syncthreads();
+        const float4* gm = gin.mv;
+        auto gather = [&](const Stencil& s, int lx, int ly, int lz, bool bad, v3<float>& vel, m3<float>& Cm) {
+            const int qx = lx - tx0, qy = ly - ty0, qz = lz - tz0;
+            const float kC = 4.0f * dd.inv_dx * dd.inv_dx;
+            if ((unsigned)qx <= TE - 3 && (unsigned)qy <= TE - 3 && (unsigned)qz <= TE - 3) {
+                const float4* t = tile + (qx * TE + qy) * TE + qz;
+                gather_sep(s, dd.dx, kC, [&](int a, int b, int c) { return t[(a * TE + b) * TE + c]; }, vel, Cm);
+            } else {
+                GlobalGather{gm, dd}(s, lx, ly, lz, bad, vel, Cm);
+            }
+        };
+        for (int j = threadIdx.x; j < U.z; j += blockDim.x) {
+            const int64_t i = (int64_t)U.y + j;
+            g2p_particle(i, g2p_load(P, i), P, dd, pp, fa, qsi, rec, err, scene, gather);
+        }
+    }
 }
 
 // The FP64 pass for the particles k_advance listed: same outputs.
@@ -549,7 +659,7 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx, bool two_pass,
-                  const SplitBatch& batch, bool store_vc) {
+                  const SplitBatch& batch, bool store_vc, bool g2p_tile) {
     const bool batched = batch.sv.chunks != nullptr;
     if (batched ? batch.n_chunks <= 0 : P.n <= 0) {  // no particle pass to empty the output grid's block list
         hipLaunchKernelGGL(k_set_zero, dim3(1), dim3(1), 0, s, gout.bcount);
@@ -562,8 +672,17 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
     const SceneView sv = batch.sv;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (mark) mark(ctx, SPLIT_ADVANCE, &e0, &e1);
-    // with timing events the launch records them around the dispatch itself
-    if (e0) {
+    // with timing events the launch records them around the dispatch itself.
+    // The G2P runs over the units (LDS-staged node tiles): one workgroup per
+    // unit, or a fixed grid walking a count known on the device only.
+    const int tg = batch.n_units_dev ? std::min(batch.n_units_bound, batch.deposit_groups) : n_units;
+    if (do_g2p && g2p_tile && tg > 0) {
+        if (e0)
+            hipExtLaunchKernelGGL(k_advance_tile, dim3(tg), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, rec, err,
+                                  sv, units);
+        else
+            hipLaunchKernelGGL(k_advance_tile, dim3(tg), dim3(256), 0, s, P, gin, d, ph, fa, rec, err, sv, units);
+    } else if (e0) {
         if (do_g2p)
             hipExtLaunchKernelGGL(k_advance<true>, dim3(g), dim3(256), 0, s, e0, e1, 0, P, gin, d, ph, fa, rec, err,
                                   sv);

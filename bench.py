@@ -60,7 +60,7 @@ BYTES_FAST = {
 }
 # Kernel that each phase's roofline describes (names as rocprofv3 lists them).
 KERNEL_OF = {
-    ("fast", "g2p"): "k_advance<true>",
+    ("fast", "g2p"): "k_advance_tile",
     ("fast", "p2g"): "k_p2g_cell<1>",
     ("fast", "update"): "k_update",
     ("reference", "p2g"): "k_p2g_ref",

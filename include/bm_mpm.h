@@ -272,7 +272,10 @@ int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallback
  * registers; always set -- the single-kernel fused form was retired and a
  * variant without bit 2 returns BM_ERR_NOT_SUPPORTED); bit 3 = the per-cell
  * deposit in two passes over each cell's particles (fewer registers,
- * particle data read twice).  Default 4.
+ * particle data read twice); bit 4 = the G2P per particle over the flat
+ * particle range, bit 5 = the G2P over the re-binning units with each unit's
+ * grid nodes staged in LDS (not both; with neither the staged form runs
+ * when the scene has at least 6144 units).  Default 4.
  * Values <= 0 / < 0 keep the current setting.  No effect in
  * BM_MODE_REFERENCE. */
 int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_every, int32_t variant);

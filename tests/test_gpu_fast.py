@@ -74,7 +74,7 @@ def test_fast_p2g_mass_bit_exact_and_folded_momentum():
     assert err < 2e-5, err
 
 
-@pytest.mark.parametrize("variant", [4, 12])
+@pytest.mark.parametrize("variant", [4, 12, 4 | 32, 12 | 32])
 def test_fast_steps_at_fp32_noise_floor(variant):
     P, prm, g, _ = _pair(12, variant=variant, strain=0.01, vel=0.05, cgrad=0.2)
     o32, o64 = _fp64_pair(P, prm)
@@ -91,7 +91,7 @@ def test_fast_steps_at_fp32_noise_floor(variant):
     _assert_at_noise_floor(g.download(), o32.get_particles(), o64.get_particles())
 
 
-@pytest.mark.parametrize("variant", [4, 12])
+@pytest.mark.parametrize("variant", [4, 12, 4 | 32])
 def test_fast_order_survives_rebinning(variant):
     P, prm, g, _ = _pair(13, sort_every=3, unit=64, variant=variant, strain=0.01, vel=0.05, cgrad=0.2)
     o32, o64 = _fp64_pair(P, prm)
@@ -117,6 +117,17 @@ def test_fast_matches_reference_mode_over_100_substeps():
     P64 = o64.get_particles()
     _assert_at_noise_floor(g.download(), o32.get_particles(), P64, fields="xF")
     _assert_at_noise_floor(r.download(), o32.get_particles(), P64, fields="xF")
+
+
+@pytest.mark.parametrize("variant", [4 | 16 | 32, 64])
+def test_unknown_variants_are_refused(variant):
+    """Bits 4 and 5 name the two G2P forms; asking for both, or for a bit
+    above 5, is an invalid argument."""
+    from basements_amd import BMError, MODE_FAST, Solver
+
+    g = Solver(DX, ORG, DIMS, 16, MODE_FAST)
+    with pytest.raises(BMError, match="INVALID_ARG"):
+        g.set_tuning(0, 0, variant)
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 3])
@@ -156,3 +167,42 @@ def test_fast_material_change_redeposits():
     o32.step(1e-4, 3)
     o64.step(1e-4, 3)
     _assert_at_noise_floor(g.download(), o32.get_particles(), o64.get_particles())
+
+
+@pytest.mark.parametrize("variant", [4 | 32, 4 | 16])
+def test_fast_particles_drifting_past_the_staged_tile(variant):
+    """The staged G2P (variant bit 5) stages each re-binning unit's 8^3 nodes
+    in LDS (the unit's block, 2 stencil layers and 1 drift layer each side)
+    and gathers a particle whose stencil base left that range straight from
+    the grid.  A block translating at 1.5 / 1.2 m/s with re-binning off for
+    100 substeps drifts up to 2.4 cells from its binned blocks, so both
+    gathers (and the deposit's off-tile path) run; the result stays at the
+    FP32 noise floor, as it does for the per-particle G2P (bit 4)."""
+    from basements_amd import MODE_FAST, Solver
+
+    dt, n = 5e-4, 100
+    P = random_scene(17, dx=DX, strain=0.01, vel=0.05, cgrad=0.2)
+    P["v"] = (P["v"] + np.array([1.5, 0.0, 1.2], np.float32)).astype(np.float32)
+    prm = params_from(g=(0.0, 0.0, 0.0))
+    g = Solver(DX, ORG, DIMS, P["x"].shape[0], MODE_FAST, sort_every=100000)
+    g.set_tuning(0, 0, variant)
+    g.set_material(prm.E, prm.nu, prm.rho)
+    g.set_plastic_alpha(prm.alpha)
+    g.set_floor_friction(prm.mu_floor)
+    g.set_gravity(prm.gravity)
+    g.upload(P)
+    o32, o64 = _fp64_pair(P, prm)
+    g.step(dt, n)
+    o32.step(dt, n)
+    o64.step(dt, n)
+    Pg = g.download()
+    # stencil base relative to the block the particle was binned in (from the
+    # initial state): the staged tile covers bases -1 .. 4 on each axis
+    def base(x):
+        return np.floor(x.astype(np.float64) / DX - 0.5).astype(np.int64) - np.array(ORG)
+
+    b0, b1 = base(P["x"]), base(Pg["x"])
+    rel = b1 - 4 * (b0 >> 2)
+    outside = ((rel < -1) | (rel > 4)).any(axis=1)
+    assert outside.sum() > 50, int(outside.sum())
+    _assert_at_noise_floor(Pg, o32.get_particles(), o64.get_particles())
