@@ -102,13 +102,15 @@ def oracle_runs():
     return out
 
 
-def _gpu_run(variant, mode):
+def _gpu_run(variant, mode, tuning=None):
     from basements_amd import BODY_DYNAMIC, M0, SHAPE_BOX, Body
 
     sc, c, v0 = _setup(variant)
     P = sc.particles
     prm = params_from(E=sc.E, nu=sc.nu, rho=sc.rho, alpha=sc.alpha, mu_floor=sc.mu_floor, g=sc.gravity)
     S = gpu_solver(P, sc.dx, sc.origin, sc.dims, prm, mode=mode)
+    if tuning is not None:
+        S.set_tuning(0, 0, tuning)
     S.set_body(Body(position=c, linear_velocity=v0, inv_mass=1.0, linear_damping=0.0, angular_damping=0.0,
                     applied_force=(0.0, -9.81, 0.0), half_extents=sc.tool["half"], friction=sc.tool["friction"],
                     mass=1.0, shape=SHAPE_BOX, integrate=BODY_DYNAMIC))
@@ -125,12 +127,15 @@ def _gpu_run(variant, mode):
     return S.download(), S.get_body()[0], rec
 
 
-@pytest.mark.parametrize("mode", [0, 1], ids=["reference", "fast"])
+# fast mode runs the box drop's 131k particles with the per-particle G2P by
+# default (too few units for the staged tiles); "fast_tiled" forces the G2P
+# over the re-binning units with staged tiles (the wheel bed's form)
+@pytest.mark.parametrize("mode,tuning", [(0, None), (1, None), (1, 4 | 32)], ids=["reference", "fast", "fast_tiled"])
 @pytest.mark.parametrize("variant", ["drop", "contact"])
-def test_box_drop_1000_substeps_at_fp32_noise_floor(oracle_runs, variant, mode):
+def test_box_drop_1000_substeps_at_fp32_noise_floor(oracle_runs, variant, mode, tuning):
     P32, r32 = oracle_runs[(variant, "f32")]
     P64, r64 = oracle_runs[(variant, "f64")]
-    Pg, body, rec = _gpu_run(variant, mode)
+    Pg, body, rec = _gpu_run(variant, mode, tuning)
     report = {}
     for key in ("x", "v", "F", "C"):
         floor = rel_err(P32[key], P64[key])
