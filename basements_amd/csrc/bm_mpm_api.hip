@@ -97,7 +97,7 @@ struct bm_mpm_solver {
         int phase;
         hipEvent_t a, b;
     };
-    int timing = 0;  // 1: every phase; 2: dispatch-recorded kernel events only
+    int timing = 0;  // 1: every phase; 2: dispatch-recorded kernel events only; 3: the particle pass's only
     std::vector<hipEvent_t> ev_pool;
     std::vector<Timed> pending;
     double t_ms[BM_NUM_PHASES] = {};
@@ -625,6 +625,7 @@ static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const
             return;
         }
         hh->end();
+        if (hh->timing == 3 && stage == SPLIT_DEPOSIT) return;  // level 3: the particle pass only
         *a = hh->take_event();
         *b = hh->take_event();
         if (*a && *b)
@@ -1042,7 +1043,7 @@ int bm_mpm_set_tuning(bm_mpm_handle h, int32_t unit_particles, int32_t sort_ever
 
 int bm_mpm_set_timing(bm_mpm_handle h, int32_t on) {
     if (int rc = check_handle(h)) return rc;
-    if (on < 0 || on > 2) return fail(BM_ERR_INVALID_ARG, "timing level %d not in 0..2", on);
+    if (on < 0 || on > 3) return fail(BM_ERR_INVALID_ARG, "timing level %d not in 0..3", on);
     h->timing = on;
     return BM_OK;
 }

@@ -534,11 +534,13 @@ class Solver:
         """Fast-mode launch parameters (bm_mpm_set_tuning); 0 / -1 keep the current value."""
         _check(lib().bm_mpm_set_tuning(self.h, int(unit_particles), int(sort_every), int(variant)))
 
-    def set_timing(self, on=True, kernels_only=False):
+    def set_timing(self, on=True, kernels_only=False, g2p_only=False):
         """on: per-phase HIP events.  kernels_only: only the fast form's G2P
         and P2G kernels, timed by events their own dispatches record (no
-        events between launches to perturb the stream)."""
-        _check(lib().bm_mpm_set_timing(self.h, (2 if kernels_only else 1) if on else 0))
+        events between launches to perturb the stream); g2p_only: the G2P
+        kernel's alone."""
+        level = (3 if g2p_only else 2 if kernels_only else 1) if on else 0
+        _check(lib().bm_mpm_set_timing(self.h, level))
 
     def read_timing(self, reset=True):
         """{phase: (total_ms, launches)} from HIP events on the handle's stream."""

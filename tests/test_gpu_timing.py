@@ -1,6 +1,7 @@
 """Device timing levels of a handle (bm_mpm_set_timing): level 1 records
 event pairs around every phase, level 2 only the fast form's G2P and P2G
-kernels through events that their own dispatches record."""
+kernels through events that their own dispatches record, level 3 the G2P
+kernel's alone."""
 from __future__ import annotations
 
 import pytest
@@ -30,6 +31,18 @@ def test_kernels_only_times_g2p_and_p2g_per_substep():
         assert tm[ph][1] == 0, ph
 
 
+def test_g2p_only_times_the_particle_pass_alone():
+    S = _solver()
+    S.step(DT, 2)
+    S.set_timing(True, g2p_only=True)
+    S.read_timing(reset=True)
+    S.step(DT, 6)
+    tm = S.read_timing(reset=True)
+    assert tm["g2p"][1] == 6 and tm["g2p"][0] > 0.0
+    for ph in ("clear", "p2g", "update", "couple", "sort", "other"):
+        assert tm[ph][1] == 0, ph
+
+
 def test_every_phase_level_and_off():
     S = _solver()
     S.step(DT, 2)
@@ -52,4 +65,4 @@ def test_unknown_level_is_refused():
 
     S = _solver()
     with pytest.raises(BMError, match="INVALID_ARG"):
-        _check(lib().bm_mpm_set_timing(S.h, 3))
+        _check(lib().bm_mpm_set_timing(S.h, 4))
