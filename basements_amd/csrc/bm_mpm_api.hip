@@ -98,6 +98,7 @@ struct bm_mpm_solver {
         hipEvent_t a, b;
     };
     int timing = 0;  // 1: every phase; 2: dispatch-recorded kernel events only; 3: the particle pass's only
+    BodyEnd pending_end;  // a coupled substep's body end left to the next fast particle pass
     std::vector<hipEvent_t> ev_pool;
     std::vector<Timed> pending;
     double t_ms[BM_NUM_PHASES] = {};
@@ -636,7 +637,8 @@ static void fast_kernels(bm_mpm_handle h, bool do_g2p, const GridBuf& gin, const
     if (ctx.adv_phase >= 0) h->open_phase = BM_PHASE_OTHER;  // the opening event now starts OTHER
     launch_split(do_g2p, h->pview(), h->bs.units, h->n_units, gin, gout, h->bever, h->dom, h->phys(), dt_g2p, dt_fold,
                  h->qscale_inv(), h->d_err, h->d_fallbacks, h->redo, h->n_redo, h->coef, h->stream, mark, &ctx,
-                 (h->variant & 8) != 0, SplitBatch{}, store_vc, g2p_tiled(h));
+                 (h->variant & 8) != 0, SplitBatch{}, store_vc, g2p_tiled(h), do_g2p ? h->pending_end : BodyEnd{});
+    if (do_g2p) h->pending_end = BodyEnd{};
 }
 
 static int p2g_fast(bm_mpm_handle h, float dt) {
@@ -861,16 +863,17 @@ int bm_mpm_get_body(bm_mpm_handle h, bm_body* body, bm_reaction* reaction) {
     return BM_OK;
 }
 
-int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* out) {
-    BM_TRACE("bm_mpm_couple_body");
-    if (int rc = check_handle(h)) return rc;
+// defer_end: the body's end of substep is left to the next fast particle
+// pass (h->pending_end), which runs it without a launch of its own.
+static int couple_body_impl(bm_mpm_handle h, int32_t method, float dt, bm_reaction* out, bool defer_end) {
     if (method < BM_COUPLER_M0_DIRICHLET || method > BM_COUPLER_M45_BSPLINE)
         return fail(BM_ERR_INVALID_ARG, "unknown coupler %d", method);
     if (!h->body_set) return fail(BM_ERR_STATE, "no device body set");
     if (h->phase != PH_UPDATED) return fail(BM_ERR_STATE, "couple needs an updated grid");
     h->begin(BM_PHASE_COUPLE);
-    launch_couple_body(method, h->d_body, h->d_col, h->grid[h->cur], h->bever, h->dom, dt, h->body_blocks,
-                       h->d_partials, h->d_out7, h->d_err, h->body_begun, h->stream);
+    h->pending_end = launch_couple_body(method, h->d_body, h->d_col, h->grid[h->cur], h->bever, h->dom, dt,
+                                        h->body_blocks, h->d_partials, h->d_out7, h->d_err, h->body_begun, h->stream,
+                                        defer_end && !out);
     h->body_begun = false;
     h->end();
     HIP_TRY(hipGetLastError());
@@ -886,6 +889,12 @@ int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* o
     return BM_OK;
 }
 
+int bm_mpm_couple_body(bm_mpm_handle h, int32_t method, float dt, bm_reaction* out) {
+    BM_TRACE("bm_mpm_couple_body");
+    if (int rc = check_handle(h)) return rc;
+    return couple_body_impl(h, method, dt, out, false);
+}
+
 int bm_mpm_step_coupled(bm_mpm_handle h, int32_t method, float dt, int32_t k) {
     BM_TRACE("bm_mpm_step_coupled");
     if (int rc = check_handle(h)) return rc;
@@ -896,8 +905,14 @@ int bm_mpm_step_coupled(bm_mpm_handle h, int32_t method, float dt, int32_t k) {
     for (int32_t i = 0; i < k; ++i) {
         if (int rc = bm_mpm_p2g(h, dt)) return rc;
         if (int rc = grid_update_impl(h, dt, true)) return rc;
-        if (int rc = bm_mpm_couple_body(h, method, dt, nullptr)) return rc;
-        if (int rc = g2p_impl(h, dt, i == k - 1)) return rc;
+        // fast mode: the body's end of substep rides in the particle pass
+        if (int rc = couple_body_impl(h, method, dt, nullptr, h->fast)) return rc;
+        const int rc = g2p_impl(h, dt, i == k - 1);
+        if (h->pending_end.body) {  // not taken by a particle pass (or the pass failed first)
+            launch_body_end(h->pending_end, h->stream);
+            h->pending_end = BodyEnd{};
+        }
+        if (rc) return rc;
     }
     return BM_OK;
 }

@@ -220,6 +220,17 @@ constexpr int COUPLE_WGS = 128;
 // Coupler workgroups per scene of a batch: a tool's scan range is a few
 // blocks, so most of COUPLE_WGS would find nothing to do in every scene.
 constexpr int COUPLE_BATCH_WGS = 16;
+// A device body's end of substep (body_device.hpp: sum the coupler's partial
+// reactions, integrate) handed to the particle pass, which runs it in the
+// first workgroup of its FP64 redo launch.  body == nullptr: none.
+struct BodyEnd {
+    bm_body* body = nullptr;
+    const double* partials = nullptr;
+    int nparts = 0;
+    double* out7 = nullptr;
+    float dt = 0.0f;
+};
+
 void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* bever, const Domain& d, float dt,
                    double* partials, double* out7, hipStream_t s);
 // Coupled substep of the device body: applied loads, collider from the body
@@ -227,9 +238,12 @@ void launch_couple(int method, const ColliderDev& b, const GridBuf& g, uint8_t* 
 // couple_blocks_bound), reaction into the body, integration.  err bit 2 is
 // set if the collider's range exceeded max_blocks (then nothing is coupled).
 // begun = true: the start was already done (fused into launch_update).
-void launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridBuf& g, uint8_t* bever,
-                        const Domain& d, float dt, int64_t max_blocks, double* partials, double* out7, int* err,
-                        bool begun, hipStream_t s);
+// defer_end: the body's end of substep is not launched but returned, for the
+// next particle pass to run (launch_split); otherwise it is launched here.
+BodyEnd launch_couple_body(int method, bm_body* body, ColliderDev* col, const GridBuf& g, uint8_t* bever,
+                           const Domain& d, float dt, int64_t max_blocks, double* partials, double* out7, int* err,
+                           bool begun, hipStream_t s, bool defer_end = false);
+void launch_body_end(const BodyEnd& be, hipStream_t s);
 // The device bodies of a batch (scene s: bodies[s], cols[s] built by the
 // update, methods[s]); partials hold COUPLE_WGS * 7 doubles and out7 8
 // doubles per scene.
@@ -300,7 +314,7 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*) = nullptr,
                   void* ctx = nullptr, bool two_pass = false, const SplitBatch& batch = SplitBatch{},
-                  bool store_vc = true, bool g2p_tile = true);
+                  bool store_vc = true, bool g2p_tile = true, const BodyEnd& body_end = BodyEnd{});
 
 // Units below which a single scene's G2P runs per particle over the flat
 // particle range: with few units the workgroups of the staged-tile G2P do not

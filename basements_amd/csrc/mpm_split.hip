@@ -23,6 +23,7 @@
 
 #include <algorithm>
 
+#include "body_device.hpp"
 #include "mpm_fast_device.hpp"
 
 namespace bm {
@@ -281,7 +282,11 @@ __global__ void __launch_bounds__(256) k_advance_tile(PView P, GridBuf gin, Doma
 // The FP64 pass for the particles k_advance listed: same outputs.
 template <bool DO_G2P>
 __global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domain d0, Phys ph0, FastArgs fa, float4* rec,
-                                                     int* err, unsigned long long* counters, SceneView sv) {
+                                                     int* err, unsigned long long* counters, SceneView sv,
+                                                     BodyEnd be) {
+    // the coupled substep's body end rides in the first workgroup (it only
+    // has to precede the next grid update)
+    if (be.body && blockIdx.x == 0) body_end(be.body, be.partials, be.nparts, be.out7, be.dt);
     const int cnt = *fa.n_redo;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
         const int64_t i = fa.redo[r];
@@ -659,10 +664,11 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
                   const GridBuf& gout, uint8_t* bever, const Domain& d, const Phys& ph, float dt_g2p, float dt_fold,
                   double qscale_inv, int* err, unsigned long long* counters, int* redo, int* n_redo, float* coef,
                   hipStream_t s, void (*mark)(void*, int, hipEvent_t*, hipEvent_t*), void* ctx, bool two_pass,
-                  const SplitBatch& batch, bool store_vc, bool g2p_tile) {
+                  const SplitBatch& batch, bool store_vc, bool g2p_tile, const BodyEnd& be) {
     const bool batched = batch.sv.chunks != nullptr;
     if (batched ? batch.n_chunks <= 0 : P.n <= 0) {  // no particle pass to empty the output grid's block list
         hipLaunchKernelGGL(k_set_zero, dim3(1), dim3(1), 0, s, gout.bcount);
+        launch_body_end(be, s);
         return;
     }
     float4* rec = (float4*)coef;  // 16 floats per particle (whole groups of 64: see rec_at)
@@ -696,10 +702,11 @@ void launch_split(bool do_g2p, const PView& P, const int4* units, int n_units, c
     }
     if (mark) mark(ctx, SPLIT_REDO, nullptr, nullptr);
     if (do_g2p)
-        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, err, counters, sv);
+        hipLaunchKernelGGL(k_advance_redo<true>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, err, counters, sv,
+                           be);
     else
-        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, err, counters,
-                           sv);
+        hipLaunchKernelGGL(k_advance_redo<false>, dim3(32), dim3(64), 0, s, P, gin, d, ph, fa, rec, err, counters, sv,
+                           be);
     e0 = e1 = nullptr;
     if (mark) mark(ctx, SPLIT_DEPOSIT, &e0, &e1);
     // a batch walks the device-counted units with a fixed grid
