@@ -531,8 +531,12 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
         const float4 r0 = pos[j];
         Stencil s;
         make_stencil(r0.x, r0.y, r0.z, inv_dx, s);
-        const int cx = s.b[0] - d.ox - tx0 - 1, cy = s.b[1] - d.oy - ty0 - 1, cz = s.b[2] - d.oz - tz0 - 1;
-        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < 4 && cy < 4 && cz < 4) {
+        const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
+        const int cx = lx - tx0 - 1, cy = ly - ty0 - 1, cz = lz - tz0 - 1;
+        // a stencil that leaves the box (a particle the G2P just moved into
+        // the box's last two cells) goes to the checked path below: the tile
+        // flush writes every node it holds
+        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < 4 && cy < 4 && cz < 4 && stencil_ok(d, lx, ly, lz)) {
             const int c = (cx * 4 + cy) * 4 + cz;
             S.cell[pi] = (unsigned char)c;
             S.rank[pi] = (unsigned short)atomicAdd(&S.cnt[c], 1);
@@ -576,7 +580,8 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
     __syncthreads();
 
     // ---- particles outside the block's cells (drifted since the last
-    // re-binning): LDS atomics inside the tile, global atomics outside
+    // re-binning) or with a stencil outside the box (flagged, not deposited):
+    // LDS atomics inside the tile, global atomics outside
     int off = 0;
     const int nl = S.nleft;
     for (int k = lane; k < nl; k += 64) {
