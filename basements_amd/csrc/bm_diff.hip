@@ -60,13 +60,41 @@ struct Box {
     int ox, oy, oz, nx, ny, nz;
 };
 
-// Parameters of one substep (the tool centre moves with the substep).
+// Parameters of one rollout; tc is the tool centre of the substep at hand
+// (par_at), tc0 its value at time 0.
 template <class R> struct Par {
     R dx, inv_dx, mu, lam, alpha, koh, mu_floor, g[3];
     int ref_svd;  // the reference's SVD stopping rule (forward comparisons only)
     int tool;
     R tc[3], he[3], Rl[9], Rw[9], vt[3], friction;
+    double tc0[3], vt0[3];
 };
+
+// ------------------------------------------------------------------ rollouts
+// A handle runs its rollouts side by side: each launch has one row of
+// workgroups (blockIdx.y) per rollout.  Rollout r's particle states sit
+// QCOMP * cap values after rollout r-1's in every state slot (the component
+// stride stays cap), its grid values (v values per node) at r * v * nn, its
+// gradient accumulators at r * GN; the particles' m, V0, x(0) and target are
+// one copy that every rollout reads.
+template <class R> __device__ __forceinline__ PS<R> roll(PS<R> s) {
+    s.p += (int64_t)blockIdx.y * QCOMP * s.cap;
+    return s;
+}
+__device__ __forceinline__ int64_t roll_off(int64_t per_rollout) { return (int64_t)blockIdx.y * per_rollout; }
+
+// This workgroup's rollout's parameters at time t: the tool centre is
+// tc0 + vt0 t, in double and unfused, so that every rollout and both
+// precisions round the same centre.
+template <class R> __device__ __forceinline__ Par<R> par_at(const Par<R>* __restrict__ Pv, double t) {
+#pragma clang fp contract(off)
+    Par<R> P = Pv[blockIdx.y];
+    if (P.tool) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) P.tc[a] = (R)(P.tc0[a] + P.vt0[a] * t);
+    }
+    return P;
+}
 
 // Gradient accumulators (double, device): tool velocity (3), alpha, koh, mu, lam.
 enum { GVT = 0, GALPHA = 3, GKOH = 4, GMU = 5, GLAM = 6, GLOSS = 7, GN = 8 };
@@ -76,6 +104,8 @@ template <class R> __device__ __forceinline__ int64_t node_index(const Box& b, i
     if (a < 0 || c < 0 || e < 0 || a >= b.nx || c >= b.ny || e >= b.nz) return -1;
     return ((int64_t)a * b.ny + c) * b.nz + e;
 }
+
+__device__ __forceinline__ int64_t box_nodes(const Box& b) { return (int64_t)b.nx * b.ny * b.nz; }
 
 template <class R> struct St {
     int b[3];
@@ -355,9 +385,14 @@ __device__ __forceinline__ void tile_flush(const TileBox& t, R (*tl)[STILE], con
 
 // ------------------------------------------------------------------ forward
 template <class R>
-__global__ void __launch_bounds__(64) k_p2g(PS<R> s, const R* m, const R* V0, int64_t n, Box b, Par<R> P, R dt,
-                                            R* gm, R* gq, int* err, const R* ks) {
+__global__ void __launch_bounds__(64) k_p2g(PS<R> s, const R* m, const R* V0, int64_t n, Box b,
+                                            const Par<R>* Pv, double t, R dt, R* gm, R* gq, int* err, const R* ks) {
     __shared__ R tl[4][STILE];
+    const Par<R> P = par_at(Pv, t);
+    s = roll(s);
+    gm += roll_off(4 * box_nodes(b));
+    gq += roll_off(4 * box_nodes(b));
+    if (ks) ks += roll_off(9 * s.cap);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < n;
     R x[3] = {R(0), R(0), R(0)};
@@ -497,10 +532,15 @@ __device__ __forceinline__ NodeFwd<R> node_forward(R m, v3<R> q, int j_node, v3<
 // clear them for the next P2G once read (clear_mq: the grid's last reader
 // this substep).
 template <class R>
-__global__ void __launch_bounds__(256) k_update(Box b, Par<R> P, R dt, R* gm, R* gq, R* gu, int clear_mq) {
+__global__ void __launch_bounds__(256) k_update(Box b, const Par<R>* Pv, double t, R dt, R* gm, R* gq, R* gu,
+                                                int clear_mq) {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nn = (int64_t)b.nx * b.ny * b.nz;
+    const int64_t nn = box_nodes(b);
     if (id >= nn) return;
+    const Par<R> P = par_at(Pv, t);
+    gm += roll_off(4 * nn);
+    gq += roll_off(4 * nn);
+    gu += roll_off(3 * nn);
     const R m = gm[id];
     const v3<R> q = mk(gq[3 * id], gq[3 * id + 1], gq[3 * id + 2]);
     if (clear_mq) {
@@ -571,10 +611,15 @@ __device__ __forceinline__ void g2p_forward(const R x[3], const m3<R>& F, const 
 }
 
 template <class R>
-__global__ void __launch_bounds__(256) k_g2p(PS<R> s, PS<R> out, int64_t n, Box b, Par<R> P, R dt, const R* gu,
-                                             int* err, R* ks) {
+__global__ void __launch_bounds__(256) k_g2p(PS<R> s, PS<R> out, int64_t n, Box b, const Par<R>* Pv, double t,
+                                             R dt, const R* gu, int* err, R* ks) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const Par<R> P = par_at(Pv, t);
+    s = roll(s);
+    out = roll(out);
+    gu += roll_off(3 * box_nodes(b));
+    if (ks) ks += roll_off(9 * out.cap);
     const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
     const m3<R> F = load_m(s, QF, p);
     G2PFwd<R> f;
@@ -606,9 +651,12 @@ __global__ void __launch_bounds__(256) k_g2p(PS<R> s, PS<R> out, int64_t n, Box 
 // The Kirchhoff stress of each particle's F (component c of particle p at
 // ks[c * cap + p]): the first deposit after a checkpoint reads it.
 template <class R>
-__global__ void __launch_bounds__(256) k_stress(PS<R> s, int64_t n, Par<R> P, R* ks) {
+__global__ void __launch_bounds__(256) k_stress(PS<R> s, int64_t n, const Par<R>* Pv, R* ks) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const Par<R> P = par_at(Pv, 0.0);
+    s = roll(s);
+    ks += roll_off(9 * s.cap);
     const m3<R> K = stress(load_m(s, QF, p), P.mu, P.lam, P.ref_svd);
 #pragma unroll
     for (int c = 0; c < 9; ++c) ks[c * s.cap + p] = K.a[c];
@@ -620,6 +668,9 @@ template <class R>
 __global__ void __launch_bounds__(256) k_loss(PS<R> fin, const R* x0, const R* tgt, int64_t n, PS<R> bar,
                                               double* acc) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    fin = roll(fin);
+    bar = roll(bar);
+    acc += roll_off(GN);
     R l = R(0);
     if (p < n) {
         const R inv = R(1) / (R)n;
@@ -640,9 +691,17 @@ __global__ void __launch_bounds__(256) k_loss(PS<R> fin, const R* x0, const R* t
 // the old state (aout; its v and C parts are zeroed for the P2G adjoint) and
 // the adjoint of the node velocities (gub, accumulated).
 template <class R>
-__global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, int64_t n, Box b, Par<R> P, R dt,
-                                                const R* gu, R* gub, double* grad) {
+__global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, int64_t n, Box b,
+                                                const Par<R>* Pv, double t, R dt, const R* gu, R* gub,
+                                                double* grad) {
     __shared__ R tl[3][STILE];
+    const Par<R> P = par_at(Pv, t);
+    s = roll(s);
+    ain = roll(ain);
+    aout = roll(aout);
+    gu += roll_off(3 * box_nodes(b));
+    gub += roll_off(3 * box_nodes(b));
+    grad += roll_off(GN);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     R galpha = R(0), gkoh = R(0);
     int base[3] = {0, 0, 0};
@@ -747,10 +806,17 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
 // Grid: from the node-velocity adjoint to the adjoint of (m, q); the tool
 // velocity's adjoint from the M0 nodes.
 template <class R>
-__global__ void __launch_bounds__(256) k_update_adj(Box b, Par<R> P, R dt, R* gm, R* gq, R* gub,
-                                                    R* gqb, R* gmb, double* grad, int clear_mq) {
+__global__ void __launch_bounds__(256) k_update_adj(Box b, const Par<R>* Pv, double t, R dt, R* gm, R* gq,
+                                                    R* gub, R* gqb, R* gmb, double* grad, int clear_mq) {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nn = (int64_t)b.nx * b.ny * b.nz;
+    const int64_t nn = box_nodes(b);
+    const Par<R> P = par_at(Pv, t);
+    gm += roll_off(4 * nn);
+    gq += roll_off(4 * nn);
+    gub += roll_off(3 * nn);
+    gqb += roll_off(3 * nn);
+    gmb += roll_off(nn);
+    grad += roll_off(GN);
     v3<R> vtb = mk(R(0), R(0), R(0));
     if (id < nn) {
         // the substep's last reader of (m, q) and of the velocity adjoint:
@@ -816,9 +882,16 @@ __global__ void __launch_bounds__(256) k_update_adj(Box b, Par<R> P, R dt, R* gm
 // P2G: from the adjoint of (m, q) to the adjoint of x, v, C, F (added to
 // aout) and of mu and lambda.
 template <class R>
-__global__ void __launch_bounds__(256) k_p2g_adj(PS<R> s, const R* m, const R* V0, int64_t n, Box b, Par<R> P, R dt,
-                                                 const R* gqb, const R* gmb, PS<R> aout, double* grad) {
+__global__ void __launch_bounds__(256) k_p2g_adj(PS<R> s, const R* m, const R* V0, int64_t n, Box b,
+                                                 const Par<R>* Pv, double t, R dt, const R* gqb, const R* gmb,
+                                                 PS<R> aout, double* grad) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const Par<R> P = par_at(Pv, t);
+    s = roll(s);
+    aout = roll(aout);
+    gqb += roll_off(3 * box_nodes(b));
+    gmb += roll_off(box_nodes(b));
+    grad += roll_off(GN);
     R gmu = R(0), glam = R(0);
     if (p < n) {
         const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
@@ -972,8 +1045,10 @@ inline int pblocks(int64_t n) { return (int)((n + PWG - 1) / PWG); }
 
 struct bm_diff {
     bm_diff_desc desc{};
-    bm_diff_params prm{};
-    bool has_prm = false;
+    int nr = 1;  // rollouts run side by side
+    std::vector<bm_diff_params> prm;  // per rollout
+    std::vector<char> prm_set;
+    void* pard = nullptr;  // the rollouts' Par<R>, uploaded by each forward
     hipStream_t stream = nullptr;
     bool own_stream = false;
     bool fp64 = false;
@@ -983,7 +1058,8 @@ struct bm_diff {
     size_t rs = 4;  // bytes per scalar
     // device buffers (scalar type per precision)
     void *m = nullptr, *V0 = nullptr, *x0 = nullptr, *tgt = nullptr, *init = nullptr;
-    void *gm = nullptr, *gq = nullptr, *gu = nullptr, *gub = nullptr, *gqb = nullptr, *gmb = nullptr;
+    // gm: per rollout the node masses (nn) then momenta (3 nn), gq = gm + nn
+    void *gm = nullptr, *gu = nullptr, *gub = nullptr, *gqb = nullptr, *gmb = nullptr;
     void *ck = nullptr, *seg = nullptr, *abar = nullptr, *abar2 = nullptr;
     // the backward's grids of one segment: (m, q) and u of each of its substeps
     void *sgmq = nullptr, *sgu = nullptr;
@@ -1010,10 +1086,44 @@ struct bm_diff {
 
 static int check_diff(bm_diff_handle h) { return h ? BM_OK : fail(BM_ERR_INVALID_ARG, "null diff handle"); }
 
-template <class R> static Par<R> params_at(const bm_diff_handle h, int step) {
-    const bm_diff_params& q = h->prm;
+static int check_params(const bm_diff_params* p) {
+    if (!p) return fail(BM_ERR_INVALID_ARG, "null params");
+    if (!(p->young_modulus > 0.0) || !(p->poisson_ratio > -1.0 && p->poisson_ratio < 0.5))
+        return fail(BM_ERR_INVALID_ARG, "bad elastic constants");
+    if (p->has_tool && !(p->tool_half_extents[0] > 0 && p->tool_half_extents[1] > 0 && p->tool_half_extents[2] > 0))
+        return fail(BM_ERR_INVALID_ARG, "tool half extents must be > 0");
+    return BM_OK;
+}
+
+// Every rollout has its parameters.
+static bool all_params_set(const bm_diff_handle h) {
+    return std::all_of(h->prm_set.begin(), h->prm_set.end(), [](char c) { return c != 0; });
+}
+
+// The gradient of one rollout's loss from its accumulators (g: GN values):
+// the solver's constants (alpha, koh, mu, lambda) chained to the caller's
+// parameters (friction angle, Young's modulus).
+static void grad_of(const bm_diff_params& q, const double* g, bm_diff_grad* out) {
+    const double E = q.young_modulus, nu = q.poisson_ratio;
+    const double mu = E / (2.0 * (1.0 + nu)), lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    // koh = (3 lam + 2 mu) / (2 mu)
+    const double gmu = g[GMU] + g[GKOH] * (-3.0 * lam / (2.0 * mu * mu));
+    const double glam = g[GLAM] + g[GKOH] * (3.0 / (2.0 * mu));
+    const double phi = q.friction_deg * M_PI / 180.0, s = std::sin(phi);
+    const double dalpha_dphi_deg = std::sqrt(2.0 / 3.0) * 6.0 * std::cos(phi) / ((3.0 - s) * (3.0 - s)) * M_PI / 180.0;
+    for (int a = 0; a < 3; ++a) out->tool_velocity[a] = g[GVT + a];
+    out->friction_deg = g[GALPHA] * dalpha_dphi_deg;
+    out->young_modulus = gmu / (2.0 * (1.0 + nu)) + glam * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    out->alpha = g[GALPHA];
+    out->mu = gmu;
+    out->lambda = glam;
+}
+
+// A rollout's parameters in the solver's constants (the tool centre at time
+// 0; the kernels move it, par_at).
+template <class R> static Par<R> par_of(const bm_diff_desc& desc, const bm_diff_params& q) {
     Par<R> P{};
-    P.dx = (R)h->desc.dx;
+    P.dx = (R)desc.dx;
     P.inv_dx = R(1) / P.dx;
     const double E = q.young_modulus, nu = q.poisson_ratio;
     const double mu = E / (2.0 * (1.0 + nu)), lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
@@ -1023,13 +1133,14 @@ template <class R> static Par<R> params_at(const bm_diff_handle h, int step) {
     P.alpha = (R)(std::sqrt(2.0 / 3.0) * 2.0 * sphi / (3.0 - sphi));
     P.koh = (R)((3.0 * lam + 2.0 * mu) / (2.0 * mu));
     P.mu_floor = (R)q.floor_friction;
-    P.ref_svd = h->desc.reference_svd ? 1 : 0;
+    P.ref_svd = desc.reference_svd ? 1 : 0;
     for (int a = 0; a < 3; ++a) P.g[a] = (R)q.gravity[a];
     P.tool = q.has_tool ? 1 : 0;
     if (P.tool) {
-        const double t = (double)step * (double)h->dt;
         for (int a = 0; a < 3; ++a) {
-            P.tc[a] = (R)(q.tool_center[a] + q.tool_velocity[a] * t);
+            P.tc0[a] = q.tool_center[a];
+            P.vt0[a] = q.tool_velocity[a];
+            P.tc[a] = (R)q.tool_center[a];
             P.he[a] = (R)q.tool_half_extents[a];
             P.vt[a] = (R)q.tool_velocity[a];
         }
@@ -1057,8 +1168,25 @@ template <class R> static Par<R> params_at(const bm_diff_handle h, int step) {
     return P;
 }
 
-template <class R> static PS<R> slot(void* base, int64_t cap, int64_t k) {
-    return PS<R>{static_cast<R*>(base) + k * QCOMP * cap, cap};
+// State slot k of a slot array: the states of all the handle's rollouts.
+template <class R> static PS<R> slot(const bm_diff_handle h, void* base, int64_t k) {
+    return PS<R>{static_cast<R*>(base) + k * h->nr * QCOMP * h->cap, h->cap};
+}
+// Bytes of one state slot.
+static size_t slot_bytes(const bm_diff_handle h) { return h->rs * QCOMP * (size_t)h->cap * h->nr; }
+// Time of substep `step` of the last forward (the kernels' tool centre).
+static double time_at(const bm_diff_handle h, int step) { return (double)step * (double)h->dt; }
+template <class R> static const Par<R>* pars(const bm_diff_handle h) { return static_cast<const Par<R>*>(h->pard); }
+// One row of workgroups per rollout over n items, w per workgroup.
+static dim3 rows(const bm_diff_handle h, int64_t n, int w) { return dim3((unsigned)((n + w - 1) / w), (unsigned)h->nr); }
+
+// Every rollout's parameters to the device, in stream order.
+template <class R> static int upload_pars(bm_diff_handle h) {
+    std::vector<Par<R>> staging;
+    for (int r = 0; r < h->nr; ++r) staging.push_back(par_of<R>(h->desc, h->prm[(size_t)r]));
+    HIP_TRY(hipMemcpyAsync(h->pard, staging.data(), sizeof(Par<R>) * h->nr, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));  // before `staging` goes
+    return BM_OK;
 }
 
 // The stress buffer the deposits read, or null where each deposit factors F
@@ -1068,36 +1196,36 @@ template <class R> static R* stress_buffer(bm_diff_handle h) {
 }
 
 // The stress of state `st` into the buffer, before its first deposit.
-template <class R> static void seed_stress(bm_diff_handle h, PS<R> st, int step) {
+template <class R> static void seed_stress(bm_diff_handle h, PS<R> st) {
     if (R* ks = stress_buffer<R>(h))
-        hipLaunchKernelGGL(k_stress<R>, dim3(blocks_for(h->n)), dim3(256), 0, h->stream, st, h->n,
-                           params_at<R>(h, step), ks);
+        hipLaunchKernelGGL(k_stress<R>, rows(h, h->n, 256), dim3(256), 0, h->stream, st, h->n, pars<R>(h), ks);
 }
 
 // Zero (m, q): once per forward or backward call.  Every substep then leaves
 // them zero for the next one (k_update or k_update_adj clears each node after
 // its last read).
 template <class R> static void clear_mass_momentum(bm_diff_handle h) {
-    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(h->nn)), dim3(256), 0, h->stream, (R*)h->gm, h->nn);
-    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * h->nn)), dim3(256), 0, h->stream, (R*)h->gq, 3 * h->nn);
+    const int64_t v = 4 * h->nn * h->nr;
+    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(v)), dim3(256), 0, h->stream, (R*)h->gm, v);
 }
 
 // One forward substep from state `in` to state `out` (grid velocity left in
 // gu; m and q zero on entry and on return).
 template <class R> static void substep(bm_diff_handle h, PS<R> in, PS<R> out, int step) {
-    const Par<R> P = params_at<R>(h, step);
+    const Par<R>* const P = pars<R>(h);
+    const double t = time_at(h, step);
     const R dt = (R)h->dt;
     R* const ks = stress_buffer<R>(h);
-    const int64_t nn = h->nn;
+    const int64_t nn = h->nn, nr = h->nr;
     // kept grids: this substep's own (m, q, u), left in place for the backward
-    R* const gm = h->all_grids ? static_cast<R*>(h->agmq) + 4 * nn * step : (R*)h->gm;
-    R* const gq = h->all_grids ? gm + nn : (R*)h->gq;
-    R* const gu = h->all_grids ? static_cast<R*>(h->agu) + 3 * nn * step : (R*)h->gu;
-    hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, (const R*)h->m,
-                       (const R*)h->V0, h->n, h->box, P, dt, gm, gq, h->err, (const R*)ks);
-    hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm, gq, gu,
+    R* const gm = h->all_grids ? static_cast<R*>(h->agmq) + 4 * nn * nr * step : (R*)h->gm;
+    R* const gq = gm + nn;
+    R* const gu = h->all_grids ? static_cast<R*>(h->agu) + 3 * nn * nr * step : (R*)h->gu;
+    hipLaunchKernelGGL(k_p2g<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, in, (const R*)h->m,
+                       (const R*)h->V0, h->n, h->box, P, t, dt, gm, gq, h->err, (const R*)ks);
+    hipLaunchKernelGGL(k_update<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm, gq, gu,
                        h->all_grids ? 0 : 1);
-    hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, in, out, h->n, h->box, P, dt,
+    hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, in, out, h->n, h->box, P, t, dt,
                        (const R*)gu, h->err, ks);
 }
 
@@ -1126,21 +1254,24 @@ static int read_err(bm_diff_handle h) {
     return BM_OK;
 }
 
-template <class R> static int forward_t(bm_diff_handle h, int substeps, double* loss) {
+// The forward of every rollout; losses[r] = rollout r's loss (may be null).
+template <class R> static int forward_t(bm_diff_handle h, int substeps, double* losses) {
     const int K = h->K;
+    const int64_t nr = h->nr;
     // checkpoint slot c holds the state after min(c K, substeps) substeps
     const int64_t nck = (substeps + K - 1) / K + 1;
     if (nck > h->ck_slots) {
-        if (int rc = ensure(&h->ck, h->rs * QCOMP * h->cap * nck)) return rc;
+        if (int rc = ensure(&h->ck, slot_bytes(h) * nck)) return rc;
         h->ck_slots = nck;
     }
     if (h->seg_slots < K) {
-        if (int rc = ensure(&h->seg, h->rs * QCOMP * h->cap * K)) return rc;
+        if (int rc = ensure(&h->seg, slot_bytes(h) * K)) return rc;
         h->seg_slots = K;
     }
-    HIP_TRY(hipMemcpyAsync(h->ck, h->init, h->rs * QCOMP * h->cap, hipMemcpyDeviceToDevice, h->stream));
-    PS<R> cur = slot<R>(h->ck, h->cap, 0);
-    const int64_t kept_bytes = (int64_t)substeps * 7 * h->nn * (int64_t)h->rs;
+    if (int rc = upload_pars<R>(h)) return rc;
+    HIP_TRY(hipMemcpyAsync(h->ck, h->init, slot_bytes(h), hipMemcpyDeviceToDevice, h->stream));
+    PS<R> cur = slot<R>(h, h->ck, 0);
+    const int64_t kept_bytes = (int64_t)substeps * 7 * h->nn * nr * (int64_t)h->rs;
     h->all_grids = kept_bytes <= h->grid_budget;
     if (h->all_grids && h->all_slots < substeps) {
         // keep the grids only where they fit the device's free memory with
@@ -1151,8 +1282,8 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
                        kept_bytes + (int64_t(1) << 30) <= (int64_t)free_b;
         if (h->all_slots > 0 || !h->all_grids) release_kept_grids(h);
         if (h->all_grids) {
-            h->all_grids = ensure(&h->agmq, h->rs * 4 * h->nn * substeps) == BM_OK &&
-                           ensure(&h->agu, h->rs * 3 * h->nn * substeps) == BM_OK;
+            h->all_grids = ensure(&h->agmq, h->rs * 4 * h->nn * nr * substeps) == BM_OK &&
+                           ensure(&h->agu, h->rs * 3 * h->nn * nr * substeps) == BM_OK;
             if (h->all_grids) {
                 h->all_slots = substeps;
             } else {
@@ -1162,33 +1293,34 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
         }
     }
     if (h->all_grids) {
-        hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(4 * h->nn * substeps)), dim3(256), 0, h->stream,
-                           static_cast<R*>(h->agmq), 4 * h->nn * substeps);
+        hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(4 * h->nn * nr * substeps)), dim3(256), 0, h->stream,
+                           static_cast<R*>(h->agmq), 4 * h->nn * nr * substeps);
     } else {
         clear_mass_momentum<R>(h);
     }
-    seed_stress<R>(h, cur, 0);
+    seed_stress<R>(h, cur);
     int pp = 0;  // states between checkpoints alternate between two scratch slots
     for (int s = 0; s < substeps; ++s) {
         const int ns = s + 1;
         PS<R> nxt;
         if (ns % K == 0 || ns == substeps) {
-            nxt = slot<R>(h->ck, h->cap, (ns + K - 1) / K);
+            nxt = slot<R>(h, h->ck, (ns + K - 1) / K);
         } else {
-            nxt = slot<R>(h->seg, h->cap, pp);
+            nxt = slot<R>(h, h->seg, pp);
             pp ^= 1;
         }
         substep<R>(h, cur, nxt, s);
         cur = nxt;
     }
-    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
-    hipLaunchKernelGGL(k_loss<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, cur, (const R*)h->x0,
-                       (const R*)h->tgt, h->n, slot<R>(h->abar, h->cap, 0), h->grad);
+    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN * nr, h->stream));
+    hipLaunchKernelGGL(k_loss<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, cur, (const R*)h->x0,
+                       (const R*)h->tgt, h->n, slot<R>(h, h->abar, 0), h->grad);
     HIP_TRY(hipGetLastError());
     if (int rc = read_err(h)) return rc;
-    double g[GN];
-    HIP_TRY(hipMemcpy(g, h->grad, sizeof(g), hipMemcpyDeviceToHost));
-    if (loss) *loss = g[GLOSS];
+    std::vector<double> g((size_t)(GN * nr));
+    HIP_TRY(hipMemcpy(g.data(), h->grad, sizeof(double) * g.size(), hipMemcpyDeviceToHost));
+    if (losses)
+        for (int64_t r = 0; r < nr; ++r) losses[r] = g[(size_t)(r * GN + GLOSS)];
     h->substeps = substeps;
     h->have_forward = true;
     return BM_OK;
@@ -1199,8 +1331,8 @@ template <class R> static int forward_t(bm_diff_handle h, int substeps, double* 
 // the slots), so backward can be called again after one forward.
 template <class R> static void seed_loss_adjoint(bm_diff_handle h) {
     const int64_t last = (h->substeps + h->K - 1) / h->K;
-    hipLaunchKernelGGL(k_loss<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, slot<R>(h->ck, h->cap, last),
-                       (const R*)h->x0, (const R*)h->tgt, h->n, slot<R>(h->abar, h->cap, 0), h->grad);
+    hipLaunchKernelGGL(k_loss<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, slot<R>(h, h->ck, last),
+                       (const R*)h->x0, (const R*)h->tgt, h->n, slot<R>(h, h->abar, 0), h->grad);
 }
 
 // The backward over the grids the forward kept: each segment's particle
@@ -1210,39 +1342,41 @@ template <class R> static int backward_kept(bm_diff_handle h, double* g) {
     const int K = h->K;
     const int T = h->substeps;
     const int nseg = (T + K - 1) / K;
-    const int64_t nn = h->nn;
+    const int64_t nn = h->nn, nr = h->nr;
     R* const gmq = static_cast<R*>(h->agmq);
     R* const gus = static_cast<R*>(h->agu);
-    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
+    const Par<R>* const P = pars<R>(h);
+    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN * nr, h->stream));
     seed_loss_adjoint<R>(h);
-    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * nn)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn);
-    PS<R> a = slot<R>(h->abar, h->cap, 0), a2 = slot<R>(h->abar2, h->cap, 0);
+    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * nn * nr)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn * nr);
+    PS<R> a = slot<R>(h, h->abar, 0), a2 = slot<R>(h, h->abar2, 0);
     const R dt = (R)h->dt;
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * K, len = std::min(K, T - s0);
-        hipMemcpyAsync(slot<R>(h->seg, h->cap, 0).p, slot<R>(h->ck, h->cap, sg).p, h->rs * QCOMP * h->cap,
-                       hipMemcpyDeviceToDevice, h->stream);
+        hipMemcpyAsync(slot<R>(h, h->seg, 0).p, slot<R>(h, h->ck, sg).p, slot_bytes(h), hipMemcpyDeviceToDevice,
+                       h->stream);
         for (int j = 0; j + 1 < len; ++j)
-            hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, slot<R>(h->seg, h->cap, j),
-                               slot<R>(h->seg, h->cap, j + 1), h->n, h->box, params_at<R>(h, s0 + j), dt,
-                               (const R*)(gus + 3 * nn * (s0 + j)), h->err, (R*)nullptr);
+            hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, slot<R>(h, h->seg, j),
+                               slot<R>(h, h->seg, j + 1), h->n, h->box, P, time_at(h, s0 + j), dt,
+                               (const R*)(gus + 3 * nn * nr * (s0 + j)), h->err, (R*)nullptr);
         for (int j = len - 1; j >= 0; --j) {
-            const PS<R> sj = slot<R>(h->seg, h->cap, j);
-            const Par<R> P = params_at<R>(h, s0 + j);
-            R* const gm = gmq + 4 * nn * (s0 + j);
-            R* const gu = gus + 3 * nn * (s0 + j);
-            hipLaunchKernelGGL(k_g2p_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, a, a2, h->n,
-                               h->box, P, dt, (const R*)gu, (R*)h->gub, h->grad);
-            hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
+            const PS<R> sj = slot<R>(h, h->seg, j);
+            const double t = time_at(h, s0 + j);
+            R* const gm = gmq + 4 * nn * nr * (s0 + j);
+            R* const gu = gus + 3 * nn * nr * (s0 + j);
+            hipLaunchKernelGGL(k_g2p_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, a, a2, h->n, h->box,
+                               P, t, dt, (const R*)gu, (R*)h->gub, h->grad);
+            hipLaunchKernelGGL(k_update_adj<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm,
                                gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad, 0);
-            hipLaunchKernelGGL(k_p2g_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
-                               (const R*)h->V0, h->n, h->box, P, dt, (const R*)h->gqb, (const R*)h->gmb, a2, h->grad);
+            hipLaunchKernelGGL(k_p2g_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
+                               (const R*)h->V0, h->n, h->box, P, t, dt, (const R*)h->gqb, (const R*)h->gmb, a2,
+                               h->grad);
             std::swap(a, a2);
         }
     }
     HIP_TRY(hipGetLastError());
     if (int rc = read_err(h)) return rc;
-    HIP_TRY(hipMemcpy(g, h->grad, sizeof(double) * GN, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(g, h->grad, sizeof(double) * GN * nr, hipMemcpyDeviceToHost));
     return BM_OK;
 }
 
@@ -1250,73 +1384,74 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
     const int K = h->K;
     const int T = h->substeps;
     const int nseg = (T + K - 1) / K;
-    const int64_t nn = h->nn;
+    const int64_t nn = h->nn, nr = h->nr;
     if (h->all_grids) return backward_kept<R>(h, g);
     if (h->grid_slots < K) {
-        if (int rc = ensure(&h->sgmq, h->rs * 4 * nn * K)) return rc;
-        if (int rc = ensure(&h->sgu, h->rs * 3 * nn * K)) return rc;
+        if (int rc = ensure(&h->sgmq, h->rs * 4 * nn * nr * K)) return rc;
+        if (int rc = ensure(&h->sgu, h->rs * 3 * nn * nr * K)) return rc;
         h->grid_slots = K;
     }
     R* const gmq = static_cast<R*>(h->sgmq);
     R* const gus = static_cast<R*>(h->sgu);
-    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN, h->stream));
+    const Par<R>* const P = pars<R>(h);
+    HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN * nr, h->stream));
     seed_loss_adjoint<R>(h);
     // the slots' (m, q) and the velocity adjoint start at zero; the update's
     // adjoint clears each after its last read, so they stay zero between uses
-    for (int j = 0; j < K; ++j)
-        hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(4 * nn)), dim3(256), 0, h->stream, gmq + 4 * nn * j, 4 * nn);
-    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * nn)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn);
-    PS<R> a = slot<R>(h->abar, h->cap, 0), a2 = slot<R>(h->abar2, h->cap, 0);
+    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(4 * nn * nr * K)), dim3(256), 0, h->stream, gmq, 4 * nn * nr * K);
+    hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * nn * nr)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn * nr);
+    PS<R> a = slot<R>(h, h->abar, 0), a2 = slot<R>(h, h->abar2, 0);
     const R dt = (R)h->dt;
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * K, len = std::min(K, T - s0);
         // recompute the segment from its checkpoint, keeping each substep's
         // grid (m, q, u) for the reverse walk; the last substep's particle
         // update is the next checkpoint, already stored
-        hipMemcpyAsync(slot<R>(h->seg, h->cap, 0).p, slot<R>(h->ck, h->cap, sg).p, h->rs * QCOMP * h->cap,
-                       hipMemcpyDeviceToDevice, h->stream);
-        seed_stress<R>(h, slot<R>(h->seg, h->cap, 0), s0);
+        hipMemcpyAsync(slot<R>(h, h->seg, 0).p, slot<R>(h, h->ck, sg).p, slot_bytes(h), hipMemcpyDeviceToDevice,
+                       h->stream);
+        seed_stress<R>(h, slot<R>(h, h->seg, 0));
         R* const ks = stress_buffer<R>(h);
         for (int j = 0; j < len; ++j) {
-            const Par<R> P = params_at<R>(h, s0 + j);
-            R* const gm = gmq + 4 * nn * j;
-            R* const gu = gus + 3 * nn * j;
-            hipLaunchKernelGGL(k_p2g<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, slot<R>(h->seg, h->cap, j),
-                               (const R*)h->m, (const R*)h->V0, h->n, h->box, P, dt, gm, gm + nn, h->err,
+            const double t = time_at(h, s0 + j);
+            R* const gm = gmq + 4 * nn * nr * j;
+            R* const gu = gus + 3 * nn * nr * j;
+            hipLaunchKernelGGL(k_p2g<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, slot<R>(h, h->seg, j),
+                               (const R*)h->m, (const R*)h->V0, h->n, h->box, P, t, dt, gm, gm + nn, h->err,
                                (const R*)ks);
-            hipLaunchKernelGGL(k_update<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
-                               gm + nn, gu, 0);
+            hipLaunchKernelGGL(k_update<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm, gm + nn,
+                               gu, 0);
             if (j + 1 < len)
-                hipLaunchKernelGGL(k_g2p<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream,
-                                   slot<R>(h->seg, h->cap, j), slot<R>(h->seg, h->cap, j + 1), h->n, h->box, P, dt,
-                                   (const R*)gu, h->err, ks);
+                hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, slot<R>(h, h->seg, j),
+                                   slot<R>(h, h->seg, j + 1), h->n, h->box, P, t, dt, (const R*)gu, h->err, ks);
         }
         for (int j = len - 1; j >= 0; --j) {
-            const PS<R> sj = slot<R>(h->seg, h->cap, j);
-            const Par<R> P = params_at<R>(h, s0 + j);
-            R* const gm = gmq + 4 * nn * j;
-            R* const gu = gus + 3 * nn * j;
-            hipLaunchKernelGGL(k_g2p_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, a, a2, h->n,
-                               h->box, P, dt, (const R*)gu, (R*)h->gub, h->grad);
-            hipLaunchKernelGGL(k_update_adj<R>, dim3(blocks_for(nn)), dim3(256), 0, h->stream, h->box, P, dt, gm,
+            const PS<R> sj = slot<R>(h, h->seg, j);
+            const double t = time_at(h, s0 + j);
+            R* const gm = gmq + 4 * nn * nr * j;
+            R* const gu = gus + 3 * nn * nr * j;
+            hipLaunchKernelGGL(k_g2p_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, a, a2, h->n, h->box,
+                               P, t, dt, (const R*)gu, (R*)h->gub, h->grad);
+            hipLaunchKernelGGL(k_update_adj<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm,
                                gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad, 1);
-            hipLaunchKernelGGL(k_p2g_adj<R>, dim3(pblocks(h->n)), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
-                               (const R*)h->V0, h->n, h->box, P, dt, (const R*)h->gqb, (const R*)h->gmb, a2, h->grad);
+            hipLaunchKernelGGL(k_p2g_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
+                               (const R*)h->V0, h->n, h->box, P, t, dt, (const R*)h->gqb, (const R*)h->gmb, a2,
+                               h->grad);
             std::swap(a, a2);
         }
     }
     HIP_TRY(hipGetLastError());
     if (int rc = read_err(h)) return rc;
-    HIP_TRY(hipMemcpy(g, h->grad, sizeof(double) * GN, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(g, h->grad, sizeof(double) * GN * nr, hipMemcpyDeviceToHost));
     return BM_OK;
 }
 
-template <class R> static int download_t(bm_diff_handle h, bm_particles* out) {
+template <class R> static int download_t(bm_diff_handle h, int r, bm_particles* out) {
     const int64_t n = h->n;
     const int64_t last = (h->substeps + h->K - 1) / h->K;
     std::vector<R> st((size_t)(QCOMP * n));
-    HIP_TRY(hipMemcpy2D(st.data(), sizeof(R) * n, slot<R>(h->ck, h->cap, last).p, sizeof(R) * h->cap, sizeof(R) * n,
-                        QCOMP, hipMemcpyDeviceToHost));
+    const R* src = slot<R>(h, h->ck, last).p + (int64_t)r * QCOMP * h->cap;
+    HIP_TRY(hipMemcpy2D(st.data(), sizeof(R) * n, src, sizeof(R) * h->cap, sizeof(R) * n, QCOMP,
+                        hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i) {
         const int64_t o = h->order[(size_t)i];
         for (int k = 0; k < 3; ++k) {
@@ -1379,6 +1514,11 @@ template <class R> static int upload_t(bm_diff_handle h, int64_t n, const bm_par
     // vectors live until the synchronisation
     HIP_TRY(hipMemcpy2DAsync(h->init, sizeof(R) * h->cap, st.data(), sizeof(R) * n, sizeof(R) * n, QCOMP,
                              hipMemcpyHostToDevice, h->stream));
+    // every rollout starts from the same state
+    const size_t one = sizeof(R) * QCOMP * (size_t)h->cap;
+    for (int r = 1; r < h->nr; ++r)
+        HIP_TRY(hipMemcpyAsync(static_cast<char*>(h->init) + r * one, h->init, one, hipMemcpyDeviceToDevice,
+                               h->stream));
     HIP_TRY(hipMemcpyAsync(h->m, mm.data(), sizeof(R) * n, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->V0, vv.data(), sizeof(R) * n, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->x0, x0.data(), sizeof(R) * 3 * n, hipMemcpyHostToDevice, h->stream));
@@ -1398,8 +1538,13 @@ int bm_diff_create(const bm_diff_desc* desc, bm_diff_handle* out) {
     if (desc->max_particles <= 0) return fail(BM_ERR_INVALID_ARG, "max_particles must be > 0");
     if (desc->precision != 32 && desc->precision != 64) return fail(BM_ERR_INVALID_ARG, "precision must be 32 or 64");
     if (desc->checkpoint_every < 0) return fail(BM_ERR_INVALID_ARG, "checkpoint_every must be >= 0");
+    if (desc->rollouts < 0 || desc->rollouts > BM_DIFF_MAX_ROLLOUTS)
+        return fail(BM_ERR_INVALID_ARG, "rollouts must be in 0..%d", BM_DIFF_MAX_ROLLOUTS);
     auto* h = new bm_diff();
     h->desc = *desc;
+    h->nr = desc->rollouts > 0 ? desc->rollouts : 1;
+    h->prm.assign((size_t)h->nr, bm_diff_params{});
+    h->prm_set.assign((size_t)h->nr, 0);
     h->fp64 = desc->precision == 64;
     h->rs = h->fp64 ? 8 : 4;
     h->K = desc->checkpoint_every > 0 ? desc->checkpoint_every : 16;
@@ -1418,23 +1563,24 @@ int bm_diff_create(const bm_diff_desc* desc, bm_diff_handle* out) {
             return bail(fail(BM_ERR_HIP, "stream creation failed"));
         h->own_stream = true;
     }
-    const size_t r = h->rs;
+    const size_t r = h->rs, nr = (size_t)h->nr;
     int rc = BM_OK;
+    // the particles' m, V0, x(0) and target: one copy for every rollout
     if (!rc) rc = ensure(&h->m, r * h->cap);
     if (!rc) rc = ensure(&h->V0, r * h->cap);
     if (!rc) rc = ensure(&h->x0, r * 3 * h->cap);
     if (!rc) rc = ensure(&h->tgt, r * 3 * h->cap);
-    if (!rc) rc = ensure(&h->init, r * QCOMP * h->cap);
-    if (!rc) rc = ensure(&h->gm, r * h->nn);
-    if (!rc) rc = ensure(&h->gq, r * 3 * h->nn);
-    if (!rc) rc = ensure(&h->gu, r * 3 * h->nn);
-    if (!rc) rc = ensure(&h->gub, r * 3 * h->nn);
-    if (!rc) rc = ensure(&h->gqb, r * 3 * h->nn);
-    if (!rc) rc = ensure(&h->gmb, r * h->nn);
-    if (!rc) rc = ensure(&h->ks, r * 9 * h->cap);
-    if (!rc) rc = ensure(&h->abar, r * QCOMP * h->cap);
-    if (!rc) rc = ensure(&h->abar2, r * QCOMP * h->cap);
-    if (!rc) rc = ensure((void**)&h->grad, sizeof(double) * GN);
+    if (!rc) rc = ensure(&h->init, r * QCOMP * h->cap * nr);
+    if (!rc) rc = ensure(&h->gm, r * 4 * h->nn * nr);
+    if (!rc) rc = ensure(&h->gu, r * 3 * h->nn * nr);
+    if (!rc) rc = ensure(&h->gub, r * 3 * h->nn * nr);
+    if (!rc) rc = ensure(&h->gqb, r * 3 * h->nn * nr);
+    if (!rc) rc = ensure(&h->gmb, r * h->nn * nr);
+    if (!rc) rc = ensure(&h->ks, r * 9 * h->cap * nr);
+    if (!rc) rc = ensure(&h->abar, r * QCOMP * h->cap * nr);
+    if (!rc) rc = ensure(&h->abar2, r * QCOMP * h->cap * nr);
+    if (!rc) rc = ensure(&h->pard, sizeof(Par<double>) * nr);
+    if (!rc) rc = ensure((void**)&h->grad, sizeof(double) * GN * nr);
     if (!rc) rc = ensure((void**)&h->err, sizeof(int));
     if (rc) return bail(rc);
     if (hipMemset(h->err, 0, sizeof(int)) != hipSuccess) return bail(fail(BM_ERR_HIP, "diff initialisation"));
@@ -1445,8 +1591,8 @@ int bm_diff_create(const bm_diff_desc* desc, bm_diff_handle* out) {
 int bm_diff_destroy(bm_diff_handle h) {
     if (!h) return BM_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
-    for (void* p : {h->m, h->V0, h->x0, h->tgt, h->init, h->gm, h->gq, h->gu, h->gub, h->gqb, h->gmb, h->ck, h->seg, h->abar,
-                    h->abar2, h->sgmq, h->sgu, h->ks, h->agmq, h->agu, (void*)h->grad, (void*)h->err})
+    for (void* p : {h->m, h->V0, h->x0, h->tgt, h->init, h->gm, h->gu, h->gub, h->gqb, h->gmb, h->ck, h->seg, h->abar,
+                    h->abar2, h->sgmq, h->sgu, h->ks, h->agmq, h->agu, h->pard, (void*)h->grad, (void*)h->err})
         if (p) hipFree(p);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -1455,13 +1601,19 @@ int bm_diff_destroy(bm_diff_handle h) {
 
 int bm_diff_set_params(bm_diff_handle h, const bm_diff_params* p) {
     if (int rc = check_diff(h)) return rc;
-    if (!p) return fail(BM_ERR_INVALID_ARG, "null params");
-    if (!(p->young_modulus > 0.0) || !(p->poisson_ratio > -1.0 && p->poisson_ratio < 0.5))
-        return fail(BM_ERR_INVALID_ARG, "bad elastic constants");
-    if (p->has_tool && !(p->tool_half_extents[0] > 0 && p->tool_half_extents[1] > 0 && p->tool_half_extents[2] > 0))
-        return fail(BM_ERR_INVALID_ARG, "tool half extents must be > 0");
-    h->prm = *p;
-    h->has_prm = true;
+    if (int rc = check_params(p)) return rc;
+    h->prm.assign((size_t)h->nr, *p);
+    h->prm_set.assign((size_t)h->nr, 1);
+    h->have_forward = false;
+    return BM_OK;
+}
+
+int bm_diff_set_rollout_params(bm_diff_handle h, int32_t r, const bm_diff_params* p) {
+    if (int rc = check_diff(h)) return rc;
+    if (r < 0 || r >= h->nr) return fail(BM_ERR_INVALID_ARG, "rollout %d of %d", (int)r, h->nr);
+    if (int rc = check_params(p)) return rc;
+    h->prm[(size_t)r] = *p;
+    h->prm_set[(size_t)r] = 1;
     h->have_forward = false;
     return BM_OK;
 }
@@ -1493,38 +1645,42 @@ int bm_diff_upload(bm_diff_handle h, int64_t n, const bm_particles* p, const flo
     return h->fp64 ? upload_t<double>(h, n, p, target_displacement) : upload_t<float>(h, n, p, target_displacement);
 }
 
-int bm_diff_forward(bm_diff_handle h, float dt, int32_t substeps, double* loss) {
+int bm_diff_forward_batch(bm_diff_handle h, float dt, int32_t substeps, double* losses) {
     BM_TRACE("bm_diff_forward");
     if (int rc = check_diff(h)) return rc;
-    if (!h->has_prm) return fail(BM_ERR_STATE, "parameters not set");
+    if (!all_params_set(h)) return fail(BM_ERR_STATE, "parameters not set");
     if (h->n <= 0) return fail(BM_ERR_STATE, "no particles");
     if (!(dt > 0.0f) || substeps <= 0) return fail(BM_ERR_INVALID_ARG, "dt and substeps must be > 0");
     h->dt = dt;
-    return h->fp64 ? forward_t<double>(h, substeps, loss) : forward_t<float>(h, substeps, loss);
+    return h->fp64 ? forward_t<double>(h, substeps, losses) : forward_t<float>(h, substeps, losses);
 }
 
-int bm_diff_backward(bm_diff_handle h, bm_diff_grad* out) {
+int bm_diff_forward(bm_diff_handle h, float dt, int32_t substeps, double* loss) {
+    if (int rc = check_diff(h)) return rc;
+    std::vector<double> losses((size_t)h->nr);
+    const int rc = bm_diff_forward_batch(h, dt, substeps, losses.data());
+    if (rc == BM_OK && loss) *loss = losses[0];
+    return rc;
+}
+
+int bm_diff_backward_batch(bm_diff_handle h, bm_diff_grad* out) {
     BM_TRACE("bm_diff_backward");
     if (int rc = check_diff(h)) return rc;
     if (!out) return fail(BM_ERR_INVALID_ARG, "null output");
     if (!h->have_forward) return fail(BM_ERR_STATE, "backward needs a forward pass with the current parameters");
-    double g[GN];
-    if (int rc = h->fp64 ? backward_t<double>(h, g) : backward_t<float>(h, g)) return rc;
-    const bm_diff_params& q = h->prm;
-    const double E = q.young_modulus, nu = q.poisson_ratio;
-    const double mu = E / (2.0 * (1.0 + nu)), lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-    // koh = (3 lam + 2 mu) / (2 mu)
-    const double gmu = g[GMU] + g[GKOH] * (-3.0 * lam / (2.0 * mu * mu));
-    const double glam = g[GLAM] + g[GKOH] * (3.0 / (2.0 * mu));
-    const double phi = q.friction_deg * M_PI / 180.0, s = std::sin(phi);
-    const double dalpha_dphi_deg = std::sqrt(2.0 / 3.0) * 6.0 * std::cos(phi) / ((3.0 - s) * (3.0 - s)) * M_PI / 180.0;
-    for (int a = 0; a < 3; ++a) out->tool_velocity[a] = g[GVT + a];
-    out->friction_deg = g[GALPHA] * dalpha_dphi_deg;
-    out->young_modulus = gmu / (2.0 * (1.0 + nu)) + glam * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-    out->alpha = g[GALPHA];
-    out->mu = gmu;
-    out->lambda = glam;
+    std::vector<double> g((size_t)(GN * h->nr));
+    if (int rc = h->fp64 ? backward_t<double>(h, g.data()) : backward_t<float>(h, g.data())) return rc;
+    for (int r = 0; r < h->nr; ++r) grad_of(h->prm[(size_t)r], g.data() + (size_t)r * GN, out + r);
     return BM_OK;
+}
+
+int bm_diff_backward(bm_diff_handle h, bm_diff_grad* out) {
+    if (int rc = check_diff(h)) return rc;
+    if (!out) return fail(BM_ERR_INVALID_ARG, "null output");
+    std::vector<bm_diff_grad> all((size_t)h->nr);
+    const int rc = bm_diff_backward_batch(h, all.data());
+    if (rc == BM_OK) *out = all[0];
+    return rc;
 }
 
 int bm_diff_map_vjp(int32_t map, int64_t n, const double* F, const double* Gbar, const bm_diff_params* params,
@@ -1533,9 +1689,8 @@ int bm_diff_map_vjp(int32_t map, int64_t n, const double* F, const double* Gbar,
     if (precision != 32 && precision != 64) return fail(BM_ERR_INVALID_ARG, "precision must be 32 or 64");
     if (n < 0 || (n > 0 && (!F || !Gbar || !params || !G || !Fbar))) return fail(BM_ERR_INVALID_ARG, "null array");
     if (n == 0) return BM_OK;
-    bm_diff tmp;
-    tmp.prm = *params;
-    tmp.desc.dx = 1.0f;
+    bm_diff_desc unit{};
+    unit.dx = 1.0f;
     const size_t bytes = sizeof(double) * 9 * (size_t)n;
     double *dF = nullptr, *dGb = nullptr, *dG = nullptr, *dFb = nullptr;
     int rc = BM_OK;
@@ -1551,10 +1706,10 @@ int bm_diff_map_vjp(int32_t map, int64_t n, const double* F, const double* Gbar,
         const dim3 grid((unsigned)((n + 63) / 64));
         if (precision == 64)
             hipLaunchKernelGGL(k_map_vjp<double>, grid, dim3(64), 0, nullptr, map, n, (const double*)dF,
-                               (const double*)dGb, params_at<double>(&tmp, 0), dG, dFb);
+                               (const double*)dGb, par_of<double>(unit, *params), dG, dFb);
         else
             hipLaunchKernelGGL(k_map_vjp<float>, grid, dim3(64), 0, nullptr, map, n, (const double*)dF,
-                               (const double*)dGb, params_at<float>(&tmp, 0), dG, dFb);
+                               (const double*)dGb, par_of<float>(unit, *params), dG, dFb);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
             hipMemcpy(G, dG, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(Fbar, dFb, bytes, hipMemcpyDeviceToHost) != hipSuccess)
@@ -1567,12 +1722,15 @@ int bm_diff_map_vjp(int32_t map, int64_t n, const double* F, const double* Gbar,
     return rc;
 }
 
-int bm_diff_download(bm_diff_handle h, bm_particles* out) {
+int bm_diff_download_rollout(bm_diff_handle h, int32_t r, bm_particles* out) {
     if (int rc = check_diff(h)) return rc;
     if (!out) return fail(BM_ERR_INVALID_ARG, "null output");
+    if (r < 0 || r >= h->nr) return fail(BM_ERR_INVALID_ARG, "rollout %d of %d", (int)r, h->nr);
     if (!h->have_forward) return fail(BM_ERR_STATE, "no forward pass");
     HIP_TRY(hipStreamSynchronize(h->stream));
-    return h->fp64 ? download_t<double>(h, out) : download_t<float>(h, out);
+    return h->fp64 ? download_t<double>(h, r, out) : download_t<float>(h, r, out);
 }
+
+int bm_diff_download(bm_diff_handle h, bm_particles* out) { return bm_diff_download_rollout(h, 0, out); }
 
 }  // extern "C"

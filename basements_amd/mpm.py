@@ -195,7 +195,7 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
 class DiffDescC(C.Structure):
     _fields_ = [("dx", C.c_float), ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_int32 * 3),
                 ("max_particles", C.c_int64), ("precision", C.c_int32), ("checkpoint_every", C.c_int32),
-                ("reference_svd", C.c_int32), ("pad", C.c_int32), ("stream", C.c_void_p)]
+                ("reference_svd", C.c_int32), ("rollouts", C.c_int32), ("stream", C.c_void_p)]
 
 
 class DiffParamsC(C.Structure):
@@ -269,11 +269,15 @@ _SIGS = {
     "bm_diff_create": (C.c_int, [C.POINTER(DiffDescC), C.POINTER(C.c_void_p)]),
     "bm_diff_destroy": (C.c_int, [C.c_void_p]),
     "bm_diff_set_params": (C.c_int, [C.c_void_p, C.POINTER(DiffParamsC)]),
+    "bm_diff_set_rollout_params": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(DiffParamsC)]),
     "bm_diff_set_grid_budget": (C.c_int, [C.c_void_p, C.c_int64]),
     "bm_diff_upload": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(HostParticles), C.POINTER(C.c_float)]),
     "bm_diff_forward": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_double)]),
+    "bm_diff_forward_batch": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(C.c_double)]),
     "bm_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(DiffGradC)]),
+    "bm_diff_backward_batch": (C.c_int, [C.c_void_p, C.POINTER(DiffGradC)]),
     "bm_diff_download": (C.c_int, [C.c_void_p, C.POINTER(HostParticles)]),
+    "bm_diff_download_rollout": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(HostParticles)]),
     "bm_mpm_couple_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ColliderC), C.c_float, C.c_void_p]),
     "bm_diff_map_vjp": (C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(DiffParamsC), C.c_int32,
                                   C.c_void_p, C.c_void_p]),
@@ -961,22 +965,32 @@ class DiffParams:
         return p
 
 
+def _grad_dict(g: DiffGradC) -> dict:
+    return {"tool_velocity": np.array(g.tool_velocity[:]), "friction_deg": g.friction_deg,
+            "young_modulus": g.young_modulus, "alpha": g.alpha, "mu": g.mu, "lambda": g.lambda_}
+
+
 class Rollout:
-    """Differentiable rollout with a hand-written adjoint (bm_diff_*)."""
+    """Differentiable rollout with a hand-written adjoint (bm_diff_*).
+    ``rollouts`` > 1 runs that many rollouts of the same particles side by
+    side, each with its own parameters (set_params(p, r)); the *_all methods
+    return one result per rollout, the others rollout 0's."""
 
     def __init__(self, dx, origin, dims, max_particles, precision=32, checkpoint_every=16, reference_svd=False,
-                 stream=None):
+                 stream=None, rollouts=1):
         d = DiffDescC()
         d.dx = float(dx)
         d.grid_origin[:] = [int(v) for v in origin]
         d.grid_dims[:] = [int(v) for v in dims]
         d.max_particles, d.precision, d.checkpoint_every = int(max_particles), int(precision), int(checkpoint_every)
         d.reference_svd = 1 if reference_svd else 0
+        d.rollouts = int(rollouts)
         d.stream = stream
         h = C.c_void_p()
         _check(lib().bm_diff_create(C.byref(d), C.byref(h)))
         self.h = h
         self.n = 0
+        self.rollouts = max(1, int(rollouts))
 
     def close(self):
         if getattr(self, "h", None):
@@ -989,9 +1003,13 @@ class Rollout:
         except Exception:
             pass
 
-    def set_params(self, p: DiffParams):
+    def set_params(self, p: DiffParams, rollout=None):
+        """Parameters of every rollout, or of rollout ``rollout`` alone."""
         pc = p.to_c()
-        _check(lib().bm_diff_set_params(self.h, C.byref(pc)))
+        if rollout is None:
+            _check(lib().bm_diff_set_params(self.h, C.byref(pc)))
+        else:
+            _check(lib().bm_diff_set_rollout_params(self.h, int(rollout), C.byref(pc)))
 
     def set_grid_budget(self, nbytes: int):
         """Device bytes the forward may keep its substeps' grids in for the
@@ -1011,17 +1029,28 @@ class Rollout:
         _check(lib().bm_diff_forward(self.h, float(dt), int(substeps), C.byref(loss)))
         return loss.value
 
+    def forward_all(self, dt, substeps):
+        """Every rollout's loss."""
+        losses = (C.c_double * self.rollouts)()
+        _check(lib().bm_diff_forward_batch(self.h, float(dt), int(substeps), losses))
+        return np.array(losses[:])
+
     def backward(self):
         """dict of d loss / d parameter."""
         g = DiffGradC()
         _check(lib().bm_diff_backward(self.h, C.byref(g)))
-        return {"tool_velocity": np.array(g.tool_velocity[:]), "friction_deg": g.friction_deg,
-                "young_modulus": g.young_modulus, "alpha": g.alpha, "mu": g.mu, "lambda": g.lambda_}
+        return _grad_dict(g)
 
-    def download(self):
+    def backward_all(self):
+        """Every rollout's gradient dict."""
+        gs = (DiffGradC * self.rollouts)()
+        _check(lib().bm_diff_backward_batch(self.h, gs))
+        return [_grad_dict(g) for g in gs]
+
+    def download(self, rollout=0):
         out = _empty_particles(self.n)
         hp = _host_particles(out)[0]
-        _check(lib().bm_diff_download(self.h, C.byref(hp)))
+        _check(lib().bm_diff_download_rollout(self.h, int(rollout), C.byref(hp)))
         return out
 
 

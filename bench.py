@@ -89,6 +89,8 @@ def _args():
     ap.add_argument("--serial", type=int, default=1, help="--config batch/serve: also time the work one at a time")
     ap.add_argument("--requests", type=int, default=48, help="--config serve: requests per arrival rate")
     ap.add_argument("--fd-check", type=int, default=1, help="--config adjoint: FP64 central-difference check")
+    ap.add_argument("--rollouts", type=int, default=8,
+                    help="--config adjoint: rollouts of a parameter sweep run side by side in one handle (0 = skip)")
     ap.add_argument("--rates", default="5,200", help="--config serve: arrival rates (requests/s)")
     ap.add_argument("--serve-particles", type=int, default=16_000_000, help="--config serve: particle room")
     ap.add_argument("--serve-blocks", type=int, default=400_000, help="--config serve: grid-block room")
@@ -404,6 +406,7 @@ def bench_adjoint(a):
                  "adjoint_fp32": g32v, "rel_err_fp32_vs_fp64": rel32, "loss_fp64": loss64, "loss_fp32": loss32,
                  "fd_step": "1e-4 relative (tool speed for velocity components)"}
     value = n * T / (t_f + t_b)
+    batched = bench_adjoint_rollouts(a, sc, loss32, g32, value) if a.rollouts > 1 else None
     # HBM model: the forward substep's 200 B per particle (state read and
     # written once), the backward twice that (the recomputed forward state plus
     # the adjoint state read and written); the bound at this size is occupancy
@@ -419,9 +422,56 @@ def bench_adjoint(a):
            "dtype": "fp32", "data": "synthetic (seeded soil bed and target field, no external data)",
            "config": {"workload": "adjoint", "particles": n, "substeps": T, "dt": dt, "checkpoint_every": 16,
                       "grid_dims": list(sc["dims"]), "tool_velocity": list(sc["params"].tool_velocity)},
-           "forward_ms": t_f * 1e3, "backward_ms": t_b * 1e3, "roofline": roof, "check": check}
+           "forward_ms": t_f * 1e3, "backward_ms": t_b * 1e3, "roofline": roof, "check": check,
+           "batched_rollouts": batched}
     print(json.dumps(out))
     return 0
+
+
+def bench_adjoint_rollouts(a, sc, loss_single, g_single, single_rate):
+    """A parameter sweep as one handle of R rollouts (bm_diff_desc.rollouts):
+    rollout r has Young's modulus x (1 + r/R) and friction angle + r/2 deg,
+    rollout 0 the scene's own parameters.  Rate = particle-substeps of all
+    rollouts per second through forward + adjoint in FP32; check = rollout 0
+    against the single-rollout handle's loss and gradient."""
+    import dataclasses
+
+    from basements_amd import Rollout
+
+    nr = a.rollouts
+    n, T, dt = sc["particles"]["x"].shape[0], sc["substeps"], sc["dt"]
+    p0 = sc["params"]
+    B = Rollout(sc["dx"], sc["origin"], sc["dims"], n, precision=32, checkpoint_every=16, rollouts=nr)
+    for r in range(nr):
+        B.set_params(dataclasses.replace(p0, young_modulus=p0.young_modulus * (1 + r / nr),
+                                         friction_deg=p0.friction_deg + 0.5 * r), rollout=r)
+    B.upload(sc["particles"], sc["target"])
+    B.forward_all(dt, T)
+    B.backward_all()  # warm-up
+    reps = max(1, a.steps // 25)
+    t_f = t_b = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        losses = B.forward_all(dt, T)
+        t1 = time.perf_counter()
+        grads = B.backward_all()
+        t2 = time.perf_counter()
+        t_f += t1 - t0
+        t_b += t2 - t1
+    t_f /= reps
+    t_b /= reps
+    rate = nr * n * T / (t_f + t_b)
+    g0 = grads[0]
+    vscale = float(np.abs(g_single["tool_velocity"]).max())
+    return {"rollouts": nr, "value": rate, "unit": "particle-substeps/s", "forward_ms": t_f * 1e3,
+            "backward_ms": t_b * 1e3, "vs_single_rollout": rate / single_rate,
+            "sweep": "young_modulus x (1 + r/R), friction_deg + r/2",
+            "losses": [float(x) for x in losses],
+            "rollout0_rel_loss_vs_single": abs(losses[0] - loss_single) / abs(loss_single),
+            "rollout0_rel_grad_vs_single": {
+                "young_modulus": abs(g0["young_modulus"] - g_single["young_modulus"]) / abs(g_single["young_modulus"]),
+                "friction_deg": abs(g0["friction_deg"] - g_single["friction_deg"]) / abs(g_single["friction_deg"]),
+                "tool_velocity": float(np.abs(g0["tool_velocity"] - g_single["tool_velocity"]).max()) / vscale}}
 
 
 def run_batch(seed, n_scenes, steps, warmup, graph=True, tuning="", serial=True):

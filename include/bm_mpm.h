@@ -482,9 +482,13 @@ typedef struct {
     int32_t checkpoint_every;  /* substeps between saved states (0 = 16) */
     int32_t reference_svd;     /* 1: the reference's SVD stopping rule (for comparing forwards; the
                                   adjoint needs the default, tighter rule) */
-    int32_t pad;
+    int32_t rollouts;          /* rollouts run side by side, each with its own parameters and
+                                  the same particles and target (0 = 1; at most
+                                  BM_DIFF_MAX_ROLLOUTS): every launch covers all of them */
     void* stream;              /* hipStream_t; NULL = the handle creates its own */
 } bm_diff_desc;
+
+#define BM_DIFF_MAX_ROLLOUTS 4096
 
 typedef struct {
     double young_modulus, poisson_ratio;
@@ -509,7 +513,11 @@ typedef struct {
 
 int bm_diff_create(const bm_diff_desc* desc, bm_diff_handle* out);
 int bm_diff_destroy(bm_diff_handle h);
+/* Parameters of every rollout. */
 int bm_diff_set_params(bm_diff_handle h, const bm_diff_params* params);
+/* Parameters of rollout r alone (a parameter sweep or a population of
+ * candidates); a forward needs every rollout's. */
+int bm_diff_set_rollout_params(bm_diff_handle h, int32_t r, const bm_diff_params* params);
 /* Device memory the forward may spend keeping every substep's grid (m, q and
  * the updated velocity: 7 values per node and substep) for the backward,
  * which then recomputes only the particle states of each checkpoint segment
@@ -519,13 +527,22 @@ int bm_diff_set_params(bm_diff_handle h, const bm_diff_params* params);
 int bm_diff_set_grid_budget(bm_diff_handle h, int64_t bytes);
 /* Initial particle state and the target displacement (3n floats; NULL = 0). */
 int bm_diff_upload(bm_diff_handle h, int64_t n, const bm_particles* p, const float* target_displacement);
-/* Run `substeps` substeps, saving a state every checkpoint_every; the loss. */
+/* Run `substeps` substeps of every rollout, saving a state every
+ * checkpoint_every; the loss of rollout 0. */
 int bm_diff_forward(bm_diff_handle h, float dt, int32_t substeps, double* loss);
-/* Gradient of the last forward's loss; may be called again (the same
- * gradient) until the next forward or parameter change. */
+/* The same; losses[r] = rollout r's loss (one per rollout, or NULL). */
+int bm_diff_forward_batch(bm_diff_handle h, float dt, int32_t substeps, double* losses);
+/* Gradient of the last forward's loss of rollout 0; may be called again (the
+ * same gradient) until the next forward or parameter change.  The backward
+ * runs every rollout. */
 int bm_diff_backward(bm_diff_handle h, bm_diff_grad* grad);
-/* Final state of the last forward (x, v, F, C; m and V0 are not written). */
+/* The same; grads[r] = rollout r's gradient (one per rollout). */
+int bm_diff_backward_batch(bm_diff_handle h, bm_diff_grad* grads);
+/* Final state of the last forward's rollout 0 (x, v, F, C; m and V0 are not
+ * written). */
 int bm_diff_download(bm_diff_handle h, bm_particles* p);
+/* The same for rollout r. */
+int bm_diff_download_rollout(bm_diff_handle h, int32_t r, bm_particles* p);
 /* The rollout's two per-particle SVD maps and their hand-written
  * vector-Jacobian products on their own, for checking the adjoint against an
  * independent differentiation: map 0 = the Drucker-Prager return map

@@ -58,6 +58,17 @@ def test_argument_errors_without_device():
     d.mode = 5
     assert L.bm_mpm_create(C.byref(d), C.byref(h)) == 1
     assert L.bm_mpm_step(None, 1e-4) == 1
+    # a rollout count outside 0..BM_DIFF_MAX_ROLLOUTS is refused before any device call
+    from basements_amd.mpm import DiffDescC
+
+    dd = DiffDescC()
+    dd.dx = 0.01
+    dd.grid_dims[:] = [8, 8, 8]
+    dd.max_particles, dd.precision = 16, 32
+    for bad in (-1, 4097):
+        dd.rollouts = bad
+        assert L.bm_diff_create(C.byref(dd), C.byref(h)) == 1
+        assert b"rollouts" in L.bm_last_error()
     assert L.bm_mpm_destroy(None) == 0
 
 
@@ -80,7 +91,8 @@ int main(void) {
   S(bm_server_desc); F(bm_server_desc, hbm_budget); F(bm_server_desc, dt); F(bm_server_desc, max_bypass);
   S(bm_request); F(bm_request, n); F(bm_request, output); F(bm_request, substeps); F(bm_request, arrival);
   S(bm_request_info); F(bm_request_info, arrival); F(bm_request_info, body); F(bm_request_info, reaction);
-  S(bm_diff_desc); F(bm_diff_desc, max_particles); F(bm_diff_desc, reference_svd); F(bm_diff_desc, stream);
+  S(bm_diff_desc); F(bm_diff_desc, max_particles); F(bm_diff_desc, reference_svd); F(bm_diff_desc, rollouts);
+  F(bm_diff_desc, stream);
   S(bm_diff_params); F(bm_diff_params, has_tool); F(bm_diff_params, tool_center); F(bm_diff_params, tool_friction);
   S(bm_diff_grad); F(bm_diff_grad, young_modulus); F(bm_diff_grad, lambda);
   return 0;

@@ -1,0 +1,125 @@
+"""Several differentiable rollouts in one handle (bm_diff_desc.rollouts): each
+rollout, with parameters of its own, must give the loss, gradient and final
+state that the same rollout gives alone in a handle of one.
+
+The rollouts share every launch (one row of workgroups per rollout) and the
+particles' m, V0, x(0) and target; nothing else is shared, so at FP64 the
+results agree to the rounding of the deposits' atomics (the bar the
+checkpoint-interval test holds the gradient to, rel 1e-9).  At FP32 two
+forwards differ by the atomics' order; the bars are those of two FP32
+backwards over recomputed grids."""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SUBSTEPS = 40  # two whole segments and a partial one at checkpoint_every 16
+
+
+def _scene():
+    from basements_amd.scenes import tine_scene
+
+    return tine_scene(seed=3, cells=(12, 4, 12), velocity=0.3)
+
+
+def _variants(p0):
+    """Three parameter sets that move every gradient component: the base
+    case, stiffer soil with a larger friction angle, and a faster tool on a
+    slanted path."""
+    v = np.array(p0.tool_velocity)
+    return [
+        p0,
+        dataclasses.replace(p0, young_modulus=p0.young_modulus * 1.7, friction_deg=p0.friction_deg + 6.0),
+        dataclasses.replace(p0, tool_velocity=tuple(v * 1.4 + np.array([0.0, -0.05, 0.03]))),
+    ]
+
+
+def _handle(sc, precision, rollouts):
+    from basements_amd import Rollout
+
+    return Rollout(sc["dx"], sc["origin"], sc["dims"], sc["particles"]["x"].shape[0], precision=precision,
+                   rollouts=rollouts)
+
+
+def _alone(sc, prm, precision, budget):
+    R = _handle(sc, precision, 1)
+    if budget is not None:
+        R.set_grid_budget(budget)
+    R.set_params(prm)
+    R.upload(sc["particles"], sc["target"])
+    loss = R.forward(sc["dt"], SUBSTEPS)
+    return loss, R.backward(), R.download()
+
+
+def _close(a, b, rel):
+    for key in ("young_modulus", "friction_deg", "mu", "lambda", "alpha"):
+        assert a[key] == pytest.approx(b[key], rel=rel, abs=1e-300), key
+    scale = np.abs(b["tool_velocity"]).max()
+    assert scale > 0
+    assert np.abs(a["tool_velocity"] - b["tool_velocity"]).max() <= rel * scale
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("budget", [None, 0], ids=["kept_grids", "per_segment"])
+def test_each_rollout_matches_the_same_rollout_alone(precision, budget):
+    sc = _scene()
+    prms = _variants(sc["params"])
+    B = _handle(sc, precision, len(prms))
+    if budget is not None:
+        B.set_grid_budget(budget)
+    for r, p in enumerate(prms):
+        B.set_params(p, rollout=r)
+    B.upload(sc["particles"], sc["target"])
+    losses = B.forward_all(sc["dt"], SUBSTEPS)
+    grads = B.backward_all()
+    rel_loss, rel_grad, tol_x = (1e-10, 1e-9, 1e-6) if precision == 64 else (1e-4, 2e-2, 1e-4)
+    for r, p in enumerate(prms):
+        loss, g, fin = _alone(sc, p, precision, budget)
+        assert losses[r] == pytest.approx(loss, rel=rel_loss), r
+        _close(grads[r], g, rel_grad)
+        got = B.download(r)
+        for key in ("x", "v", "F"):
+            np.testing.assert_allclose(got[key], fin[key], rtol=0, atol=tol_x, err_msg=f"rollout {r} {key}")
+    # the parameter sets do move the results apart
+    assert len({round(float(x), 12) for x in losses}) == len(prms)
+    assert grads[1]["young_modulus"] != pytest.approx(grads[0]["young_modulus"], rel=1e-3, abs=0)
+
+
+def test_single_rollout_calls_report_rollout_zero():
+    sc = _scene()
+    prms = _variants(sc["params"])[:2]
+    B = _handle(sc, 64, 2)
+    B.set_params(prms[1])  # every rollout
+    B.set_params(prms[0], rollout=0)
+    B.upload(sc["particles"], sc["target"])
+    loss0 = B.forward(sc["dt"], SUBSTEPS)
+    g0 = B.backward()
+    all_g = B.backward_all()
+    _close(g0, all_g[0], 1e-9)
+    assert all_g[1]["young_modulus"] != pytest.approx(all_g[0]["young_modulus"], rel=1e-3, abs=0)
+    losses = B.forward_all(sc["dt"], SUBSTEPS)
+    assert loss0 == pytest.approx(losses[0], rel=1e-10)
+    assert losses[1] != pytest.approx(losses[0], rel=1e-6, abs=0)
+
+
+def test_rollout_argument_errors():
+    from basements_amd import BMError
+
+    sc = _scene()
+    B = _handle(sc, 32, 2)
+    B.upload(sc["particles"], sc["target"])
+    B.set_params(sc["params"], rollout=0)
+    with pytest.raises(BMError, match="STATE"):  # rollout 1 has no parameters yet
+        B.forward(sc["dt"], 4)
+    with pytest.raises(BMError, match="INVALID_ARG"):
+        B.set_params(sc["params"], rollout=2)
+    with pytest.raises(BMError, match="INVALID_ARG"):
+        B.set_params(sc["params"], rollout=-1)
+    B.set_params(sc["params"], rollout=1)
+    B.forward(sc["dt"], 4)
+    with pytest.raises(BMError, match="INVALID_ARG"):
+        B.download(2)
