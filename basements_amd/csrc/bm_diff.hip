@@ -303,21 +303,49 @@ template <class R> __device__ __forceinline__ void wave_add(double* dst, R v) {
     if ((threadIdx.x & 63) == 0 && x != 0.0) atomicAdd(dst, x);
 }
 
-// ------------------------------------------------------------------ scatter order
-// The deposit's lanes walk their 27 stencil nodes starting at node `rot`
-// (lane index mod 27): the particles of one cell share a stencil base and sit in
-// consecutive lanes, so with a common start they would add to the same LDS
-// address at every step, and same-address LDS atomics serialise.
-__device__ __forceinline__ void rotated_node(int t, int rot, int& i, int& j, int& k) {
-    int q = t + rot;
-    q = q >= 27 ? q - 27 : q;
-    i = q / 9;
-    j = (q / 3) % 3;
-    k = q % 3;
+// ------------------------------------------------------------------ lane runs
+// Upload groups a cell's particles into consecutive lanes, and lanes with one
+// stencil base walking the 27 nodes in one order add to the same node at each
+// step.  A run = the consecutive lanes of an aligned 8-lane group with the
+// same base: its lanes sum their values in registers (DPP row shifts, which
+// stay inside the 16-lane row that holds the group) and only the run's first
+// lane issues the atomic, so a wave's LDS atomics carry about one lane in
+// four instead of every lane.
+struct LaneRun {
+    bool head;  // the run's first lane (every dead lane is a run of its own)
+    int rem;    // lanes after this one in its run (0..7)
+};
+__device__ __forceinline__ LaneRun lane_run(bool live, const int b[3]) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int px = __shfl_up(b[0], 1), py = __shfl_up(b[1], 1), pz = __shfl_up(b[2], 1);
+    const bool plive = __shfl_up(live ? 1 : 0, 1) != 0;
+    LaneRun r;
+    r.head = !live || (lane & 7) == 0 || !plive || px != b[0] || py != b[1] || pz != b[2];
+    const unsigned long long cont = __ballot(!r.head);
+    const unsigned long long after = lane == 63 ? 0ull : cont >> (lane + 1);
+    r.rem = __builtin_ctzll(~after);
+    return r;
 }
-// a[i] for a run-time i without an indexed (scratch) access
-template <class R> __device__ __forceinline__ R pick3(const R (&a)[3], int i) {
-    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+// The value of lane + N of the same 16-lane row (DPP row_shl:N).
+template <int N> __device__ __forceinline__ float lane_above(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + N, 0xf, 0xf, true));
+}
+template <int N> __device__ __forceinline__ double lane_above(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u & 0xffffffffull), 0x100 + N, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x100 + N, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// Segmented suffix sum inside the run: the head gets the run's total
+// (runs are at most 8 lanes, so three doubling steps).
+template <class R> __device__ __forceinline__ R run_sum(R v, int rem) {
+    R t = lane_above<1>(v);
+    v += rem >= 1 ? t : R(0);
+    t = lane_above<2>(v);
+    v += rem >= 2 ? t : R(0);
+    t = lane_above<4>(v);
+    v += rem >= 4 ? t : R(0);
+    return v;
 }
 
 // ------------------------------------------------------------------ scatter tile
@@ -404,11 +432,12 @@ __global__ void __launch_bounds__(64) k_p2g(PS<R> s, const R* m, const R* V0, in
         stencil(x, P.inv_dx, st);
     }
     const TileBox tb = tile_box(live, st.b);
+    const LaneRun run = lane_run(live, st.b);
     if (tb.on) {
         for (int e = threadIdx.x; e < STILE; e += 64) tl[0][e] = tl[1][e] = tl[2][e] = tl[3][e] = R(0);
         __syncthreads();
     }
-    if (live) {
+    if (live) {  // a run's lanes are all live
         const v3<R> v = mk(s.at(QV, p), s.at(QV + 1, p), s.at(QV + 2, p));
         const m3<R> C = load_m(s, QC, p), F = load_m(s, QF, p);
         const R mp = m[p];
@@ -422,32 +451,37 @@ __global__ void __launch_bounds__(64) k_p2g(PS<R> s, const R* m, const R* V0, in
         }
 #pragma unroll
         for (int c = 0; c < 9; ++c) K.a[c] *= kf;
-        const int rot = (int)(threadIdx.x % 27u);
-        for (int t = 0; t < 27; ++t) {
-                int i, j, k;
-                rotated_node(t, rot, i, j, k);
-                const R w = pick3(st.w[0], i) * pick3(st.w[1], j) * pick3(st.w[2], k);
-                const int ni = st.b[0] + i, nj = st.b[1] + j, nk = st.b[2] + k;
-                const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
-                const v3<R> a = (v + mvec(C, d)) * mp + mvec(K, d) * dt;
-                if (tb.on) {
-                    const int e = tb.at(ni, nj, nk);
-                    atomicAdd(&tl[0][e], w * mp);
-                    atomicAdd(&tl[1][e], w * a.x);
-                    atomicAdd(&tl[2][e], w * a.y);
-                    atomicAdd(&tl[3][e], w * a.z);
-                    continue;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const R w = st.w[0][i] * st.w[1][j] * st.w[2][k];
+                    const int ni = st.b[0] + i, nj = st.b[1] + j, nk = st.b[2] + k;
+                    const v3<R> d = mk((R)ni * P.dx - x[0], (R)nj * P.dx - x[1], (R)nk * P.dx - x[2]);
+                    const v3<R> a = (v + mvec(C, d)) * mp + mvec(K, d) * dt;
+                    const R dm = run_sum(w * mp, run.rem), dqx = run_sum(w * a.x, run.rem),
+                            dqy = run_sum(w * a.y, run.rem), dqz = run_sum(w * a.z, run.rem);
+                    if (!run.head) continue;
+                    if (tb.on) {
+                        const int e = tb.at(ni, nj, nk);
+                        atomicAdd(&tl[0][e], dm);
+                        atomicAdd(&tl[1][e], dqx);
+                        atomicAdd(&tl[2][e], dqy);
+                        atomicAdd(&tl[3][e], dqz);
+                        continue;
+                    }
+                    const int64_t id = node_index<R>(b, ni, nj, nk);
+                    if (id < 0) {
+                        atomicOr(err, 1);
+                        continue;
+                    }
+                    atomicAdd(gm + id, dm);
+                    atomicAdd(gq + 3 * id + 0, dqx);
+                    atomicAdd(gq + 3 * id + 1, dqy);
+                    atomicAdd(gq + 3 * id + 2, dqz);
                 }
-                const int64_t id = node_index<R>(b, ni, nj, nk);
-                if (id < 0) {
-                    atomicOr(err, 1);
-                    continue;
-                }
-                atomicAdd(gm + id, w * mp);
-                atomicAdd(gq + 3 * id + 0, w * a.x);
-                atomicAdd(gq + 3 * id + 1, w * a.y);
-                atomicAdd(gq + 3 * id + 2, w * a.z);
-            }
     }
     if (tb.on) {
         __syncthreads();
@@ -714,11 +748,12 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
         base[2] = st.b[2];
     }
     const TileBox tb = tile_box(p < n, base);
+    const LaneRun run = lane_run(p < n, base);
     if (tb.on) {
         for (int e = threadIdx.x; e < STILE; e += 64) tl[0][e] = tl[1][e] = tl[2][e] = R(0);
         __syncthreads();
     }
-    if (p < n) {
+    if (p < n) {  // a run's lanes are all live
         const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
         const m3<R> F = load_m(s, QF, p);
         G2PFwd<R> f;
@@ -754,10 +789,13 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
         vb = vb + xb * dt;
         v3<R> xo = xb;
         const R kC = R(4) * P.inv_dx * P.inv_dx;
-        // a fixed node order here: the run-time rotation of the deposit costs
-        // this heavier loop more than its LDS collisions do
+        // one node order for every lane: a run's lanes add to the same node
+        // at each step (lane runs above)
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j)
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const R w = f.st.w[0][i] * f.st.w[1][j] * f.st.w[2][k];
                     const v3<R> gw = mk(f.st.dw[0][i] * f.st.w[1][j] * f.st.w[2][k],
@@ -771,15 +809,21 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
                     const v3<R> Cd = mvec(Cb, d);
                     // v' = sum w u;  C' = kC sum w u d^T
                     const v3<R> ub = (vb + Cd * kC) * w;
-                    if (tb.on) {
-                        const int e = tb.at(ni, nj, nk);
-                        atomicAdd(&tl[0][e], ub.x);
-                        atomicAdd(&tl[1][e], ub.y);
-                        atomicAdd(&tl[2][e], ub.z);
-                    } else {
-                        atomicAdd(gub + 3 * id + 0, ub.x);
-                        atomicAdd(gub + 3 * id + 1, ub.y);
-                        atomicAdd(gub + 3 * id + 2, ub.z);
+                    // (a run shares its base, so the whole run skipped an
+                    // out-of-box node together above)
+                    const R ubx = run_sum(ub.x, run.rem), uby = run_sum(ub.y, run.rem),
+                            ubz = run_sum(ub.z, run.rem);
+                    if (run.head) {
+                        if (tb.on) {
+                            const int e = tb.at(ni, nj, nk);
+                            atomicAdd(&tl[0][e], ubx);
+                            atomicAdd(&tl[1][e], uby);
+                            atomicAdd(&tl[2][e], ubz);
+                        } else {
+                            atomicAdd(gub + 3 * id + 0, ubx);
+                            atomicAdd(gub + 3 * id + 1, uby);
+                            atomicAdd(gub + 3 * id + 2, ubz);
+                        }
                     }
                     const R su = dot(u, vb) + kC * dot(u, Cd);
                     const v3<R> CTu = mk(Cb.a[0] * u.x + Cb.a[3] * u.y + Cb.a[6] * u.z,

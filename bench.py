@@ -91,6 +91,8 @@ def _args():
     ap.add_argument("--fd-check", type=int, default=1, help="--config adjoint: FP64 central-difference check")
     ap.add_argument("--rollouts", type=int, default=8,
                     help="--config adjoint: rollouts of a parameter sweep run side by side in one handle (0 = skip)")
+    ap.add_argument("--single-rollout", type=int, default=1,
+                    help="--config adjoint: 0 = time only the --rollouts sweep (a profile of the batched form)")
     ap.add_argument("--rates", default="5,200", help="--config serve: arrival rates (requests/s)")
     ap.add_argument("--serve-particles", type=int, default=16_000_000, help="--config serve: particle room")
     ap.add_argument("--serve-blocks", type=int, default=400_000, help="--config serve: grid-block room")
@@ -355,6 +357,18 @@ def bench_adjoint(a):
         R.upload(sc["particles"], sc["target"])
         return R
 
+    if not a.single_rollout:
+        b = bench_adjoint_rollouts(a, sc, None, None, None)
+        print(json.dumps({"metric": f"particle-substeps/s (forward + adjoint, FP32, checkpoint every 16, "
+                                    f"{b['rollouts']} rollouts per handle)", "value": b["value"],
+                          "unit": "particle-substeps/s", "n_gpus": 1, "steps": max(1, a.steps // 25), "warmup": 1,
+                          "ms_per_step": (b["forward_ms"] + b["backward_ms"]) / T, "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+                          "data": "synthetic (seeded soil bed and target field, no external data)",
+                          "config": {"workload": "adjoint", "particles": n, "substeps": T, "dt": dt,
+                                     "checkpoint_every": 16, "grid_dims": list(sc["dims"])},
+                          "batched_rollouts": b}))
+        return 0
     R32 = rollout(32)
     R32.forward(dt, T)
     R32.backward()  # warm-up
@@ -461,12 +475,14 @@ def bench_adjoint_rollouts(a, sc, loss_single, g_single, single_rate):
     t_f /= reps
     t_b /= reps
     rate = nr * n * T / (t_f + t_b)
+    out = {"rollouts": nr, "value": rate, "unit": "particle-substeps/s", "forward_ms": t_f * 1e3,
+           "backward_ms": t_b * 1e3, "vs_single_rollout": rate / single_rate if single_rate else None,
+           "sweep": "young_modulus x (1 + r/R), friction_deg + r/2", "losses": [float(x) for x in losses]}
+    if g_single is None:
+        return out
     g0 = grads[0]
     vscale = float(np.abs(g_single["tool_velocity"]).max())
-    return {"rollouts": nr, "value": rate, "unit": "particle-substeps/s", "forward_ms": t_f * 1e3,
-            "backward_ms": t_b * 1e3, "vs_single_rollout": rate / single_rate,
-            "sweep": "young_modulus x (1 + r/R), friction_deg + r/2",
-            "losses": [float(x) for x in losses],
+    return {**out,
             "rollout0_rel_loss_vs_single": abs(losses[0] - loss_single) / abs(loss_single),
             "rollout0_rel_grad_vs_single": {
                 "young_modulus": abs(g0["young_modulus"] - g_single["young_modulus"]) / abs(g_single["young_modulus"]),
