@@ -957,8 +957,11 @@ __global__ void __launch_bounds__(256) k_p2g_adj(PS<R> s, const R* m, const R* V
         m3<R> Cb, Kb;
 #pragma unroll
         for (int c = 0; c < 9; ++c) Cb.a[c] = Kb.a[c] = R(0);
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j)
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const R w = st.w[0][i] * st.w[1][j] * st.w[2][k];
                     const v3<R> gw = mk(st.dw[0][i] * st.w[1][j] * st.w[2][k],
