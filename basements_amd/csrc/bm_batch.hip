@@ -101,7 +101,7 @@ struct bm_batch {
     bool prepared = false, binned = false;
     float dt_prepared = 0.0f;
     int since_bin = 0;
-    int unit_p = 512, sort_every = 16, deposit_groups = 4096;
+    int unit_p = 640, sort_every = 16, deposit_groups = 4096;
     PinnedStage stage;  // host side of uploads and downloads
 
     // Captured substeps.  A replay starts and ends in the state it was
@@ -797,7 +797,7 @@ int bm_batch_autotune(bm_batch_handle b, float dt, int32_t substeps, int32_t* un
         float ms;
     };
     std::vector<Cand> cands;
-    for (int u : {384, 512})
+    for (int u : {512, 640})
         for (int so : {8, 16, 32})
             for (int g : {2048, 4096, 8192}) cands.push_back({u, so, g, 0.0f});
     hipEvent_t e0 = nullptr, e1 = nullptr;

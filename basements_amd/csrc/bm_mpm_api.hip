@@ -86,7 +86,7 @@ struct bm_mpm_solver {
     int since_bin = 0;        // substeps since the last re-binning
     bool next_cleared = false;  // the last grid update cleared the other grid for the next G2P
     int sort_every = 32;
-    int unit_p = 512;
+    int unit_p = 640;
     int variant = 4;          // fast-kernel variant (see bm_mpm_set_tuning): two-kernel form
     bool prepared = false;    // grid[prep] already holds the next substep's P2G
     float dt_prepared = 0.0f;
@@ -170,11 +170,14 @@ struct bm_mpm_solver {
 
 // Mass quantum: 2^(ilogb(max mass) - 22) kg, so a particle of the largest
 // mass is 2^22..2^23 - 1 quanta (its float mantissa exactly) and a node
-// deposit resolves 2^-23 of a particle's mass -- FP32 resolution.  Every
-// particle deposits at most 2^23 - 1 quanta, so the sum over any 512
-// particles fits in 32 bits: the fast path's LDS tiles (units of at most 512
-// particles) accumulate node mass in one 32-bit word.  Products w * q_p need
-// at most 24 + 23 bits, so the double-precision rounding of w * q_p is exact.
+// deposit resolves 2^-23 of a particle's mass -- FP32 resolution.  A particle
+// gives one node at most (27/64) q_p + 13 quanta (the largest weight is
+// (3/4)^3; the stencil centre's remainder exceeds its own share by at most
+// the other 26 nodes' roundings of 1/2), so the sum over any 1,024 particles
+// stays below 2^32: the fast path's LDS tiles (units of at most UNIT_MAX =
+// 1,024 particles) accumulate node mass in one 32-bit word.  Products
+// w * q_p need at most 24 + 23 bits, so the double-precision rounding of
+// w * q_p is exact.
 static int quantum_exp_for(float max_mass) {
     if (!(max_mass > 0.0f) || !std::isfinite(max_mass)) return 0;
     return std::ilogb(max_mass) - 22;

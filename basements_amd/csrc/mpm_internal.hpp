@@ -287,9 +287,13 @@ void launch_iota(int* p, int64_t n, int64_t first, hipStream_t s);
 // Slots [base, base + n) of P into insertion order in dst (same pitch):
 // dst[c * cap + pid[base + i]] = P[c * cap + base + i].
 void launch_unpermute(const PView& P, float* dst, const int* pid, int64_t base, int64_t n, hipStream_t s);
-// Largest unit the fast kernels accept: 512 particles of at most 2^23 - 1
-// mass quanta each keep every LDS node-mass sum below 2^32.
-constexpr int UNIT_MAX = 512;
+// Largest unit the fast kernels accept: a particle of at most 2^23 - 1 mass
+// quanta gives a node at most (27/64) q_p + 13 of them, so 1,024 particles
+// keep every LDS node-mass sum below 2^32 (1,213 would).  The default unit
+// (640) holds a whole 4^3 block of the 8-per-cell bed with the drift of a
+// re-binning period: blocks of 512 + a few particles split into two units
+// at 512, the second nearly empty but staging and flushing a tile of its own.
+constexpr int UNIT_MAX = 1024;
 // The fast substep's two kernels: per-particle G2P writing a 16-float
 // deposit record per particle into `coef`, then the per-cell block P2G.
 // `mark(ctx, stage, start, stop)` is called on the host right before each

@@ -29,7 +29,7 @@
 namespace bm {
 
 constexpr int NCOEF = 12;
-constexpr int SUNIT_MAX = 512;   // largest unit (see UNIT_MAX)
+constexpr int SUNIT_MAX = 1024;   // largest unit (see UNIT_MAX)
 
 __device__ __forceinline__ int stix(int ix, int iy, int iz) { return ((ix * TE + iy) * TE) + ((iz + 4 * (iy & 1)) & 7); }
 __device__ __forceinline__ void stile_coords(int k, int& ix, int& iy, int& iz) {
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domai
 // offset the 64 lanes address 64 different nodes, and program order across
 // offsets keeps neighbouring cells' shared nodes consistent.
 struct CellSmem {
-    float4 pt[TNODES];  // momentum xyz; w = mass quanta (uint, < 2^32 for <= 512 particles)
+    float4 pt[TNODES];  // momentum xyz; w = mass quanta (uint, < 2^32 for <= UNIT_MAX particles)
     unsigned short ord[SUNIT_MAX];
     unsigned short rank[SUNIT_MAX];
     unsigned char cell[SUNIT_MAX];
@@ -460,7 +460,7 @@ __device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, cons
         }
         if (CENTRE && !FULL) accm[NC] += (unsigned)((int)qp - rest);
     }
-    if (FULL) {  // modulo 2^32: the true sum is below 2^32 (at most 512 particles of < 2^23 quanta)
+    if (FULL) {  // modulo 2^32: the centre's true sum, a node sum, is below 2^32 (UNIT_MAX)
         unsigned others = 0u;
 #pragma unroll
         for (int n = 0; n < 27; ++n)
@@ -517,7 +517,7 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
     __syncthreads();
 
     // ---- count the particles per cell of the block (a unit holds at most
-    // SUNIT_MAX = 8 x 64 particles: all of a lane's positions are in flight at once)
+    // SUNIT_MAX = 16 x 64 particles: all of a lane's positions are in flight at once)
     float4 pos[SUNIT_MAX / 64];
 #pragma unroll
     for (int j = 0; j < SUNIT_MAX / 64; ++j) {

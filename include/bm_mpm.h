@@ -265,7 +265,7 @@ int bm_mpm_stats(bm_mpm_handle h, int64_t* active_blocks, int64_t* fp64_fallback
 
 /* ---------------------------------------------------------------- tuning */
 /* BM_MODE_FAST launch parameters (what the autotuner searches):
- * unit_particles: particles per workgroup unit (16..512, default 512);
+ * unit_particles: particles per workgroup unit (16..1024, default 640);
  * sort_every: substeps between particle re-binnings (>= 1, default 32);
  * variant: bit 2 = the two-kernel form (per-particle G2P writing deposit
  * records, then a per-block P2G accumulating each cell's particles in
@@ -364,7 +364,7 @@ int bm_batch_usage(bm_batch_handle b, int32_t* scenes, int64_t* particles, int64
  * deposit_groups = workgroups of the cell deposit, which walk the units the
  * device counted (default 4096).  Values <= 0 keep the current setting. */
 int bm_batch_set_tuning(bm_batch_handle b, int32_t unit_particles, int32_t sort_every, int32_t deposit_groups);
-/* Time each candidate (unit 384 / 512 x sort_every 8 / 16 / 32 x deposit
+/* Time each candidate (unit 512 / 640 x sort_every 8 / 16 / 32 x deposit
  * groups 2048 / 4096 / 8192) over at least `substeps` substeps (a whole
  * number of re-binning cycles), keep the fastest and report it.  The scenes'
  * state is restored afterwards; needs device memory for one copy of the

@@ -154,7 +154,7 @@ def test_autotune_keeps_the_scenes_state():
     A, aids = _batch(scenes)
     T, tids = _batch(scenes)
     unit, sort_every, groups = T.autotune(DT, substeps=8)
-    assert unit in (384, 512) and sort_every in (8, 16, 32) and groups in (2048, 4096, 8192)
+    assert unit in (512, 640) and sort_every in (8, 16, 32) and groups in (2048, 4096, 8192)
     A.step(DT, 10)
     T.step(DT, 10)
     for a, b in zip(aids, tids):
