@@ -119,6 +119,61 @@ def test_fast_matches_reference_mode_over_100_substeps():
     _assert_at_noise_floor(r.download(), o32.get_particles(), P64, fields="xF")
 
 
+@pytest.mark.parametrize("unit", [1024, 640, 64])
+def test_fast_units_up_to_the_largest_size(unit):
+    """A bed of 16 particles per cell (two interleaved jittered 8-per-cell
+    beds), so an interior 4^3 block holds 1,024 particles: with units of 1,024
+    (UNIT_MAX) each such block is one unit until the drift between
+    re-binnings (every 4 substeps) pushes it past that, at 640 it is two.
+    The staged G2P (forced) and the cell deposit then run the largest units
+    the kernels accept; the grid mass stays bit-exact every substep and the
+    state stays at the FP32 noise floor of the FP64 oracle."""
+    from basements_amd import MODE_FAST, Solver
+    from basements_amd.scenes import soil_block
+
+    lo, hi = (6, 0, 6), (14, 5, 14)  # stencil bases 0-3 in y fully populated: one block layer of 1,024
+    A, B = soil_block(DX, lo, hi, seed=21), soil_block(DX, lo, hi, seed=22)
+    P = {k: np.concatenate([A[k], B[k]]) for k in A}
+    n = P["x"].shape[0]
+    rng = np.random.default_rng(23)
+    P["v"] = (0.3 * rng.normal(size=(n, 3))).astype(np.float32)
+    P["F"] = (np.eye(3) + 0.01 * rng.normal(size=(n, 3, 3))).astype(np.float32)
+    P["C"] = (0.2 * rng.normal(size=(n, 3, 3))).astype(np.float32)
+    cell = np.floor(P["x"].astype(np.float64) / DX - 0.5).astype(np.int64)
+    blk, counts = np.unique(cell // 4, axis=0, return_counts=True)
+    assert counts.max() == 1024  # full interior blocks
+    prm = params_from()
+    g = Solver(DX, ORG, DIMS, n, MODE_FAST, sort_every=4)
+    g.set_tuning(unit, 0, 4 | 32)
+    g.set_material(prm.E, prm.nu, prm.rho)
+    g.set_plastic_alpha(prm.alpha)
+    g.set_floor_friction(prm.mu_floor)
+    g.set_gravity(prm.gravity)
+    g.upload(P)
+    o32, o64 = _fp64_pair(P, prm)
+    e = g.mass_quantum_exp()
+    q = int(np.rint(np.ldexp(P["m"].astype(np.float64), -e)).sum())
+    dt = 1e-4
+    for _ in range(12):
+        g.p2g(dt)
+        assert g.grid_mass_total() == q
+        g.grid_update(dt)
+        g.g2p(dt)
+    o32.step(dt, 12)
+    o64.step(dt, 12)
+    _assert_at_noise_floor(g.download(), o32.get_particles(), o64.get_particles())
+
+
+def test_unit_sizes_beyond_the_largest_are_refused():
+    from basements_amd import BMError, MODE_FAST, Solver
+
+    g = Solver(DX, ORG, DIMS, 64, MODE_FAST)
+    with pytest.raises(BMError, match="INVALID_ARG"):
+        g.set_tuning(1025, 0, -1)
+    with pytest.raises(BMError, match="INVALID_ARG"):
+        g.set_tuning(15, 0, -1)
+
+
 @pytest.mark.parametrize("variant", [4 | 16 | 32, 64])
 def test_unknown_variants_are_refused(variant):
     """Bits 4 and 5 name the two G2P forms; asking for both, or for a bit
