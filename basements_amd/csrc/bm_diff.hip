@@ -594,6 +594,36 @@ __global__ void __launch_bounds__(256) k_update(Box b, const Par<R>* Pv, double 
     gu[3 * id + 2] = f.u2.z;
 }
 
+// ------------------------------------------------------------------ kept factors
+// The SVD U diag(s) V^T of each particle's trial F at one substep, kept by
+// the backward's recompute of a segment for both adjoint kernels: the G2P
+// adjoint of that substep factors the same trial F, and the P2G adjoint of
+// the next substep factors the projected F = U diag(h(s)) V^T, whose SVD is
+// (U, h(s), V).  Component c of particle p at f[c * cap + p]; a rollout's
+// factors FCOMP * cap values after the previous rollout's.
+constexpr int FCOMP = 21;
+template <class R>
+__device__ __forceinline__ void store_fac(R* f, int64_t cap, int64_t p, const m3<R>& U, const v3<R>& s,
+                                          const m3<R>& V) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        f[c * cap + p] = U.a[c];
+        f[(12 + c) * cap + p] = V.a[c];
+    }
+    f[9 * cap + p] = s.x;
+    f[10 * cap + p] = s.y;
+    f[11 * cap + p] = s.z;
+}
+template <class R>
+__device__ __forceinline__ void load_fac(const R* f, int64_t cap, int64_t p, m3<R>& U, v3<R>& s, m3<R>& V) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        U.a[c] = f[c * cap + p];
+        V.a[c] = f[(12 + c) * cap + p];
+    }
+    s = mk(f[9 * cap + p], f[10 * cap + p], f[11 * cap + p]);
+}
+
 // G2P forward quantities of one particle (shared by the forward kernel and
 // the adjoint's recomputation).
 template <class R> struct G2PFwd {
@@ -607,7 +637,8 @@ template <class R> struct G2PFwd {
 };
 template <class R>
 __device__ __forceinline__ void g2p_forward(const R x[3], const m3<R>& F, const Box& b, const Par<R>& P, R dt,
-                                            const R* gu, G2PFwd<R>& o, int* err) {
+                                            const R* gu, G2PFwd<R>& o, int* err, const R* fac = nullptr,
+                                            int64_t cap = 0, int64_t p = 0) {
     stencil(x, P.inv_dx, o.st);
     const R kC = R(4) * P.inv_dx * P.inv_dx;
     o.vn = mk(R(0), R(0), R(0));
@@ -639,25 +670,33 @@ __device__ __forceinline__ void g2p_forward(const R x[3], const m3<R>& F, const 
     A.a[4] += R(1);
     A.a[8] += R(1);
     o.Ftr = mmul(A, F);
-    svd_d(o.Ftr, o.U, o.s, o.V, P.ref_svd);
+    if (fac)
+        load_fac(fac, cap, p, o.U, o.s, o.V);  // the factors the segment's recompute kept
+    else
+        svd_d(o.Ftr, o.U, o.s, o.V, P.ref_svd);
     o.dp = dp_h(o.s, P.alpha, P.koh);
     o.clamped = o.xn.y < R(0);
 }
 
+// fac (optional): the trial F's factors of this substep, kept for the
+// backward; out.p == nullptr: the factors alone (the segment's last substep,
+// whose new state is the next checkpoint, already stored).
 template <class R>
 __global__ void __launch_bounds__(256) k_g2p(PS<R> s, PS<R> out, int64_t n, Box b, const Par<R>* Pv, double t,
-                                             R dt, const R* gu, int* err, R* ks) {
+                                             R dt, const R* gu, int* err, R* ks, R* fac) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const Par<R> P = par_at(Pv, t);
     s = roll(s);
-    out = roll(out);
     gu += roll_off(3 * box_nodes(b));
-    if (ks) ks += roll_off(9 * out.cap);
     const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
     const m3<R> F = load_m(s, QF, p);
     G2PFwd<R> f;
     g2p_forward(x, F, b, P, dt, gu, f, err);
+    if (fac) store_fac(fac + roll_off(FCOMP * s.cap), s.cap, p, f.U, f.s, f.V);
+    if (!out.p) return;
+    out = roll(out);
+    if (ks) ks += roll_off(9 * out.cap);
     v3<R> xn = f.xn, vn = f.vn;
     if (f.clamped) {
         xn.y = R(0);
@@ -727,7 +766,7 @@ __global__ void __launch_bounds__(256) k_loss(PS<R> fin, const R* x0, const R* t
 template <class R>
 __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, int64_t n, Box b,
                                                 const Par<R>* Pv, double t, R dt, const R* gu, R* gub,
-                                                double* grad) {
+                                                double* grad, const R* fac) {
     __shared__ R tl[3][STILE];
     const Par<R> P = par_at(Pv, t);
     s = roll(s);
@@ -736,6 +775,7 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
     gu += roll_off(3 * box_nodes(b));
     gub += roll_off(3 * box_nodes(b));
     grad += roll_off(GN);
+    if (fac) fac += roll_off(FCOMP * s.cap);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     R galpha = R(0), gkoh = R(0);
     int base[3] = {0, 0, 0};
@@ -757,7 +797,7 @@ __global__ void __launch_bounds__(64) k_g2p_adj(PS<R> s, PS<R> ain, PS<R> aout, 
         const R x[3] = {s.at(QX, p), s.at(QX + 1, p), s.at(QX + 2, p)};
         const m3<R> F = load_m(s, QF, p);
         G2PFwd<R> f;
-        g2p_forward(x, F, b, P, dt, gu, f, (int*)nullptr);
+        g2p_forward(x, F, b, P, dt, gu, f, (int*)nullptr, fac, s.cap, p);
         v3<R> xb = mk(ain.at(QX, p), ain.at(QX + 1, p), ain.at(QX + 2, p));
         v3<R> vb = mk(ain.at(QV, p), ain.at(QV + 1, p), ain.at(QV + 2, p));
         m3<R> Cb = load_m(ain, QC, p);
@@ -925,10 +965,12 @@ __global__ void __launch_bounds__(256) k_update_adj(Box b, const Par<R>* Pv, dou
 
 // P2G: from the adjoint of (m, q) to the adjoint of x, v, C, F (added to
 // aout) and of mu and lambda.
+// facp (optional): the previous substep's kept factors (U, s, V of its trial
+// F): this state's F is U diag(h(s)) V^T, so its SVD is (U, h(s), V).
 template <class R>
 __global__ void __launch_bounds__(256) k_p2g_adj(PS<R> s, const R* m, const R* V0, int64_t n, Box b,
                                                  const Par<R>* Pv, double t, R dt, const R* gqb, const R* gmb,
-                                                 PS<R> aout, double* grad) {
+                                                 PS<R> aout, double* grad, const R* facp) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const Par<R> P = par_at(Pv, t);
     s = roll(s);
@@ -945,7 +987,13 @@ __global__ void __launch_bounds__(256) k_p2g_adj(PS<R> s, const R* m, const R* V
         const R kf = -R(4) * P.inv_dx * P.inv_dx * V0[p];
         m3<R> U, V;
         v3<R> sg;
-        svd_d(F, U, sg, V, P.ref_svd);
+        if (facp) {
+            v3<R> sp;
+            load_fac(facp + roll_off(FCOMP * s.cap), s.cap, p, U, sp, V);
+            sg = dp_h(sp, P.alpha, P.koh).h;
+        } else {
+            svd_d(F, U, sg, V, P.ref_svd);
+        }
         const Hh<R> H = hencky_h(sg, P.mu, P.lam);
         const m3<R> Pm = usv(U, H.h, V);
         m3<R> K = mmt(Pm, F);
@@ -1120,7 +1168,10 @@ struct bm_diff {
     // (unused with the reference's SVD stopping rule, whose comparisons
     // factor every F as the reference does)
     void* ks = nullptr;
-    int64_t ck_slots = 0, seg_slots = 0, grid_slots = 0;
+    // the backward's SVD factors of each substep of the segment it walks
+    // (FCOMP values per particle and substep)
+    void* fac = nullptr;
+    int64_t ck_slots = 0, seg_slots = 0, grid_slots = 0, fac_slots = 0;
     // device slot -> the caller's particle index (upload reorders, download restores)
     std::vector<int64_t> order;
     double* grad = nullptr;
@@ -1273,7 +1324,19 @@ template <class R> static void substep(bm_diff_handle h, PS<R> in, PS<R> out, in
     hipLaunchKernelGGL(k_update<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm, gq, gu,
                        h->all_grids ? 0 : 1);
     hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, in, out, h->n, h->box, P, t, dt,
-                       (const R*)gu, h->err, ks);
+                       (const R*)gu, h->err, ks, (R*)nullptr);
+}
+
+// Slot j of the backward's kept factors (every rollout's), allocated on first use.
+template <class R> static R* fac_slot(const bm_diff_handle h, int j) {
+    return static_cast<R*>(h->fac) + (int64_t)j * FCOMP * h->cap * h->nr;
+}
+static int ensure(void** p, size_t bytes);
+static int ensure_fac(bm_diff_handle h) {
+    if (h->fac_slots >= h->K) return BM_OK;
+    if (int rc = ensure(&h->fac, h->rs * FCOMP * (size_t)h->cap * h->nr * h->K)) return rc;
+    h->fac_slots = h->K;
+    return BM_OK;
 }
 
 static int ensure(void** p, size_t bytes) {
@@ -1393,6 +1456,7 @@ template <class R> static int backward_kept(bm_diff_handle h, double* g) {
     R* const gmq = static_cast<R*>(h->agmq);
     R* const gus = static_cast<R*>(h->agu);
     const Par<R>* const P = pars<R>(h);
+    if (int rc = ensure_fac(h)) return rc;
     HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN * nr, h->stream));
     seed_loss_adjoint<R>(h);
     hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * nn * nr)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn * nr);
@@ -1402,22 +1466,25 @@ template <class R> static int backward_kept(bm_diff_handle h, double* g) {
         const int s0 = sg * K, len = std::min(K, T - s0);
         hipMemcpyAsync(slot<R>(h, h->seg, 0).p, slot<R>(h, h->ck, sg).p, slot_bytes(h), hipMemcpyDeviceToDevice,
                        h->stream);
-        for (int j = 0; j + 1 < len; ++j)
+        // the segment's states and every substep's factors (the last
+        // substep's new state is the next checkpoint: factors only)
+        for (int j = 0; j < len; ++j)
             hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, slot<R>(h, h->seg, j),
-                               slot<R>(h, h->seg, j + 1), h->n, h->box, P, time_at(h, s0 + j), dt,
-                               (const R*)(gus + 3 * nn * nr * (s0 + j)), h->err, (R*)nullptr);
+                               j + 1 < len ? slot<R>(h, h->seg, j + 1) : PS<R>{nullptr, h->cap}, h->n, h->box, P,
+                               time_at(h, s0 + j), dt, (const R*)(gus + 3 * nn * nr * (s0 + j)), h->err, (R*)nullptr,
+                               fac_slot<R>(h, j));
         for (int j = len - 1; j >= 0; --j) {
             const PS<R> sj = slot<R>(h, h->seg, j);
             const double t = time_at(h, s0 + j);
             R* const gm = gmq + 4 * nn * nr * (s0 + j);
             R* const gu = gus + 3 * nn * nr * (s0 + j);
             hipLaunchKernelGGL(k_g2p_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, a, a2, h->n, h->box,
-                               P, t, dt, (const R*)gu, (R*)h->gub, h->grad);
+                               P, t, dt, (const R*)gu, (R*)h->gub, h->grad, (const R*)fac_slot<R>(h, j));
             hipLaunchKernelGGL(k_update_adj<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm,
                                gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad, 0);
             hipLaunchKernelGGL(k_p2g_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
                                (const R*)h->V0, h->n, h->box, P, t, dt, (const R*)h->gqb, (const R*)h->gmb, a2,
-                               h->grad);
+                               h->grad, j > 0 ? (const R*)fac_slot<R>(h, j - 1) : (const R*)nullptr);
             std::swap(a, a2);
         }
     }
@@ -1441,6 +1508,7 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
     R* const gmq = static_cast<R*>(h->sgmq);
     R* const gus = static_cast<R*>(h->sgu);
     const Par<R>* const P = pars<R>(h);
+    if (int rc = ensure_fac(h)) return rc;
     HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN * nr, h->stream));
     seed_loss_adjoint<R>(h);
     // the slots' (m, q) and the velocity adjoint start at zero; the update's
@@ -1452,8 +1520,8 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * K, len = std::min(K, T - s0);
         // recompute the segment from its checkpoint, keeping each substep's
-        // grid (m, q, u) for the reverse walk; the last substep's particle
-        // update is the next checkpoint, already stored
+        // grid (m, q, u) and factors for the reverse walk; the last
+        // substep's particle update is the next checkpoint, already stored
         hipMemcpyAsync(slot<R>(h, h->seg, 0).p, slot<R>(h, h->ck, sg).p, slot_bytes(h), hipMemcpyDeviceToDevice,
                        h->stream);
         seed_stress<R>(h, slot<R>(h, h->seg, 0));
@@ -1467,9 +1535,9 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
                                (const R*)ks);
             hipLaunchKernelGGL(k_update<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm, gm + nn,
                                gu, 0);
-            if (j + 1 < len)
-                hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, slot<R>(h, h->seg, j),
-                                   slot<R>(h, h->seg, j + 1), h->n, h->box, P, t, dt, (const R*)gu, h->err, ks);
+            hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, slot<R>(h, h->seg, j),
+                               j + 1 < len ? slot<R>(h, h->seg, j + 1) : PS<R>{nullptr, h->cap}, h->n, h->box, P, t,
+                               dt, (const R*)gu, h->err, ks, fac_slot<R>(h, j));
         }
         for (int j = len - 1; j >= 0; --j) {
             const PS<R> sj = slot<R>(h, h->seg, j);
@@ -1477,12 +1545,12 @@ template <class R> static int backward_t(bm_diff_handle h, double* g) {
             R* const gm = gmq + 4 * nn * nr * j;
             R* const gu = gus + 3 * nn * nr * j;
             hipLaunchKernelGGL(k_g2p_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, a, a2, h->n, h->box,
-                               P, t, dt, (const R*)gu, (R*)h->gub, h->grad);
+                               P, t, dt, (const R*)gu, (R*)h->gub, h->grad, (const R*)fac_slot<R>(h, j));
             hipLaunchKernelGGL(k_update_adj<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm,
                                gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad, 1);
             hipLaunchKernelGGL(k_p2g_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
                                (const R*)h->V0, h->n, h->box, P, t, dt, (const R*)h->gqb, (const R*)h->gmb, a2,
-                               h->grad);
+                               h->grad, j > 0 ? (const R*)fac_slot<R>(h, j - 1) : (const R*)nullptr);
             std::swap(a, a2);
         }
     }
@@ -1639,7 +1707,7 @@ int bm_diff_destroy(bm_diff_handle h) {
     if (!h) return BM_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : {h->m, h->V0, h->x0, h->tgt, h->init, h->gm, h->gu, h->gub, h->gqb, h->gmb, h->ck, h->seg, h->abar,
-                    h->abar2, h->sgmq, h->sgu, h->ks, h->agmq, h->agu, h->pard, (void*)h->grad, (void*)h->err})
+                    h->abar2, h->sgmq, h->sgu, h->ks, h->agmq, h->agu, h->pard, h->fac, (void*)h->grad, (void*)h->err})
         if (p) hipFree(p);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
