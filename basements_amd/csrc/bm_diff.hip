@@ -1146,6 +1146,13 @@ struct bm_diff {
     void* pard = nullptr;  // the rollouts' Par<R>, uploaded by each forward
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // the kept-grid backward recomputes the next segment on a second stream
+    // (default priority) while the current one is walked: a second set of
+    // segment states and factors, and events that order their reuse
+    hipStream_t side = nullptr;
+    hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_walk[2] = {nullptr, nullptr};
+    void *seg2 = nullptr, *fac2 = nullptr;
+    int64_t seg2_slots = 0;
     bool fp64 = false;
     int K = 16;
     Box box{};
@@ -1445,9 +1452,33 @@ template <class R> static void seed_loss_adjoint(bm_diff_handle h) {
                        (const R*)h->x0, (const R*)h->tgt, h->n, slot<R>(h, h->abar, 0), h->grad);
 }
 
+// The second segment buffer, stream and events of the overlapped recompute.
+static int ensure_side(bm_diff_handle h) {
+    if (!h->side) {
+        if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) {
+            h->side = nullptr;
+            return fail(BM_ERR_HIP, "stream creation failed");
+        }
+        for (int b = 0; b < 2; ++b)
+            if (hipEventCreateWithFlags(&h->ev_rec[b], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&h->ev_walk[b], hipEventDisableTiming) != hipSuccess)
+                return fail(BM_ERR_HIP, "event creation failed");
+    }
+    if (h->seg2_slots < h->K) {
+        h->seg2_slots = 0;
+        if (int rc = ensure(&h->seg2, slot_bytes(h) * h->K)) return rc;
+        if (int rc = ensure(&h->fac2, h->rs * FCOMP * (size_t)h->cap * h->nr * h->K)) return rc;
+        h->seg2_slots = h->K;
+    }
+    return BM_OK;
+}
+
 // The backward over the grids the forward kept: each segment's particle
-// states are recomputed from its checkpoint with the kept velocities, then
-// walked in reverse.
+// states (and every substep's SVD factors) are recomputed from its checkpoint
+// with the kept velocities, then walked in reverse.  The recompute of the
+// segment before runs on the side stream meanwhile, into the other of two
+// buffers: at one rollout a launch fills half a wave per SIMD, so the two
+// overlap.
 template <class R> static int backward_kept(bm_diff_handle h, double* g) {
     const int K = h->K;
     const int T = h->substeps;
@@ -1457,37 +1488,57 @@ template <class R> static int backward_kept(bm_diff_handle h, double* g) {
     R* const gus = static_cast<R*>(h->agu);
     const Par<R>* const P = pars<R>(h);
     if (int rc = ensure_fac(h)) return rc;
+    if (int rc = ensure_side(h)) return rc;
+    void* const segb[2] = {h->seg, h->seg2};
+    void* const facb[2] = {h->fac, h->fac2};
+    auto fac_of = [&](int b, int j) { return static_cast<R*>(facb[b]) + (int64_t)j * FCOMP * h->cap * nr; };
+    const R dt = (R)h->dt;
+    // the side stream follows everything queued on the handle's stream so far
+    HIP_TRY(hipEventRecord(h->ev_walk[0], h->stream));
+    HIP_TRY(hipStreamWaitEvent(h->side, h->ev_walk[0], 0));
+    auto recompute = [&](int sg, int b) {
+        const int s0 = sg * K, len = std::min(K, T - s0);
+        hipMemcpyAsync(slot<R>(h, segb[b], 0).p, slot<R>(h, h->ck, sg).p, slot_bytes(h), hipMemcpyDeviceToDevice,
+                       h->side);
+        // the last substep's new state is the next checkpoint: factors only
+        for (int j = 0; j < len; ++j)
+            hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->side, slot<R>(h, segb[b], j),
+                               j + 1 < len ? slot<R>(h, segb[b], j + 1) : PS<R>{nullptr, h->cap}, h->n, h->box, P,
+                               time_at(h, s0 + j), dt, (const R*)(gus + 3 * nn * nr * (s0 + j)), h->err, (R*)nullptr,
+                               fac_of(b, j));
+        hipEventRecord(h->ev_rec[b], h->side);
+    };
+    recompute(nseg - 1, 0);
     HIP_TRY(hipMemsetAsync(h->grad, 0, sizeof(double) * GN * nr, h->stream));
     seed_loss_adjoint<R>(h);
     hipLaunchKernelGGL(k_zero<R>, dim3(fill_blocks(3 * nn * nr)), dim3(256), 0, h->stream, (R*)h->gub, 3 * nn * nr);
     PS<R> a = slot<R>(h, h->abar, 0), a2 = slot<R>(h, h->abar2, 0);
-    const R dt = (R)h->dt;
-    for (int sg = nseg - 1; sg >= 0; --sg) {
+    for (int sg = nseg - 1, b = 0; sg >= 0; --sg, b ^= 1) {
+        if (sg > 0) {
+            // the other buffer holds the segment after this one: its walk (the
+            // previous iteration) must be done before the recompute overwrites it
+            if (sg < nseg - 1) hipStreamWaitEvent(h->side, h->ev_walk[b ^ 1], 0);
+            recompute(sg - 1, b ^ 1);
+        }
+        hipStreamWaitEvent(h->stream, h->ev_rec[b], 0);
         const int s0 = sg * K, len = std::min(K, T - s0);
-        hipMemcpyAsync(slot<R>(h, h->seg, 0).p, slot<R>(h, h->ck, sg).p, slot_bytes(h), hipMemcpyDeviceToDevice,
-                       h->stream);
-        // the segment's states and every substep's factors (the last
-        // substep's new state is the next checkpoint: factors only)
-        for (int j = 0; j < len; ++j)
-            hipLaunchKernelGGL(k_g2p<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, slot<R>(h, h->seg, j),
-                               j + 1 < len ? slot<R>(h, h->seg, j + 1) : PS<R>{nullptr, h->cap}, h->n, h->box, P,
-                               time_at(h, s0 + j), dt, (const R*)(gus + 3 * nn * nr * (s0 + j)), h->err, (R*)nullptr,
-                               fac_slot<R>(h, j));
         for (int j = len - 1; j >= 0; --j) {
-            const PS<R> sj = slot<R>(h, h->seg, j);
+            const PS<R> sj = slot<R>(h, segb[b], j);
             const double t = time_at(h, s0 + j);
             R* const gm = gmq + 4 * nn * nr * (s0 + j);
             R* const gu = gus + 3 * nn * nr * (s0 + j);
             hipLaunchKernelGGL(k_g2p_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, a, a2, h->n, h->box,
-                               P, t, dt, (const R*)gu, (R*)h->gub, h->grad, (const R*)fac_slot<R>(h, j));
+                               P, t, dt, (const R*)gu, (R*)h->gub, h->grad, (const R*)fac_of(b, j));
             hipLaunchKernelGGL(k_update_adj<R>, rows(h, nn, 256), dim3(256), 0, h->stream, h->box, P, t, dt, gm,
                                gm + nn, (R*)h->gub, (R*)h->gqb, (R*)h->gmb, h->grad, 0);
             hipLaunchKernelGGL(k_p2g_adj<R>, rows(h, h->n, PWG), dim3(PWG), 0, h->stream, sj, (const R*)h->m,
                                (const R*)h->V0, h->n, h->box, P, t, dt, (const R*)h->gqb, (const R*)h->gmb, a2,
-                               h->grad, j > 0 ? (const R*)fac_slot<R>(h, j - 1) : (const R*)nullptr);
+                               h->grad, j > 0 ? (const R*)fac_of(b, j - 1) : (const R*)nullptr);
             std::swap(a, a2);
         }
+        hipEventRecord(h->ev_walk[b], h->stream);
     }
+    // every recompute was waited for by a walk: the handle's stream covers the side stream
     HIP_TRY(hipGetLastError());
     if (int rc = read_err(h)) return rc;
     HIP_TRY(hipMemcpy(g, h->grad, sizeof(double) * GN * nr, hipMemcpyDeviceToHost));
@@ -1709,6 +1760,12 @@ int bm_diff_destroy(bm_diff_handle h) {
     for (void* p : {h->m, h->V0, h->x0, h->tgt, h->init, h->gm, h->gu, h->gub, h->gqb, h->gmb, h->ck, h->seg, h->abar,
                     h->abar2, h->sgmq, h->sgu, h->ks, h->agmq, h->agu, h->pard, h->fac, (void*)h->grad, (void*)h->err})
         if (p) hipFree(p);
+    if (h->side) hipStreamSynchronize(h->side);
+    for (void* p : {h->seg2, h->fac2})
+        if (p) hipFree(p);
+    for (hipEvent_t e : {h->ev_rec[0], h->ev_rec[1], h->ev_walk[0], h->ev_walk[1]})
+        if (e) hipEventDestroy(e);
+    if (h->side) hipStreamDestroy(h->side);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
     return BM_OK;
