@@ -63,6 +63,19 @@ __device__ __forceinline__ DepositRec load_rec(const float4* __restrict__ rec, i
     d.A = m3<float>{{r1.w, r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w}};
     return d;
 }
+// The same record with its first quad (position, quanta) from LDS copies.
+__device__ __forceinline__ DepositRec load_rec_lds(const float4* __restrict__ rec, int64_t i, const float* px,
+                                                   const float* py, const float* pz, const float* pq, int k) {
+    const float4 r1 = rec[rec_at(i, 1)], r2 = rec[rec_at(i, 2)], r3 = rec[rec_at(i, 3)];
+    DepositRec d;
+    d.x = px[k];
+    d.y = py[k];
+    d.z = pz[k];
+    d.qp = pq[k];
+    d.mvel = mk(r1.x, r1.y, r1.z);
+    d.A = m3<float>{{r1.w, r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w}};
+    return d;
+}
 __device__ __forceinline__ float quanta_of(float m, double qscale_inv) { return (float)rint((double)m * qscale_inv); }
 
 __global__ void k_set_zero(int* p) { *p = 0; }
@@ -374,8 +387,16 @@ __global__ void __launch_bounds__(64) k_advance_redo(PView P, GridBuf gin, Domai
 // LDS tile with one ordered read-modify-write chain.  For a fixed stencil
 // offset the 64 lanes address 64 different nodes, and program order across
 // offsets keeps neighbouring cells' shared nodes consistent.
+// Units of at most SPOS particles keep the positions and quanta the count
+// phase loaded for the accumulation (x, y, z in the tile's space, which the
+// chain only needs afterwards): one scattered 16-byte load per particle less.
+constexpr int SPOS = 640;
 struct CellSmem {
-    float4 pt[TNODES];  // momentum xyz; w = mass quanta (uint, < 2^32 for <= UNIT_MAX particles)
+    union {
+        float4 pt[TNODES];   // momentum xyz; w = mass quanta (uint, < 2^32 for <= UNIT_MAX particles)
+        float pos[3][SPOS];  // before the chain: the unit's particle positions
+    };
+    float pq[SPOS];
     unsigned short ord[SUNIT_MAX];
     unsigned short rank[SUNIT_MAX];
     unsigned char cell[SUNIT_MAX];
@@ -398,7 +419,7 @@ struct CellSmem {
 template <int A0, int A1>
 __device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, const int4& U, const unsigned short* ord,
                                             int s0, int mine, float dx, float inv_dx, v3<float>* acc,
-                                            unsigned* accm) {
+                                            unsigned* accm, const CellSmem* lp) {
 #pragma unroll
     for (int k = 0; k < (A1 - A0) * 9; ++k) {
         acc[k] = mk(0.f, 0.f, 0.f);
@@ -409,11 +430,16 @@ __device__ __forceinline__ void accum_slabs(const float4* __restrict__ rec, cons
     constexpr int NC = ((1 - A0) * 3 + 1) * 3 + 1;  // the centre's register slot (when CENTRE)
     unsigned qsum = 0u;                              // the cell's quanta (FULL)
     // the next particle's record is fetched while this one is deposited
+    auto fetch = [&](int k) {
+        const int pi = ord[s0 + k];
+        return lp ? load_rec_lds(rec, (int64_t)U.y + pi, lp->pos[0], lp->pos[1], lp->pos[2], lp->pq, pi)
+                  : load_rec(rec, (int64_t)U.y + pi);
+    };
     DepositRec nxt;
-    if (mine > 0) nxt = load_rec(rec, (int64_t)U.y + ord[s0]);
+    if (mine > 0) nxt = fetch(0);
     for (int k = 0; k < mine; ++k) {
         const DepositRec r = nxt;
-        if (k + 1 < mine) nxt = load_rec(rec, (int64_t)U.y + ord[s0 + k + 1]);
+        if (k + 1 < mine) nxt = fetch(k + 1);
         Stencil s;
         make_stencil(r.x, r.y, r.z, inv_dx, s);
         const v3<float>& mvel = r.mvel;
@@ -510,7 +536,10 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
     const int lane = threadIdx.x;
     const float dx = d.dx, inv_dx = d.inv_dx;
 
-    for (int k = lane; k < TNODES; k += 64) S.pt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // one pass with positions in LDS (they share the tile's space until the chain)
+    const bool lds_pos = NPASS == 1 && U.z <= SPOS;
+    if (!lds_pos)
+        for (int k = lane; k < TNODES; k += 64) S.pt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     S.cnt[lane] = 0;
     if (lane < 27) S.tb[lane] = 0;
     if (lane == 0) S.nleft = 0;
@@ -529,6 +558,12 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
         const int pi = lane + 64 * j;
         if (pi >= U.z) break;
         const float4 r0 = pos[j];
+        if (lds_pos) {
+            S.pos[0][pi] = r0.x;
+            S.pos[1][pi] = r0.y;
+            S.pos[2][pi] = r0.z;
+            S.pq[pi] = r0.w;
+        }
         Stencil s;
         make_stencil(r0.x, r0.y, r0.z, inv_dx, s);
         const int lx = s.b[0] - d.ox, ly = s.b[1] - d.oy, lz = s.b[2] - d.oz;
@@ -567,14 +602,19 @@ __device__ __forceinline__ void p2g_unit(int u, CellSmem& S, const float4* __res
     if (NPASS == 1) {
         v3<float> acc[27];
         unsigned accm[27];
-        accum_slabs<0, 3>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm);
+        accum_slabs<0, 3>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm, lds_pos ? &S : nullptr);
+        if (lds_pos) {  // every lane is done with the positions: the tile takes their space
+            __syncthreads();
+            for (int k = lane; k < TNODES; k += 64) S.pt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            __syncthreads();
+        }
         if (mine > 0) chain_slabs<0, 3>(S.pt, lane, acc, accm);
     } else {
         v3<float> acc[18];
         unsigned accm[18];
-        accum_slabs<0, 2>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm);
+        accum_slabs<0, 2>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm, nullptr);
         if (mine > 0) chain_slabs<0, 2>(S.pt, lane, acc, accm);
-        accum_slabs<2, 3>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm);
+        accum_slabs<2, 3>(rec, U, S.ord, s0, mine, dx, inv_dx, acc, accm, nullptr);
         if (mine > 0) chain_slabs<2, 3>(S.pt, lane, acc, accm);
     }
     __syncthreads();
